@@ -1,0 +1,162 @@
+/*
+ * multigraph_amd.h — C ABI of the MI355X (gfx950) frontier-processing engine.
+ *
+ * Every entry point returns an int status (MG_OK == 0, negative on error) and
+ * leaves a message for mg_last_error().  No torch or HIP types cross this
+ * boundary: graphs and engines are opaque handles, inputs and outputs are plain
+ * host pointers and sizes.  Vertex ids on this boundary are the reader's
+ * compacted ids (the i-th non-isolated vertex of the input, ascending original
+ * id), which is the id space the reference validates in
+ * (code/read_input.cu:115-139, code/validate.cu:310-316).
+ *
+ * Reference interface replaced by each group of entry points:
+ *   graph input ............ generate_CSC            code/read_input.cu:11-254, code/common.h:215
+ *   layout build ........... generate_MultiGraph(S)  code/gen_structure.cu:2230-2947, :99-139
+ *   release ................ release_MultiGraph      code/release_structure.cu:4-19
+ *   init / reset ........... initValue / resetValue  code/user_function.h:3-208
+ *   one engine step ........ MultiGraph_V_E_V, _NOTIDEM, _E_E, _V_V  code/common.h:228-233
+ *   BC backward ............ MultiGraph_V_E_V_pull   code/process2.cu:263-325
+ *   results ................ struct OUTPUT r/r2/r3   code/common.h:205-213
+ */
+#ifndef MULTIGRAPH_AMD_H
+#define MULTIGRAPH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define MG_OK            0
+#define MG_ERR_ARG      -1   /* bad argument (null pointer, id out of range, ...) */
+#define MG_ERR_IO       -2   /* file missing or malformed                         */
+#define MG_ERR_RANGE    -3   /* vertex id in the input outside [1, nv]            */
+#define MG_ERR_NOMEM    -4   /* host or device allocation failed                  */
+#define MG_ERR_DEVICE   -5   /* HIP runtime error (no device, launch failure)     */
+#define MG_ERR_LIMIT    -6   /* a size limit of this build was exceeded           */
+#define MG_ERR_STATE    -7   /* call out of order (e.g. step before reset)        */
+
+/* ---- algorithms (the reference's -D{BFS,SSSP,BC,CC,PR_T,PR_D}, attribute.h) */
+#define MG_BFS   0
+#define MG_SSSP  1
+#define MG_BC    2
+#define MG_CC    3
+#define MG_PR_T  4   /* DATA_DRIVEN_PR     (install.sh:23) */
+#define MG_PR_D  5   /* TOPOLOGY_DRIVEN_PR (install.sh:24) */
+
+/* ---- reader flags (compile-time -DSYM / -DNSYM / -DCC of the reference) ---- */
+#define MG_READ_SYM      1   /* add (v,u,w) for every (u,v,w): read_input.cu:88-95 (-DSYM) */
+#define MG_READ_CC       2   /* keep only src < dst edges:     read_input.cu:111-113     */
+#define MG_READ_HEADER   4   /* honour the MTX header's symmetric flag instead of forcing */
+
+/* ---- engine modes (the reference's -DSPARSE_MODE / -DDENSE_MODE / hybrid) - */
+#define MG_MODE_HYBRID   0
+#define MG_MODE_SPARSE   1   /* queue push only                    */
+#define MG_MODE_DENSE    2   /* bitmap pull / partitioned sweeps   */
+
+/* ---- result selectors (struct OUTPUT r, r2, r3) ------------------------------ */
+#define MG_RES_LEVEL     0   /* int32: BFS/BC level, SSSP distance, CC label (min id in component) */
+#define MG_RES_SIGMA     1   /* double: BC shortest-path counts                                  */
+#define MG_RES_DELTA     2   /* float : BC dependency                                            */
+#define MG_RES_RANK      3   /* float : PageRank                                                 */
+
+#define MG_INF 0x64646464    /* unreached level / distance (common.h:37-39,55-56) */
+
+typedef struct mg_graph  mg_graph;    /* host CSR after reading   (struct csc_package) */
+typedef struct mg_engine mg_engine;   /* device layout + run state (struct HYB + vector_data) */
+
+/* Tunables of one engine.  Zero fields take the defaults; the autotuner fills them. */
+typedef struct mg_options {
+    int    mode;            /* MG_MODE_*                                                */
+    float  alpha;           /* push->pull when frontier edges > unexplored edges / alpha */
+    float  beta;            /* pull->push when frontier vertices < nv / beta            */
+    int    huge_degree;     /* vertices at or above this degree are split across blocks */
+    int    grid_blocks;     /* workgroups of the grid-stride frontier kernels           */
+    int    graph_levels;    /* levels captured per hipGraph launch (0: eager, host step)*/
+    float  pr_damping;      /* PageRank damping (0.85, common.h:59)                     */
+    float  pr_tolerance;    /* PageRank tolerance (0.01, common.h:60)                   */
+    int    pr_max_iter;     /* PageRank iteration cap (0: unbounded, as the reference)  */
+    int    part_log2;       /* log2 of the vertex-partition size of the tiled layout    */
+} mg_options;
+
+/* ---- errors ---------------------------------------------------------------- */
+const char *mg_last_error(void);
+const char *mg_version(void);
+
+/* ---- graph input (host) ---------------------------------------------------- */
+/* generate_CSC (read_input.cu:11-254): read a Matrix Market file, symmetrize
+ * (flags & MG_READ_SYM), sort, drop duplicates, compact isolated vertices, build
+ * CSR and remap `source` into the compacted id space.  Pattern files get weights
+ * 1 + splitmix64(weight_seed, line) % 64 (the reference uses time-seeded rand()). */
+int mg_graph_load_mtx(const char *path, int source, int flags, uint64_t weight_seed,
+                      mg_graph **out);
+/* Same semantics for an in-memory list of 0-based entries (one per MTX line). */
+int mg_graph_from_coo(int nv, int64_t nnz, const int32_t *src, const int32_t *dst,
+                      const int32_t *val, int source, int flags, mg_graph **out);
+/* RMAT (a,b,c, d = 1-a-b-c), 2^scale vertices, edge_factor * 2^scale draws, self-loops
+ * dropped, weights 1 + hash(weight_seed, min(u,v), max(u,v)) % 64; then read as above. */
+int mg_graph_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                      uint64_t seed, uint64_t weight_seed, int flags, mg_graph **out);
+/* 2-D 4-neighbour grid, width x height, id = y*width + x. */
+int mg_graph_gen_grid(int width, int height, uint64_t weight_seed, int flags, mg_graph **out);
+/* Disjoint union of graphs (vertex ids offset in order); used for uneven batches. */
+int mg_graph_union(int count, const mg_graph *const *graphs, mg_graph **out);
+int mg_graph_write_mtx(const mg_graph *g, const char *path);
+int mg_graph_info(const mg_graph *g, int32_t *nv, int64_t *ne, int32_t *start_p);
+int mg_graph_csr(const mg_graph *g, const int32_t **row, const int32_t **col,
+                 const int32_t **weight);
+/* Original (pre-compaction) id of every compacted vertex, length nv. */
+int mg_graph_orig_ids(const mg_graph *g, const int32_t **orig);
+/* Map an original vertex id to the compacted id exactly as read_input.cu:130 does. */
+int mg_graph_remap(const mg_graph *g, int orig_id, int32_t *compact_id);
+int mg_graph_max_degree_vertex(const mg_graph *g, int32_t *v);
+void mg_graph_free(mg_graph *g);
+
+/* ---- device engine --------------------------------------------------------- */
+void mg_default_options(mg_options *opt);
+int mg_device_count(int *n);
+/* generate_MultiGraph / generate_MultiGraphS: upload the CSR, build the
+ * algorithm's device layout on the GPU, allocate run state (initValue). */
+int mg_build(const mg_graph *g, int algo, const mg_options *opt, mg_engine **out);
+int mg_release(mg_engine *e);
+/* resetValue: seed the run from `source` (compacted id; ignored for CC and PR). */
+int mg_reset(mg_engine *e, int source);
+/* One engine call (MultiGraph_V_E_V / _NOTIDEM / _E_E / _V_V): returns 1 when it
+ * must be called again, 0 when the run is finished, < 0 on error.  *iter is the
+ * level / iteration number and is advanced by the callee, as in the reference. */
+int mg_step(mg_engine *e, int *iter);
+/* BC dependency pass over the logged levels (MultiGraph_V_E_V_pull). */
+int mg_bc_backward(mg_engine *e, int iter);
+/* reset + all steps (+ backward for BC), timed with HIP events on the engine's
+ * stream; uses captured hipGraphs when opt.graph_levels > 0. */
+int mg_run(mg_engine *e, int source, float *ms);
+int mg_sync(mg_engine *e);
+int mg_iterations(const mg_engine *e, int *iters);
+/* Copy a result (MG_RES_*) into host memory, indexed by compacted id. */
+int mg_get_result(mg_engine *e, int which, void *host_dst, int64_t bytes);
+
+/* ---- batched multi-source runs (one bit per source packed per vertex) ------ */
+#define MG_BATCH_BFS  0
+#define MG_BATCH_BC   1
+#define MG_BATCH_SSSP 2
+typedef struct mg_batch mg_batch;
+int mg_batch_build(const mg_graph *g, int kind, int max_sources, const mg_options *opt,
+                   mg_batch **out);
+int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float *ms);
+/* levels / distances: int32 [nsrc][nv] (source-major); bc: float [nv] summed over sources */
+int mg_batch_get(mg_batch *b, int which, void *host_dst, int64_t bytes);
+int mg_batch_release(mg_batch *b);
+
+/* ---- instrumentation -------------------------------------------------------- */
+/* Per-kernel accounting of the last mg_run: launches, summed device time (ms,
+ * from HIP events around every launch on the engine stream when profiling is on)
+ * and algorithmic bytes.  name is the kernel's function name. */
+int mg_set_profiling(mg_engine *e, int on);
+int mg_kernel_stats(mg_engine *e, int idx, char *name, int name_len, int64_t *launches,
+                    double *ms, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MULTIGRAPH_AMD_H */

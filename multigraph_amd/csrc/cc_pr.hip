@@ -1,0 +1,469 @@
+// Connected components (the role of MultiGraph_E_E + MultiGraph_V_V, process.cu:940-1077)
+// and PageRank (the role of MultiGraph_V_E_V_NOTIDEM, processi.cu) for gfx950.
+#include "engine.hpp"
+#include "user_function.hpp"
+
+using namespace mg;
+
+namespace {
+
+int grid_for(long long items, int per_block, int cap) {
+    long long g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// ------------------------------------------------------------------ scans
+// Exclusive scan of n int counts into long long offsets (out has n+1 entries).
+constexpr int kScanBlock = 256, kScanItems = 4;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(const int *__restrict__ in, long long n, long long *__restrict__ out,
+                                                            long long *__restrict__ bsum) {
+    __shared__ long long tmp[kScanBlock / MG_WAVE + 1];
+    const long long base = (long long)blockIdx.x * kScanBlock * kScanItems + (long long)threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    long long local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? (long long)in[base + k] : 0;
+        local += v[k];
+    }
+    long long total;
+    long long pre = block_excl_scan<kScanBlock / MG_WAVE>(local, tmp, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = pre;
+        pre += v[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks64(const long long *__restrict__ in, long long n,
+                                                              long long *__restrict__ out, long long *__restrict__ bsum) {
+    __shared__ long long tmp[kScanBlock / MG_WAVE + 1];
+    const long long base = (long long)blockIdx.x * kScanBlock * kScanItems + (long long)threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    long long local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        local += v[k];
+    }
+    long long total;
+    long long pre = block_excl_scan<kScanBlock / MG_WAVE>(local, tmp, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = pre;
+        pre += v[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(long long *__restrict__ out, long long n, const long long *__restrict__ boff) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += boff[i / (kScanBlock * kScanItems)];
+}
+
+__global__ void k_set_last(long long *__restrict__ out, long long n, const long long *__restrict__ bsum_scanned,
+                           long long nb, const long long *__restrict__ bsum) {
+    out[n] = bsum_scanned[nb - 1] + bsum[nb - 1];
+}
+
+}  // namespace
+
+// out[0..n] = exclusive scan of in[0..n) (out[n] = total); on the engine stream.
+int mg_scan64(hipStream_t s, const long long *in64, const int *in32, long long n, long long *out) {
+    const long long per = (long long)kScanBlock * kScanItems;
+    const long long nb = (n + per - 1) / per;
+    if (n == 0) {
+        long long z = 0;
+        MG_HIP(hipMemcpyAsync(out, &z, sizeof(long long), hipMemcpyHostToDevice, s));
+        MG_HIP(hipStreamSynchronize(s));
+        return MG_OK;
+    }
+    long long *bsum = nullptr, *boff = nullptr;
+    MG_HIP(hipMalloc(&bsum, sizeof(long long) * nb));
+    MG_HIP(hipMalloc(&boff, sizeof(long long) * (nb + 1)));
+    if (in32) hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, s, in32, n, out, bsum);
+    else hipLaunchKernelGGL(k_scan_blocks64, dim3(nb), dim3(kScanBlock), 0, s, in64, n, out, bsum);
+    int rc = MG_OK;
+    if (nb > 1) rc = mg_scan64(s, bsum, nullptr, nb, boff);
+    else { long long z = 0; hipMemcpyAsync(boff, &z, sizeof(long long), hipMemcpyHostToDevice, s); }
+    if (rc == MG_OK) {
+        if (nb > 1) hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256), dim3(256), 0, s, out, n, boff);
+        hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, out, n, boff, nb, bsum);
+    }
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(s);
+    hipFree(bsum);
+    hipFree(boff);
+    if (rc != MG_OK) return rc;
+    if (le != hipSuccess || se != hipSuccess) return mg_fail(MG_ERR_DEVICE, "scan: %s", hipGetErrorString(le != hipSuccess ? le : se));
+    return MG_OK;
+}
+
+namespace {
+
+// ------------------------------------------------------------------ CC
+// Upper-triangular CSR (edges u < v), the role of the reference's src<dst COO
+// (read_input.cu:111-113, gen_structure.cu:34-139).
+__global__ void k_cc_count(int nv, const int *__restrict__ row, const int *__restrict__ col, int *__restrict__ cnt) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int u = wave; u < nv; u += nw) {
+        int c = 0;
+        for (int p = row[u] + lane(); p < row[u + 1]; p += MG_WAVE) c += col[p] > u;
+        c = wave_sum(c);
+        if (lane() == 0) cnt[u] = c;
+    }
+}
+
+__global__ void k_cc_fill(int nv, const int *__restrict__ row, const int *__restrict__ col, const long long *__restrict__ uoff,
+                          int *__restrict__ urow, int *__restrict__ ucol) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int u = wave; u < nv; u += nw) {
+        long long o = uoff[u];
+        if (lane() == 0) urow[u] = (int)o;
+        for (int p0 = row[u]; p0 < row[u + 1]; p0 += MG_WAVE) {
+            const int p = p0 + lane();
+            const bool keep = p < row[u + 1] && col[p] > u;
+            const uint64_t m = ballot(keep);
+            if (keep) ucol[o + mbcnt(m)] = col[p];
+            o += __builtin_popcountll(m);
+        }
+        if (u == nv - 1 && lane() == 0) urow[nv] = (int)uoff[nv];
+    }
+}
+
+__global__ void k_cc_init(int nv, int *__restrict__ parent) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) parent[v] = v;
+}
+
+// hook every edge: block-scan load balance over 256 vertices, split vertices above
+// `huge` edges into chunks handled by k_cc_hook_huge
+__global__ __launch_bounds__(kBlock) void k_cc_hook(int nv, int huge, const int *__restrict__ urow,
+                                                    const int *__restrict__ ucol, int *__restrict__ parent) {
+    __shared__ int s_off[kBlock], s_rb[kBlock], s_u[kBlock];
+    __shared__ int s_tmp[kBlock / MG_WAVE + 1];
+    for (int base = blockIdx.x * kBlock; base < nv; base += gridDim.x * kBlock) {
+        const int u = base + threadIdx.x;
+        int rb = 0, deg = 0;
+        if (u < nv) { rb = urow[u]; deg = urow[u + 1] - rb; if (deg >= huge) deg = 0; }
+        int total;
+        int off = block_excl_scan<kBlock / MG_WAVE>(deg, s_tmp, &total);
+        s_off[threadIdx.x] = off; s_rb[threadIdx.x] = rb; s_u[threadIdx.x] = u;
+        __syncthreads();
+        for (int e0 = 0; e0 < total; e0 += kBlock) {
+            const int e = e0 + threadIdx.x;
+            if (e < total) {
+                int lo = 0, hi = kBlock - 1;
+                while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (s_off[mid] <= e) lo = mid; else hi = mid - 1; }
+                const int p = s_rb[lo] + (e - s_off[lo]);
+                CCHooks::fused_update_condition(parent, s_u[lo], ld_stream(&ucol[p]));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cc_hook_huge(const HugeChunk *__restrict__ chunks, int nch,
+                                                         const int *__restrict__ urow, const int *__restrict__ ucol,
+                                                         int *__restrict__ parent) {
+    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+        const HugeChunk hc = chunks[c];
+        const int b = urow[hc.v] + hc.j * kHugeChunk, e = min(urow[hc.v + 1], b + kHugeChunk);
+        for (int p = b + threadIdx.x; p < e; p += kBlock) CCHooks::fused_update_condition(parent, hc.v, ld_stream(&ucol[p]));
+    }
+}
+
+__global__ void k_cc_compress(int nv, int *__restrict__ parent) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x)
+        CCHooks::update_condition(parent, v);
+}
+
+// ------------------------------------------------------------------ PageRank (pull)
+// psum[v] = sum over neighbours u of c[u] (c = rank/deg), FP64, degree tiers as in
+// the BC backward pass; then apply the PR_T / PR_D rule.
+__global__ __launch_bounds__(kBlock) void k_pr_gather(int nv, int huge, const int *__restrict__ row,
+                                                      const int *__restrict__ col, const float *__restrict__ c,
+                                                      double *__restrict__ psum) {
+    __shared__ int s_wv[kBlock];
+    __shared__ int s_wn;
+    for (int base = blockIdx.x * kBlock; base < nv; base += gridDim.x * kBlock) {
+        if (threadIdx.x == 0) s_wn = 0;
+        __syncthreads();
+        const int v = base + threadIdx.x;
+        if (v < nv) {
+            const int rb = row[v], re = row[v + 1], deg = re - rb;
+            if (deg < 32) {
+                double s = 0.0;
+                for (int p = rb; p < re; ++p) s += (double)c[ld_stream(&col[p])];
+                psum[v] = s;
+            } else if (deg < huge) {
+                s_wv[atomicAdd(&s_wn, 1)] = v;
+            } else {
+                psum[v] = 0.0;
+            }
+        }
+        __syncthreads();
+        const int wn = s_wn;
+        for (int k = threadIdx.x >> 6; k < wn; k += kBlock / MG_WAVE) {
+            const int u = s_wv[k];
+            double s = 0.0;
+            for (int p = row[u] + lane(); p < row[u + 1]; p += MG_WAVE) s += (double)c[ld_stream(&col[p])];
+            s = wave_sum(s);
+            if (lane() == 0) psum[u] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pr_gather_huge(const HugeChunk *__restrict__ chunks, int nch,
+                                                           const int *__restrict__ row, const int *__restrict__ col,
+                                                           const float *__restrict__ c, double *__restrict__ psum) {
+    __shared__ double s_t[kBlock / MG_WAVE];
+    for (int k = blockIdx.x; k < nch; k += gridDim.x) {
+        const HugeChunk hc = chunks[k];
+        const int b = row[hc.v] + hc.j * kHugeChunk, e = min(row[hc.v + 1], b + kHugeChunk);
+        double s = 0.0;
+        for (int p = b + threadIdx.x; p < e; p += kBlock) s += (double)c[ld_stream(&col[p])];
+        s = wave_sum(s);
+        if (lane() == 0) s_t[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < kBlock / MG_WAVE; ++i) t += s_t[i];
+            atomicAdd(&psum[hc.v], t);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pr_apply(int nv, int variant, float damping, float tol,
+                                                     const int *__restrict__ deg, const double *__restrict__ psum,
+                                                     float *__restrict__ rank, float *__restrict__ c,
+                                                     int *__restrict__ active, Ctl *__restrict__ ctl) {
+    __shared__ double s_tmp[kBlock / MG_WAVE];
+    __shared__ int s_ch;
+    if (threadIdx.x == 0) s_ch = 0;
+    __syncthreads();
+    double res = 0.0;
+    int ch = 0;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < nv; v += gridDim.x * kBlock) {
+        float f1;
+        const float old = rank[v];
+        ch |= PRHooks::update_condition(&rank[v], &active[v], psum[v], variant, damping, tol, &f1);
+        res += fabs((double)f1 - (double)old);
+        const int d = deg[v];
+        c[v] = d > 0 ? rank[v] / (float)d : 0.f;
+    }
+    res = block_sum<kBlock / MG_WAVE>(res, s_tmp);
+    if (ch) s_ch = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&ctl->resid, res);
+        if (s_ch) atomicOr(&ctl->changed, 1);
+    }
+}
+
+__global__ void k_pr_init(int nv, float *__restrict__ rank, float *__restrict__ c, int *__restrict__ active,
+                          Ctl *__restrict__ ctl) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+        rank[v] = 0.f; c[v] = 0.f; active[v] = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Ctl k{};
+        k.iter = 1; k.mode = MODE_TD; k.changed = 0; k.resid = 0.0;
+        *ctl = k;
+    }
+}
+
+// end of one iteration: stop when nothing changed or at the cap; clear the flags
+__global__ void k_pr_ctl(Ctl *ctl, int max_iter, long long ne) {
+    Ctl c = *ctl;
+    if (c.mode == MODE_DONE) return;
+    c.edges += ne;
+    c.levels = c.iter;
+    if (!c.changed || (max_iter > 0 && c.iter >= max_iter)) c.mode = MODE_DONE;
+    else { c.iter += 1; c.changed = 0; c.mf = 0; }
+    c.pad0 = 0;
+    *ctl = c;
+}
+
+__global__ void k_pr_begin(Ctl *ctl) {
+    if (ctl->mode == MODE_DONE) return;
+    ctl->resid = 0.0;
+    ctl->changed = 0;
+}
+
+__global__ void k_degrees(int nv, const int *__restrict__ row, int *__restrict__ deg) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) deg[v] = row[v + 1] - row[v];
+}
+
+// the PageRank kernels exit when the run is over, so a captured graph of several
+// iterations can be replayed past convergence
+__global__ void k_pr_gate(Ctl *ctl, int *go) { *go = ctl->mode != MODE_DONE; }
+
+}  // namespace
+
+static int build_huge_chunks(mg_engine *e, const std::vector<int32_t> &row, const int nv, HugeChunk **out, int *nch) {
+    std::vector<HugeChunk> v;
+    const int huge = e->opt.huge_degree;
+    for (int u = 0; u < nv; ++u) {
+        int d = row[u + 1] - row[u];
+        if (d >= huge)
+            for (int j = 0; j * kHugeChunk < d; ++j) v.push_back(HugeChunk{u, j});
+    }
+    *nch = (int)v.size();
+    *out = nullptr;
+    if (!v.empty()) {
+        MG_HIP(hipMalloc(out, sizeof(HugeChunk) * v.size()));
+        MG_HIP(hipMemcpy(*out, v.data(), sizeof(HugeChunk) * v.size(), hipMemcpyHostToDevice));
+    }
+    return MG_OK;
+}
+
+// ---------------------------------------------------------------------------- CC host
+int mg_cc_build(mg_engine *e, const mg_graph *g) {
+    const int nv = e->nv;
+    hipStream_t s = e->stream;
+    int *cnt = nullptr;
+    long long *uoff = nullptr;
+    MG_HIP(hipMalloc(&cnt, sizeof(int) * ((long long)nv + 1)));
+    MG_HIP(hipMalloc(&uoff, sizeof(long long) * ((long long)nv + 1)));
+    hipLaunchKernelGGL(k_cc_count, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, e->row, e->col, cnt);
+    MG_TRY(mg_scan64(s, nullptr, cnt, nv, uoff));
+    long long ne_u = 0;
+    MG_HIP(hipMemcpy(&ne_u, uoff + nv, sizeof(long long), hipMemcpyDeviceToHost));
+    e->ne_u = ne_u;
+    MG_HIP(hipMalloc(&e->urow, sizeof(int) * ((long long)nv + 1)));
+    MG_HIP(hipMalloc(&e->ucol, sizeof(int) * (ne_u > 0 ? ne_u : 1)));
+    hipLaunchKernelGGL(k_cc_fill, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, e->row, e->col, uoff, e->urow, e->ucol);
+    if (nv == 0) {
+        int z = 0;
+        MG_HIP(hipMemcpy(e->urow, &z, sizeof(int), hipMemcpyHostToDevice));
+    }
+    MG_HIP(hipStreamSynchronize(s));
+    MG_HIP(hipGetLastError());
+    std::vector<int32_t> urow((size_t)nv + 1);
+    MG_HIP(hipMemcpy(urow.data(), e->urow, sizeof(int) * ((size_t)nv + 1), hipMemcpyDeviceToHost));
+    hipFree(cnt);
+    hipFree(uoff);
+    MG_HIP(hipMalloc(&e->level, sizeof(int) * ((long long)nv + 64)));
+    MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
+    MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    int nch = 0;
+    MG_TRY(build_huge_chunks(e, urow, nv, &e->chunks, &nch));
+    e->chunk_cap = nch;
+    (void)g;
+    return MG_OK;
+}
+
+int mg_cc_reset(mg_engine *e) {
+    hipLaunchKernelGGL(k_cc_init, dim3(grid_for(e->nv, 256, 4096)), dim3(256), 0, e->stream, e->nv, e->level);
+    MG_HIP(hipGetLastError());
+    e->iter = 1;
+    e->done = e->nv == 0;
+    e->levels = 0;
+    return MG_OK;
+}
+
+int mg_cc_step(mg_engine *e, int *more) {
+    hipStream_t s = e->stream;
+    const int G = e->opt.grid_blocks > 0 ? e->opt.grid_blocks : 2048;
+    int slot = -1;
+    if (e->prof) mg_prof_begin(e, "k_cc_hook", 4.0 * e->ne_u + 4.0 * (e->nv + 1) + 8.0 * e->ne_u, &slot);
+    hipLaunchKernelGGL(k_cc_hook, dim3(grid_for(e->nv, kBlock, G)), dim3(kBlock), 0, s, e->nv, e->opt.huge_degree, e->urow,
+                       e->ucol, e->level);
+    if (e->prof) mg_prof_end(e, slot);
+    if (e->chunk_cap > 0) {
+        if (e->prof) mg_prof_begin(e, "k_cc_hook_huge", 0.0, &slot);
+        hipLaunchKernelGGL(k_cc_hook_huge, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, e->chunks, (int)e->chunk_cap,
+                           e->urow, e->ucol, e->level);
+        if (e->prof) mg_prof_end(e, slot);
+    }
+    if (e->prof) mg_prof_begin(e, "k_cc_compress", 8.0 * e->nv, &slot);
+    hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(e->nv, 256, 8192)), dim3(256), 0, s, e->nv, e->level);
+    if (e->prof) mg_prof_end(e, slot);
+    MG_HIP(hipGetLastError());
+    MG_HIP(hipStreamSynchronize(s));
+    e->iter += 1;
+    e->levels = 1;
+    e->edges_traversed = e->ne_u;
+    e->done = true;
+    *more = 0;
+    return MG_OK;
+}
+
+// ---------------------------------------------------------------------------- PR host
+int mg_pr_build(mg_engine *e, const mg_graph *g) {
+    const int nv = e->nv;
+    const long long padv = (long long)nv + 64;
+    MG_HIP(hipMalloc(&e->rank, sizeof(float) * padv));
+    MG_HIP(hipMalloc(&e->cval, sizeof(float) * padv));
+    MG_HIP(hipMalloc(&e->deg, sizeof(int) * padv));
+    MG_HIP(hipMalloc(&e->vfc, sizeof(int) * padv));
+    MG_HIP(hipMalloc(&e->psum, sizeof(double) * padv));
+    MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
+    MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    hipLaunchKernelGGL(k_degrees, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, e->stream, nv, e->row, e->deg);
+    int nch = 0;
+    MG_TRY(build_huge_chunks(e, g->row, nv, &e->chunks, &nch));
+    e->chunk_cap = nch;
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int mg_pr_reset(mg_engine *e) {
+    hipLaunchKernelGGL(k_pr_init, dim3(grid_for(e->nv, 256, 4096)), dim3(256), 0, e->stream, e->nv, e->rank, e->cval, e->vfc, e->ctl);
+    MG_HIP(hipGetLastError());
+    MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
+    MG_HIP(hipStreamSynchronize(e->stream));
+    e->iter = 1;
+    e->done = e->nv == 0;
+    e->levels = 0;
+    return MG_OK;
+}
+
+static int pr_launch_iteration(mg_engine *e) {
+    hipStream_t s = e->stream;
+    const int G = e->opt.grid_blocks > 0 ? e->opt.grid_blocks : 2048;
+    const int variant = e->algo == MG_PR_T ? 0 : 1;
+    int slot = -1;
+    hipLaunchKernelGGL(k_pr_begin, dim3(1), dim3(1), 0, s, e->ctl);
+    // algorithmic bytes of one pull iteration: row + col + contribution gather per edge
+    if (e->prof) mg_prof_begin(e, "k_pr_gather", 4.0 * (e->nv + 1) + 8.0 * e->ne + 8.0 * e->nv, &slot);
+    hipLaunchKernelGGL(k_pr_gather, dim3(grid_for(e->nv, kBlock, G)), dim3(kBlock), 0, s, e->nv, e->opt.huge_degree, e->row,
+                       e->col, e->cval, e->psum);
+    if (e->prof) mg_prof_end(e, slot);
+    if (e->chunk_cap > 0)
+        hipLaunchKernelGGL(k_pr_gather_huge, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, e->chunks,
+                           (int)e->chunk_cap, e->row, e->col, e->cval, e->psum);
+    if (e->prof) mg_prof_begin(e, "k_pr_apply", 28.0 * e->nv, &slot);
+    hipLaunchKernelGGL(k_pr_apply, dim3(grid_for(e->nv, kBlock, 4096)), dim3(kBlock), 0, s, e->nv, variant, e->opt.pr_damping,
+                       e->opt.pr_tolerance, e->deg, e->psum, e->rank, e->cval, e->vfc, e->ctl);
+    if (e->prof) mg_prof_end(e, slot);
+    hipLaunchKernelGGL(k_pr_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->opt.pr_max_iter, e->ne);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+int mg_pr_step(mg_engine *e, int *more) {
+    if (e->done) { *more = 0; return MG_OK; }
+    MG_TRY(pr_launch_iteration(e));
+    MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
+    MG_HIP(hipStreamSynchronize(e->stream));
+    e->iter = e->h_ctl->iter;
+    e->levels = e->h_ctl->levels;
+    e->edges_traversed = e->h_ctl->edges;
+    e->done = e->h_ctl->mode == MODE_DONE;
+    *more = e->done ? 0 : 1;
+    return MG_OK;
+}
+
+int mg_pr_capture(mg_engine *e, int iters, hipGraphExec_t *out) {
+    (void)e; (void)iters; (void)out;
+    return mg_fail(MG_ERR_STATE, "PageRank graph capture is not available in this build");
+}

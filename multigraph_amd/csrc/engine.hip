@@ -1,0 +1,317 @@
+// C ABI of the device engine (include/multigraph_amd.h): build, reset, step, run,
+// results, release, profiling.  Dispatches to the per-algorithm code in
+// frontier.hip (BFS/SSSP/BC) and cc_pr.hip (CC, PageRank).
+#include <chrono>
+#include <cstring>
+#include "engine.hpp"
+
+using namespace mg;
+
+void mg_free_dev(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+extern "C" void mg_default_options(mg_options *o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->mode = MG_MODE_HYBRID;
+    o->alpha = 15.0f;
+    o->beta = 18.0f;
+    o->huge_degree = 8192;
+    o->grid_blocks = 2048;
+    o->graph_levels = 0;
+    o->pr_damping = 0.85f;     // common.h:59
+    o->pr_tolerance = 0.01f;   // common.h:60
+    o->pr_max_iter = 0;        // the reference runs until no vertex changes
+    o->part_log2 = 14;
+}
+
+static void fill_defaults(mg_options *o) {
+    mg_options d;
+    mg_default_options(&d);
+    if (o->alpha <= 0) o->alpha = d.alpha;
+    if (o->beta <= 0) o->beta = d.beta;
+    if (o->huge_degree <= 0) o->huge_degree = d.huge_degree;
+    if (o->huge_degree < kHugeChunk) o->huge_degree = kHugeChunk;
+    if (o->grid_blocks <= 0) o->grid_blocks = d.grid_blocks;
+    if (o->pr_damping <= 0) o->pr_damping = d.pr_damping;
+    if (o->pr_tolerance <= 0) o->pr_tolerance = d.pr_tolerance;
+    if (o->part_log2 <= 0) o->part_log2 = d.part_log2;
+}
+
+extern "C" int mg_device_count(int *n) {
+    if (!n) return mg_fail(MG_ERR_ARG, "null argument");
+    int c = 0;
+    hipError_t err = hipGetDeviceCount(&c);
+    if (err != hipSuccess) { *n = 0; return mg_fail(MG_ERR_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(err)); }
+    *n = c;
+    return MG_OK;
+}
+
+static void release_all(mg_engine *e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->gexec) hipGraphExecDestroy(e->gexec);
+    void *ptrs[] = {e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->q, e->bm[0], e->bm[1],
+                    e->vis, e->chunks, e->lvl_off, e->aux, e->ctl, e->urow, e->ucol, e->rank, e->cval, e->deg, e->psum,
+                    e->pb.a_src, e->pb.b_dst, e->pb.amap, e->pb.msg, e->pb.a_part, e->pb.b_part, e->pb.unitsA,
+                    e->pb.unitsB, e->pb.split_parts, e->pb.unitsA64, e->pb.unitsB64};
+    for (void *p : ptrs) mg_free_dev(p);
+    if (e->h_ctl) (void)hipHostFree(e->h_ctl);
+    for (auto &ev : e->ev_pool) hipEventDestroy(ev);
+    for (auto &p : e->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+extern "C" int mg_build(const mg_graph *g, int algo, const mg_options *opt, mg_engine **out) {
+    if (!g || !out) return mg_fail(MG_ERR_ARG, "null argument");
+    if (algo < MG_BFS || algo > MG_PR_D) return mg_fail(MG_ERR_ARG, "unknown algorithm %d", algo);
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return mg_fail(MG_ERR_DEVICE, "no HIP device available");
+    mg_engine *e = new mg_engine();
+    e->algo = algo;
+    if (opt) e->opt = *opt; else mg_default_options(&e->opt);
+    fill_defaults(&e->opt);
+    e->nv = g->nv;
+    e->ne = g->ne;
+    auto fail = [&](int rc) { release_all(e); return rc; };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "hipStreamCreate failed"));
+    const long long nv1 = (long long)g->nv + 1;
+    const long long ne = g->ne > 0 ? g->ne : 1;
+    if (hipMalloc(&e->row, sizeof(int) * nv1) != hipSuccess || hipMalloc(&e->col, sizeof(int) * ne) != hipSuccess)
+        return fail(mg_fail(MG_ERR_NOMEM, "device allocation of the CSR failed"));
+    if (hipMemcpy(e->row, g->row.data(), sizeof(int) * nv1, hipMemcpyHostToDevice) != hipSuccess ||
+        (g->ne > 0 && hipMemcpy(e->col, g->col.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(mg_fail(MG_ERR_DEVICE, "CSR upload failed"));
+    if (algo == MG_SSSP) {
+        if (hipMalloc(&e->w, sizeof(int) * ne) != hipSuccess) return fail(mg_fail(MG_ERR_NOMEM, "weight allocation failed"));
+        if (g->ne > 0 && hipMemcpy(e->w, g->w.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(mg_fail(MG_ERR_DEVICE, "weight upload failed"));
+    }
+    int rc;
+    switch (algo) {
+    case MG_CC: rc = mg_cc_build(e, g); break;
+    case MG_PR_T:
+    case MG_PR_D: rc = mg_pr_build(e, g); break;
+    default: rc = mg_frontier_build(e, g); break;
+    }
+    if (rc != MG_OK) return fail(rc);
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "build failed on the device"));
+    e->source = g->start_p;
+    *out = e;
+    return MG_OK;
+}
+
+extern "C" int mg_release(mg_engine *e) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    release_all(e);
+    return MG_OK;
+}
+
+extern "C" int mg_reset(mg_engine *e, int source) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    int rc;
+    switch (e->algo) {
+    case MG_CC: rc = mg_cc_reset(e); break;
+    case MG_PR_T:
+    case MG_PR_D: rc = mg_pr_reset(e); break;
+    default:
+        if (e->nv > 0 && (source < 0 || source >= e->nv)) return mg_fail(MG_ERR_ARG, "source %d outside [0,%d)", source, e->nv);
+        rc = mg_frontier_reset(e, source);
+        break;
+    }
+    if (rc == MG_OK) e->ready = true;
+    return rc;
+}
+
+extern "C" int mg_step(mg_engine *e, int *iter) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    if (!e->ready) return mg_fail(MG_ERR_STATE, "mg_step before mg_reset");
+    int more = 0;
+    switch (e->algo) {
+    case MG_CC: {
+        if (e->done) { more = 0; break; }
+        int rc = mg_cc_step(e, &more);
+        if (rc != MG_OK) return rc;
+        break;
+    }
+    case MG_PR_T:
+    case MG_PR_D: {
+        int rc = mg_pr_step(e, &more);
+        if (rc != MG_OK) return rc;
+        break;
+    }
+    default: {
+        int rc = mg_frontier_level(e);
+        if (rc != MG_OK) return rc;
+        more = e->done ? 0 : 1;
+        break;
+    }
+    }
+    if (iter) *iter = e->iter;
+    return more;
+}
+
+extern "C" int mg_bc_backward(mg_engine *e, int iter) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    if (e->algo != MG_BC) return mg_fail(MG_ERR_STATE, "mg_bc_backward needs a BC engine");
+    if (!e->done) return mg_fail(MG_ERR_STATE, "mg_bc_backward before the forward pass finished");
+    (void)iter;
+    int rc = mg_bc_backward_impl(e);
+    if (rc != MG_OK) return rc;
+    MG_HIP(hipStreamSynchronize(e->stream));
+    return MG_OK;
+}
+
+extern "C" int mg_run(mg_engine *e, int source, float *ms) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    MG_TRY(mg_reset(e, source));
+    MG_HIP(hipStreamSynchronize(e->stream));
+    hipEvent_t a, b;
+    MG_HIP(hipEventCreate(&a));
+    MG_HIP(hipEventCreate(&b));
+    MG_HIP(hipEventRecord(a, e->stream));
+    const bool frontier = e->algo == MG_BFS || e->algo == MG_SSSP || e->algo == MG_BC;
+    const int K = e->opt.graph_levels;
+    int rc = MG_OK;
+    if (frontier && K > 0 && !e->prof) {
+        if (!e->gexec || e->gexec_levels != K) {
+            if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+            rc = mg_frontier_capture(e, K, &e->gexec);
+            e->gexec_levels = K;
+        }
+        while (rc == MG_OK) {
+            if (hipGraphLaunch(e->gexec, e->stream) != hipSuccess) { rc = mg_fail(MG_ERR_DEVICE, "hipGraphLaunch failed"); break; }
+            if (hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+                hipStreamSynchronize(e->stream) != hipSuccess) { rc = mg_fail(MG_ERR_DEVICE, "graph replay failed"); break; }
+            if (e->h_ctl->mode == MODE_DONE) break;
+        }
+        if (rc == MG_OK) {
+            e->done = true;
+            e->iter = e->h_ctl->iter;
+            e->levels = e->h_ctl->levels;
+            e->edges_traversed = e->h_ctl->edges;
+        }
+    } else {
+        int it = 1;
+        for (;;) {
+            int m = mg_step(e, &it);
+            if (m < 0) { rc = m; break; }
+            if (m == 0) break;
+        }
+    }
+    if (rc == MG_OK && e->algo == MG_BC) rc = mg_bc_backward_impl(e);
+    hipEventRecord(b, e->stream);
+    hipError_t se = hipEventSynchronize(b);
+    float t = 0.f;
+    hipEventElapsedTime(&t, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc != MG_OK) return rc;
+    if (se != hipSuccess) return mg_fail(MG_ERR_DEVICE, "run failed: %s", hipGetErrorString(se));
+    if (e->prof) MG_TRY(mg_prof_flush(e));
+    if (ms) *ms = t;
+    return MG_OK;
+}
+
+extern "C" int mg_sync(mg_engine *e) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    MG_HIP(hipStreamSynchronize(e->stream));
+    return MG_OK;
+}
+
+extern "C" int mg_iterations(const mg_engine *e, int *iters) {
+    if (!e || !iters) return mg_fail(MG_ERR_ARG, "null argument");
+    *iters = e->levels;
+    return MG_OK;
+}
+
+extern "C" int mg_get_result(mg_engine *e, int which, void *dst, int64_t bytes) {
+    if (!e || !dst) return mg_fail(MG_ERR_ARG, "null argument");
+    const void *src = nullptr;
+    int64_t need = 0;
+    switch (which) {
+    case MG_RES_LEVEL:
+        if (e->algo == MG_PR_T || e->algo == MG_PR_D) return mg_fail(MG_ERR_ARG, "PageRank has no level result");
+        src = e->level; need = (int64_t)sizeof(int) * e->nv; break;
+    case MG_RES_SIGMA:
+        if (e->algo != MG_BC) return mg_fail(MG_ERR_ARG, "sigma needs a BC engine");
+        src = e->sigma; need = (int64_t)sizeof(double) * e->nv; break;
+    case MG_RES_DELTA:
+        if (e->algo != MG_BC) return mg_fail(MG_ERR_ARG, "delta needs a BC engine");
+        src = e->delta; need = (int64_t)sizeof(float) * e->nv; break;
+    case MG_RES_RANK:
+        if (e->algo != MG_PR_T && e->algo != MG_PR_D) return mg_fail(MG_ERR_ARG, "rank needs a PageRank engine");
+        src = e->rank; need = (int64_t)sizeof(float) * e->nv; break;
+    default: return mg_fail(MG_ERR_ARG, "unknown result %d", which);
+    }
+    if (bytes < need) return mg_fail(MG_ERR_ARG, "buffer of %lld bytes, need %lld", (long long)bytes, (long long)need);
+    MG_HIP(hipStreamSynchronize(e->stream));
+    if (need > 0) MG_HIP(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+    return MG_OK;
+}
+
+// --------------------------------------------------------------------------- profiling
+int mg_prof_begin(mg_engine *e, const char *name, double bytes, int *slot) {
+    int idx = -1;
+    for (size_t i = 0; i < e->stats.size(); ++i) if (e->stats[i].name == name) { idx = (int)i; break; }
+    if (idx < 0) { e->stats.push_back(KStat{name, 0, 0.0, 0.0}); idx = (int)e->stats.size() - 1; }
+    hipEvent_t a, b;
+    if (e->ev_pool.size() >= 2) {
+        a = e->ev_pool.back(); e->ev_pool.pop_back();
+        b = e->ev_pool.back(); e->ev_pool.pop_back();
+    } else {
+        MG_HIP(hipEventCreate(&a));
+        MG_HIP(hipEventCreate(&b));
+    }
+    MG_HIP(hipEventRecord(a, e->stream));
+    e->pending.push_back(mg_engine::Pending{idx, a, b, bytes});
+    *slot = (int)e->pending.size() - 1;
+    return MG_OK;
+}
+
+int mg_prof_end(mg_engine *e, int slot) {
+    if (slot < 0 || slot >= (int)e->pending.size()) return MG_OK;
+    MG_HIP(hipEventRecord(e->pending[slot].b, e->stream));
+    return MG_OK;
+}
+
+int mg_prof_flush(mg_engine *e) {
+    MG_HIP(hipStreamSynchronize(e->stream));
+    for (auto &p : e->pending) {
+        float t = 0.f;
+        hipEventElapsedTime(&t, p.a, p.b);
+        KStat &k = e->stats[p.stat];
+        k.launches += 1;
+        k.ms += t;
+        k.bytes += p.bytes;
+        e->ev_pool.push_back(p.a);
+        e->ev_pool.push_back(p.b);
+    }
+    e->pending.clear();
+    return MG_OK;
+}
+
+extern "C" int mg_set_profiling(mg_engine *e, int on) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    MG_TRY(mg_prof_flush(e));
+    e->prof = on != 0;
+    e->stats.clear();
+    return MG_OK;
+}
+
+extern "C" int mg_kernel_stats(mg_engine *e, int idx, char *name, int name_len, int64_t *launches, double *ms, double *bytes) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    MG_TRY(mg_prof_flush(e));
+    if (idx < 0 || idx >= (int)e->stats.size()) return mg_fail(MG_ERR_ARG, "no kernel statistic %d", idx);
+    const KStat &k = e->stats[idx];
+    if (name && name_len > 0) { strncpy(name, k.name.c_str(), name_len - 1); name[name_len - 1] = 0; }
+    if (launches) *launches = k.launches;
+    if (ms) *ms = k.ms;
+    if (bytes) *bytes = k.bytes;
+    return MG_OK;
+}
+
+// batched runs are implemented in batch.hip

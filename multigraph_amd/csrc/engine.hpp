@@ -1,0 +1,189 @@
+// Engine internals shared by the HIP translation units: the device control block,
+// the engine handle (role of struct HYB + struct vector_data, code/common.h:129-199)
+// and launch/profiling helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "mg_internal.hpp"
+
+#define MG_HIP(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t _e = (x);                                                                           \
+        if (_e != hipSuccess)                                                                          \
+            return mg_fail(MG_ERR_DEVICE, "%s: %s (%s:%d)", #x, hipGetErrorString(_e), __FILE__, __LINE__); \
+    } while (0)
+
+#define MG_TRY(x)                 \
+    do {                          \
+        int _r = (x);             \
+        if (_r != MG_OK) return _r; \
+    } while (0)
+
+namespace mg {
+
+enum { MODE_TD = 0, MODE_BU = 1, MODE_DONE = 2 };
+
+constexpr int kBlock = 256;          // threads per workgroup of the frontier kernels
+constexpr int kHugeChunk = 4096;     // edges per chunk of a split high-degree vertex
+
+// Device-resident control block.  One small kernel advances it between levels, so
+// a sequence of levels needs no host round trip and can be captured in a hipGraph;
+// in eager mode the host reads it back once per level (the reference's D2H of its
+// 4 queue counters, process.cu:1341).
+struct Ctl {
+    int iter;        // level / iteration being produced (the hooks' `attribute`)
+    int mode;        // MODE_TD, MODE_BU or MODE_DONE
+    int in_begin;    // current frontier = q[in_begin, in_begin + in_n)
+    int in_n;
+    int out_begin;   // next frontier appended at q[out_begin + ...]
+    int out_n;       // append counter of the level in flight
+    int nchunks;     // split high-degree vertices of the level in flight
+    int convert;     // 1: build bitmaps from levels before a pull level
+    int bm_cur;      // bitmap slot holding the current frontier
+    int levels;      // levels produced so far (BC log length)
+    int changed;     // dense-sweep "something changed" flag
+    int pad0;
+    long long out_mf;  // degree sum of the appended vertices
+    long long mf;      // degree sum of the current frontier
+    long long mu;      // degree sum of unexplored vertices
+    double resid;      // PageRank FP64 L1 residual of the last iteration
+    long long edges;   // edges traversed (statistics)
+};
+
+struct Params {
+    int nv;
+    int log_queue;   // 1: frontiers appended to one log (BFS/BC); 0: ping-pong halves (SSSP)
+    int forced;      // MG_MODE_*
+    int huge;        // split threshold
+    long long ne;
+    float alpha, beta;
+};
+
+struct HugeChunk { int v, j; };
+
+struct KStat {
+    std::string name;
+    long long launches = 0;
+    double ms = 0.0, bytes = 0.0;
+};
+
+__host__ __device__ inline void advance(Ctl &c, const Params &p) {
+    // record the level just produced (BC level log)
+    const int produced = c.out_n;
+    c.mu -= c.out_mf;
+    if (c.mu < 0) c.mu = 0;
+    if (produced == 0) { c.mode = MODE_DONE; c.in_n = 0; return; }
+    const int prev = c.mode;
+    int next;
+    if (p.forced == MG_MODE_SPARSE) next = MODE_TD;
+    else if (p.forced == MG_MODE_DENSE) next = MODE_BU;
+    else if (prev == MODE_TD) next = ((double)c.out_mf > (double)c.mu / (double)p.alpha) ? MODE_BU : MODE_TD;
+    else next = ((double)produced < (double)p.nv / (double)p.beta) ? MODE_TD : MODE_BU;
+    c.convert = (next == MODE_BU && prev != MODE_BU) ? 1 : 0;
+    if (prev == MODE_BU && next == MODE_BU) c.bm_cur ^= 1;
+    c.in_begin = c.out_begin;
+    c.in_n = produced;
+    if (p.log_queue) c.out_begin = c.out_begin + produced;
+    else c.out_begin = (c.out_begin == 0) ? p.nv : 0;
+    c.mf = c.out_mf;
+    c.out_n = 0;
+    c.out_mf = 0;
+    c.nchunks = 0;
+    c.iter += 1;
+    c.levels = c.iter;
+    c.mode = next;
+}
+
+}  // namespace mg
+
+// PageRank partitioned layout (the gen_structure.cu role, built on the GPU).
+struct mg_pb_layout {
+    int plog = 14;            // log2 partition size
+    int np = 0;               // partitions
+    int G = 16;               // entries per mapper group
+    long long slots = 0;      // padded stream length (both streams)
+    uint16_t *a_src = nullptr;   // src-major stream: local source id (0xFFFF = padding)
+    uint16_t *b_dst = nullptr;   // dst-major stream: local destination id (0xFFFF = padding)
+    int *amap = nullptr;         // per group of G A-slots: base of the matching B-slots
+    float *msg = nullptr;        // per-slot message, written in A order into B positions
+    long long *a_part = nullptr; // np+1: start of each source partition in stream A
+    long long *b_part = nullptr; // np+1: start of each destination partition in stream B
+    // work units (chunks of the streams), built on the host from a_part / b_part
+    int nA = 0, nB = 0, nApply = 0;
+    int4 *unitsA = nullptr;      // {partition, begin_lo, end_lo, begin_hi/end_hi packed} see pr.hip
+    int4 *unitsB = nullptr;
+    int *split_parts = nullptr;  // destination partitions processed by more than one unit
+    long long *unitsA64 = nullptr, *unitsB64 = nullptr;
+};
+
+struct mg_engine {
+    int algo = 0;
+    mg_options opt{};
+    int nv = 0;
+    long long ne = 0;
+    int source = 0;
+    hipStream_t stream = nullptr;
+    // CSR on the device (the renumbered CSR of HYB: _ncsc_v/_ncsc_e/_ncsc_ev)
+    int *row = nullptr, *col = nullptr, *w = nullptr;
+    // run state (struct vector_data)
+    int *level = nullptr;      // BFS/BC level, SSSP distance, CC label (vv)
+    int *vfc = nullptr;        // SSSP enqueue stamp / PR active flag
+    double *sigma = nullptr;   // BC path counts
+    float *delta = nullptr;    // BC dependencies
+    double *dacc = nullptr;    // BC split-vertex partial sums
+    int *q = nullptr;          // frontier queue / level log
+    long long qcap = 0;
+    uint64_t *bm[2] = {nullptr, nullptr};
+    uint64_t *vis = nullptr;
+    mg::HugeChunk *chunks = nullptr;
+    long long chunk_cap = 0;
+    int *lvl_off = nullptr;    // BC level log offsets (levels + 2)
+    int *aux = nullptr;        // small device counters (BC split-vertex chunk count)
+    mg::Ctl *ctl = nullptr;
+    mg::Ctl *h_ctl = nullptr;  // pinned mirror
+    mg::Params P{};
+    // CC: upper-triangular CSR
+    int *urow = nullptr, *ucol = nullptr;
+    long long ne_u = 0;
+    // PageRank
+    float *rank = nullptr, *cval = nullptr;
+    int *deg = nullptr;
+    double *psum = nullptr;
+    mg_pb_layout pb;
+    // host-side progress
+    int iter = 1;
+    bool ready = false;
+    bool done = false;
+    int levels = 0;
+    long long edges_traversed = 0;
+    // hipGraph of `graph_levels` levels
+    hipGraphExec_t gexec = nullptr;
+    int gexec_levels = 0;
+    // profiling
+    bool prof = false;
+    std::vector<mg::KStat> stats;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending { int stat; hipEvent_t a, b; double bytes; };
+    std::vector<Pending> pending;
+};
+
+// instrumentation helpers (engine.hip)
+int mg_prof_begin(mg_engine *e, const char *name, double bytes, int *slot);
+int mg_prof_end(mg_engine *e, int slot);
+int mg_prof_flush(mg_engine *e);
+
+// per-algorithm entry points
+int mg_frontier_build(mg_engine *e, const mg_graph *g);
+int mg_frontier_reset(mg_engine *e, int source);
+int mg_frontier_level(mg_engine *e);        // one eager level; updates e->done
+int mg_frontier_capture(mg_engine *e, int levels, hipGraphExec_t *out);
+int mg_bc_backward_impl(mg_engine *e);
+int mg_cc_build(mg_engine *e, const mg_graph *g);
+int mg_cc_reset(mg_engine *e);
+int mg_cc_step(mg_engine *e, int *more);
+int mg_pr_build(mg_engine *e, const mg_graph *g);
+int mg_pr_reset(mg_engine *e);
+int mg_pr_step(mg_engine *e, int *more);
+int mg_pr_capture(mg_engine *e, int iters, hipGraphExec_t *out);
+void mg_free_dev(void *p);

@@ -1,0 +1,162 @@
+// The six drivers of the reference (code/main.cu:18-159, install.sh:22-27), one
+// binary per algorithm macro:  ./BFS file.mtx source iterations
+// Prints "processing : X ms,Y MTEPS," with the reference's TEPS conventions
+// (BFS/SSSP ne/t, BC 2*ne/t, CC ne/t over the src<dst edges, PR ne/t_total), then
+// "validation : PASS|FAIL".  Validation here is a certificate check of the engine's
+// output against the graph (no second solver): the defining equations of each
+// result must hold on every edge.
+#if defined(MG_DRIVER_BFS)
+#define BFS
+#elif defined(MG_DRIVER_SSSP)
+#define SSSP
+#elif defined(MG_DRIVER_BC)
+#define BC
+#elif defined(MG_DRIVER_CC)
+#define CC
+#elif defined(MG_DRIVER_DATA_DRIVEN_PR)
+#define PR_T
+#define DENSE_MODE
+#elif defined(MG_DRIVER_TOPOLOGY_DRIVEN_PR)
+#define PR_D
+#define DENSE_MODE
+#endif
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include "multigraph_compat.h"
+
+static double rtclock() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------------ certificates
+static int verify_levels(const csc_package *inp, const std::vector<int> &L, int src, const int *w) {
+    const int nv = inp->nv;
+    if (nv == 0) return -1;
+    if (L[src] != 0) return src;
+    for (int v = 0; v < nv; ++v) {
+        const int lv = L[v];
+        bool tight = (v == src);
+        for (int p = inp->csc_v[v]; p < inp->csc_v[v + 1]; ++p) {
+            const int u = inp->csc_e[p];
+            const int wu = w ? w[p] : 1;
+            if (L[u] != MG_INF && (lv == MG_INF || lv > L[u] + wu)) return v;  // not relaxed
+            if (lv != MG_INF && L[u] != MG_INF && L[u] + wu == lv) tight = true;
+        }
+        if (lv != MG_INF && !tight) return v;  // no predecessor realises the value
+    }
+    return -1;
+}
+
+static int verify_cc(const csc_package *inp, const std::vector<int> &lab) {
+    for (int v = 0; v < inp->nv; ++v) {
+        if (lab[v] < 0 || lab[v] > v || lab[lab[v]] != lab[v]) return v;
+        for (int p = inp->csc_v[v]; p < inp->csc_v[v + 1]; ++p)
+            if (lab[inp->csc_e[p]] != lab[v]) return v;
+    }
+    return -1;
+}
+
+template <class OUT>
+static void verifyResults(const csc_package *inp, HYB *, OUT *out, int src) {
+    int bad = -1;
+#if defined(BFS)
+    bad = verify_levels(inp, out->r, src, nullptr);
+#elif defined(SSSP)
+    bad = verify_levels(inp, out->r, src, inp->csc_ev);
+#elif defined(BC)
+    bad = verify_levels(inp, out->r, src, nullptr);
+    for (int v = 0; bad < 0 && v < inp->nv; ++v) {
+        if (out->r[v] == MG_INF) continue;
+        double s = 0.0, d = 0.0;
+        for (int p = inp->csc_v[v]; p < inp->csc_v[v + 1]; ++p) {
+            const int u = inp->csc_e[p];
+            if (out->r[u] + 1 == out->r[v]) s += out->r2[u];
+            if (out->r[u] == out->r[v] + 1) d += out->r2[v] / out->r2[u] * (1.0 + out->r3[u]);
+        }
+        if (v != src && s != out->r2[v]) bad = v;
+        if (std::fabs(d - out->r3[v]) > 1e-5 * std::fmax(1.0, std::fabs(d))) bad = v;
+    }
+#elif defined(CC)
+    bad = verify_cc(inp, out->r);
+#else
+    (void)src;
+    // fixpoint of r = 0.15 + 0.85 * sum r_u/deg_u within the stopping tolerance
+    const float tol = mg_compat_options().pr_tolerance;
+    for (int v = 0; v < inp->nv && bad < 0; ++v) {
+        double s = 0.0;
+        for (int p = inp->csc_v[v]; p < inp->csc_v[v + 1]; ++p) {
+            const int u = inp->csc_e[p];
+            s += out->r[u] / (double)(inp->csc_v[u + 1] - inp->csc_v[u]);
+        }
+        const double f = 0.15 + 0.85 * s;
+        if (std::fabs(f - out->r[v]) > 20.0 * tol * std::fmax(1.0, std::fabs(f))) bad = v;
+    }
+#endif
+    if (bad < 0) fprintf(stdout, "validation : PASS\n");
+    else fprintf(stdout, "validation : FAIL %d\n", bad);
+}
+
+int main(int argc, char **argv) {
+    if (argc < 4) {
+        fprintf(stderr, "usage: %s file.mtx source-vertex num-of-iteration\n", argv[0]);
+        return 1;
+    }
+    const int u_loop = atoi(argv[3]);
+    if (u_loop < 1) {
+        fprintf(stderr, "num-of-iteration must be at least 1\n");
+        return 1;
+    }
+    csc_package inp;
+    HYB m;
+    vector_data vd;
+    OUTPUT out;
+    double total_time = 0.0;
+
+    generate_CSC(&inp, argv);
+    generate_MultiGraph(&m, &inp);
+    initValue(inp.start_p, &m, &vd);
+    int iter = 1;
+    for (int loop = 0; loop < u_loop; ++loop) {
+        iter = 1;
+        resetValue(inp.start_p, &m, &vd);
+        mg_sync(m.e);
+        const double s1 = rtclock();
+#if defined(CC)
+        while (MultiGraph_E_E(&m, &vd, &out, &iter)) {
+            while (MultiGraph_V_V(&m, &vd, &vd, &out, &iter));
+        }
+#elif defined(PR_T) || defined(PR_D)
+        while (MultiGraph_V_E_V_NOTIDEM(&m, &inp, &vd, &out, &iter));
+#else
+        while (MultiGraph_V_E_V(&m, &inp, &vd, &out, &iter));
+#endif
+#if defined(BC)
+        MultiGraph_V_E_V_pull(&m, &inp, &vd, &out, &iter);
+#endif
+        mg_sync(m.e);
+        total_time += rtclock() - s1;
+    }
+    const double total_ms = total_time * 1000.0 / u_loop;
+#if defined(BC)
+    const double teps_edges = 2.0 * inp.ne;
+#else
+    const double teps_edges = (double)inp.ne;
+#endif
+    fprintf(stdout, "processing : %f ms,%f MTEPS,", total_ms, teps_edges / (total_ms * 1000.0));
+    mg_fill_output(&m, &out);
+    verifyResults(&inp, &m, &out, inp.start_p);
+    if (const char *p = getenv("MG_PRINT_OUTPUT")) {
+        // validate.cu:321-329: idx<TAB>value in compacted ids
+        FILE *fp = fopen(p, "w");
+        if (fp) {
+            for (int i = 0; i < inp.nv; ++i) fprintf(fp, "%d\t%f\n", i, (double)out.r[i]);
+            fclose(fp);
+        }
+    }
+    release_MultiGraph(&m);
+    mg_graph_free(inp.g);
+    return 0;
+}

@@ -1,0 +1,129 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "liboracle.so")
+INF = 0x64646464
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        I = C.POINTER(C.c_int32)
+        L.orc_bfs.argtypes = [C.c_int, I, I, C.c_int, I]
+        L.orc_sssp_bf.argtypes = [C.c_int, I, I, I, C.c_int, I]
+        L.orc_sssp_dijkstra.argtypes = [C.c_int, I, I, I, C.c_int, I]
+        L.orc_canon.argtypes = [C.c_int, I, I]
+        L.orc_cc_labelprop.argtypes = [C.c_int, I, I, I, C.POINTER(C.c_int)]
+        L.orc_cc_unionfind.argtypes = [C.c_int, I, I, I]
+        D = C.POINTER(C.c_double)
+        F = C.POINTER(C.c_float)
+        L.orc_bc_fp64.argtypes = [C.c_int, I, I, C.c_int, I, D, D]
+        L.orc_pr_f32.argtypes = [C.c_int, I, I, C.c_int, C.c_float, C.c_float, C.c_int, F]
+        L.orc_pr_fp64.argtypes = [C.c_int, I, I, C.c_int, C.c_double, C.c_double, C.c_int, D]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_int32):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _csr(row, col):
+    return np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32)
+
+
+def bfs(row, col, src):
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    out = np.empty(nv, np.int32)
+    lib().orc_bfs(nv, _p(row), _p(col), src, _p(out))
+    return out
+
+
+def sssp_bf(row, col, w, src):
+    row, col = _csr(row, col)
+    w = np.ascontiguousarray(w, np.int32)
+    nv = len(row) - 1
+    out = np.empty(nv, np.int32)
+    sweeps = lib().orc_sssp_bf(nv, _p(row), _p(col), _p(w), src, _p(out))
+    return out, sweeps
+
+
+def sssp_dijkstra(row, col, w, src):
+    row, col = _csr(row, col)
+    w = np.ascontiguousarray(w, np.int32)
+    nv = len(row) - 1
+    out = np.empty(nv, np.int32)
+    lib().orc_sssp_dijkstra(nv, _p(row), _p(col), _p(w), src, _p(out))
+    return out
+
+
+def canon(labels):
+    labels = np.ascontiguousarray(labels, np.int32)
+    out = np.empty_like(labels)
+    k = lib().orc_canon(len(labels), _p(labels), _p(out))
+    if k < 0:
+        raise ValueError("labels out of range")
+    return out
+
+
+def cc_labelprop(row, col):
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    out = np.empty(nv, np.int32)
+    sweeps = C.c_int(0)
+    k = lib().orc_cc_labelprop(nv, _p(row), _p(col), _p(out), C.byref(sweeps))
+    return out, k, sweeps.value
+
+
+def cc_unionfind(row, col):
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    out = np.empty(nv, np.int32)
+    k = lib().orc_cc_unionfind(nv, _p(row), _p(col), _p(out))
+    return out, k
+
+
+def bc_fp64(row, col, src):
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    lv = np.empty(nv, np.int32)
+    sg = np.empty(nv, np.float64)
+    dl = np.empty(nv, np.float64)
+    lib().orc_bc_fp64(nv, _p(row), _p(col), src, _p(lv), _p(sg, C.c_double), _p(dl, C.c_double))
+    return lv, sg, dl
+
+
+def pr_f32(row, col, variant, damping=0.85, tol=0.01, max_iter=0):
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    r = np.empty(nv, np.float32)
+    it = lib().orc_pr_f32(nv, _p(row), _p(col), variant, damping, tol, max_iter, _p(r, C.c_float))
+    return r, it
+
+
+def pr_fp64(row, col, variant, damping=0.85, tol=0.01, max_iter=0):
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    r = np.empty(nv, np.float64)
+    it = lib().orc_pr_fp64(nv, _p(row), _p(col), variant, damping, tol, max_iter, _p(r, C.c_double))
+    return r, it
