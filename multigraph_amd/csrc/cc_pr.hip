@@ -2,6 +2,7 @@
 // and PageRank (the role of MultiGraph_V_E_V_NOTIDEM, processi.cu) for gfx950.
 #include "engine.hpp"
 #include "user_function.hpp"
+#include "lb.hpp"
 
 using namespace mg;
 
@@ -141,40 +142,95 @@ __global__ void k_cc_init(int nv, int *__restrict__ parent) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) parent[v] = v;
 }
 
-// hook every edge: block-scan load balance over 256 vertices, split vertices above
-// `huge` edges into chunks handled by k_cc_hook_huge
-__global__ __launch_bounds__(kBlock) void k_cc_hook(int nv, int huge, const int *__restrict__ urow,
-                                                    const int *__restrict__ ucol, int *__restrict__ parent) {
-    __shared__ int s_off[kBlock], s_rb[kBlock], s_u[kBlock];
-    __shared__ int s_tmp[kBlock / MG_WAVE + 1];
-    for (int base = blockIdx.x * kBlock; base < nv; base += gridDim.x * kBlock) {
-        const int u = base + threadIdx.x;
-        int rb = 0, deg = 0;
-        if (u < nv) { rb = urow[u]; deg = urow[u + 1] - rb; if (deg >= huge) deg = 0; }
-        int total;
-        int off = block_excl_scan<kBlock / MG_WAVE>(deg, s_tmp, &total);
-        s_off[threadIdx.x] = off; s_rb[threadIdx.x] = rb; s_u[threadIdx.x] = u;
-        __syncthreads();
-        for (int e0 = 0; e0 < total; e0 += kBlock) {
-            const int e = e0 + threadIdx.x;
-            if (e < total) {
-                int lo = 0, hi = kBlock - 1;
-                while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (s_off[mid] <= e) lo = mid; else hi = mid - 1; }
-                const int p = s_rb[lo] + (e - s_off[lo]);
-                CCHooks::fused_update_condition(parent, s_u[lo], ld_stream(&ucol[p]));
-            }
-        }
+// Edge-balanced hooking: workgroup c takes edges [c*kEPB, (c+1)*kEPB) of the
+// upper-triangular CSR; owners come from chunk_owners (lb.hpp).
+__global__ __launch_bounds__(kBlock) void k_cc_hook_lb(int nch, long long ne, const int *__restrict__ off,
+                                                       const int *__restrict__ adj, const int *__restrict__ cstart,
+                                                       int *__restrict__ parent) {
+    __shared__ int s_own[kEPB];
+    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+        const int e0 = c * kEPB;
+        const int e1 = (int)min((long long)e0 + kEPB, ne);
+        chunk_owners(off, cstart[c], cstart[c + 1], e0, e1, s_own);
+        for (int e = e0 + threadIdx.x; e < e1; e += kBlock) CCHooks::fused_update_condition(parent, s_own[e - e0], ld_stream(&adj[e]));
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cc_hook_huge(const HugeChunk *__restrict__ chunks, int nch,
-                                                         const int *__restrict__ urow, const int *__restrict__ ucol,
+// Sampling pass without atomics: rows are sorted, so col[row[v]] is v's smallest
+// neighbour; parent[v] = min(v, that neighbour) joins v with a vertex of its own
+// component, keeps parent[v] <= v, and is written only by v's thread.
+__global__ void k_cc_link_min(int nv, const int *__restrict__ row, const int *__restrict__ col, int *__restrict__ parent) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+        const int b = row[v];
+        const int m = (row[v + 1] > b) ? col[b] : v;
+        parent[v] = m < v ? m : v;
+    }
+}
+
+// hook the second neighbour of every vertex (the first was linked by k_cc_link_min)
+__global__ void k_cc_link_second(int nv, const int *__restrict__ row, const int *__restrict__ col, int *__restrict__ parent) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+        const int b = row[v] + 1;
+        if (b < row[v + 1]) CCHooks::fused_update_condition(parent, v, col[b]);
+    }
+}
+
+// the most frequent root among 1024 evenly spaced vertices (bitonic sort in LDS)
+__global__ __launch_bounds__(1024) void k_cc_sample(int nv, int *__restrict__ parent, int *__restrict__ out) {
+    __shared__ int s[1024];
+    __shared__ int s_best, s_cnt;
+    const int t = threadIdx.x;
+    const long long v = ((long long)t * 2654435761ll) % nv;
+    s[t] = CCHooks::find_ro(parent, (int)v);
+    __syncthreads();
+    for (int k = 2; k <= 1024; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int i = t ^ j;
+            if (i > t) {
+                const bool up = (t & k) == 0;
+                const int a = s[t], b = s[i];
+                if ((a > b) == up) { s[t] = b; s[i] = a; }
+            }
+            __syncthreads();
+        }
+    if (t == 0) {
+        int best = s[0], bc = 0, cur = s[0], cc = 0;
+        for (int i = 0; i < 1024; ++i) {
+            if (s[i] == cur) ++cc; else { cur = s[i]; cc = 1; }
+            if (cc > bc) { bc = cc; best = cur; }
+        }
+        s_best = best; s_cnt = bc;
+        out[0] = best;
+        out[1] = bc;
+    }
+}
+
+// hook the remaining neighbours of every vertex outside the sampled giant tree
+__global__ __launch_bounds__(kBlock) void k_cc_link_rest(int nv, int k, const int *__restrict__ row,
+                                                         const int *__restrict__ col, const int *__restrict__ giant,
                                                          int *__restrict__ parent) {
-    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
-        const HugeChunk hc = chunks[c];
-        const int b = urow[hc.v] + hc.j * kHugeChunk, e = min(urow[hc.v + 1], b + kHugeChunk);
-        for (int p = b + threadIdx.x; p < e; p += kBlock) CCHooks::fused_update_condition(parent, hc.v, ld_stream(&ucol[p]));
+    __shared__ int s_wv[kBlock];
+    __shared__ int s_wn;
+    const int c = giant[0];
+    for (int base = blockIdx.x * kBlock; base < nv; base += gridDim.x * kBlock) {
+        if (threadIdx.x == 0) s_wn = 0;
+        __syncthreads();
+        const int v = base + threadIdx.x;
+        if (v < nv && CCHooks::find(parent, v) != c) {
+            const int r0 = row[v], b = r0 + k, e = row[v + 1];
+            // k_cc_link_min joined v with its smallest neighbour only if that one is smaller
+            if (e > r0 && col[r0] > v) CCHooks::fused_update_condition(parent, v, col[r0]);
+            if (e - b >= 64) s_wv[atomicAdd(&s_wn, 1)] = v;
+            else for (int p = b; p < e; ++p) CCHooks::fused_update_condition(parent, v, col[p]);
+        }
+        __syncthreads();
+        const int wn = s_wn;
+        for (int i = threadIdx.x >> 6; i < wn; i += kBlock / MG_WAVE) {
+            const int u = s_wv[i];
+            for (int p = row[u] + k + lane(); p < row[u + 1]; p += MG_WAVE) CCHooks::fused_update_condition(parent, u, col[p]);
+        }
+        __syncthreads();
     }
 }
 
@@ -288,7 +344,6 @@ __global__ void k_pr_ctl(Ctl *ctl, int max_iter, long long ne) {
     c.levels = c.iter;
     if (!c.changed || (max_iter > 0 && c.iter >= max_iter)) c.mode = MODE_DONE;
     else { c.iter += 1; c.changed = 0; c.mf = 0; }
-    c.pad0 = 0;
     *ctl = c;
 }
 
@@ -345,19 +400,20 @@ int mg_cc_build(mg_engine *e, const mg_graph *g) {
         int z = 0;
         MG_HIP(hipMemcpy(e->urow, &z, sizeof(int), hipMemcpyHostToDevice));
     }
+    // chunk owners of the upper-triangular edge list (static: computed once)
+    const int nch = (int)((ne_u + kEPB - 1) / kEPB);
+    e->chunk_cap = nch;
+    MG_HIP(hipMalloc(&e->cstart, sizeof(int) * (nch + 2)));
+    if (nch > 0) hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(nch + 1, 256, 4096)), dim3(256), 0, s, nv, ne_u, e->urow, nch, e->cstart);
     MG_HIP(hipStreamSynchronize(s));
     MG_HIP(hipGetLastError());
-    std::vector<int32_t> urow((size_t)nv + 1);
-    MG_HIP(hipMemcpy(urow.data(), e->urow, sizeof(int) * ((size_t)nv + 1), hipMemcpyDeviceToHost));
     hipFree(cnt);
     hipFree(uoff);
     MG_HIP(hipMalloc(&e->level, sizeof(int) * ((long long)nv + 64)));
+    MG_HIP(hipMalloc(&e->aux, sizeof(int) * 16));
     MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
     MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
-    int nch = 0;
-    MG_TRY(build_huge_chunks(e, urow, nv, &e->chunks, &nch));
-    e->chunk_cap = nch;
-    (void)g;
+    e->cc_afforest = g->symmetric && nv >= 1024;
     return MG_OK;
 }
 
@@ -370,22 +426,49 @@ int mg_cc_reset(mg_engine *e) {
     return MG_OK;
 }
 
+// One call does all the work (hook + compress), so MultiGraph_E_E-style loops end
+// after their first call.  With a symmetric CSR the sampling pass runs first: link
+// every vertex to its smallest neighbour (no atomics), compress, hook its second
+// neighbour, compress, take the most frequent root of 1024 samples as the
+// giant tree, and hook the remaining edges only of vertices outside it; every edge
+// with an endpoint outside the giant tree is hooked from that endpoint's row, and an
+// edge with both endpoints inside joins vertices already in one tree.
 int mg_cc_step(mg_engine *e, int *more) {
     hipStream_t s = e->stream;
     const int G = e->opt.grid_blocks > 0 ? e->opt.grid_blocks : 2048;
+    const int nv = e->nv;
     int slot = -1;
-    if (e->prof) mg_prof_begin(e, "k_cc_hook", 4.0 * e->ne_u + 4.0 * (e->nv + 1) + 8.0 * e->ne_u, &slot);
-    hipLaunchKernelGGL(k_cc_hook, dim3(grid_for(e->nv, kBlock, G)), dim3(kBlock), 0, s, e->nv, e->opt.huge_degree, e->urow,
-                       e->ucol, e->level);
-    if (e->prof) mg_prof_end(e, slot);
-    if (e->chunk_cap > 0) {
-        if (e->prof) mg_prof_begin(e, "k_cc_hook_huge", 0.0, &slot);
-        hipLaunchKernelGGL(k_cc_hook_huge, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, e->chunks, (int)e->chunk_cap,
-                           e->urow, e->ucol, e->level);
+    bool hooked = false;
+    if (e->cc_afforest) {
+        const int k = 2;
+        if (e->prof) mg_prof_begin(e, "k_cc_link_min", 16.0 * nv, &slot);
+        hipLaunchKernelGGL(k_cc_link_min, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->row, e->col, e->level);
+        if (e->prof) mg_prof_end(e, slot);
+        hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->level);
+        if (e->prof) mg_prof_begin(e, "k_cc_link_second", 16.0 * nv, &slot);
+        hipLaunchKernelGGL(k_cc_link_second, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->row, e->col, e->level);
+        if (e->prof) mg_prof_end(e, slot);
+        hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->level);
+        hipLaunchKernelGGL(k_cc_sample, dim3(1), dim3(1024), 0, s, nv, e->level, e->aux);
+        int best[2] = {0, 0};
+        MG_HIP(hipMemcpyAsync(best, e->aux, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        MG_HIP(hipStreamSynchronize(s));
+        e->cc_giant_frac = best[1] / 1024.0;
+        if (best[1] >= 512) {
+            if (e->prof) mg_prof_begin(e, "k_cc_link_rest", 8.0 * nv, &slot);
+            hipLaunchKernelGGL(k_cc_link_rest, dim3(grid_for(nv, kBlock, G)), dim3(kBlock), 0, s, nv, k, e->row, e->col, e->aux, e->level);
+            if (e->prof) mg_prof_end(e, slot);
+            hooked = true;
+        }
+    }
+    if (!hooked && e->chunk_cap > 0) {
+        if (e->prof) mg_prof_begin(e, "k_cc_hook_lb", 4.0 * e->ne_u + 4.0 * (nv + 1) + 8.0 * e->ne_u, &slot);
+        hipLaunchKernelGGL(k_cc_hook_lb, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, (int)e->chunk_cap, e->ne_u,
+                           e->urow, e->ucol, e->cstart, e->level);
         if (e->prof) mg_prof_end(e, slot);
     }
-    if (e->prof) mg_prof_begin(e, "k_cc_compress", 8.0 * e->nv, &slot);
-    hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(e->nv, 256, 8192)), dim3(256), 0, s, e->nv, e->level);
+    if (e->prof) mg_prof_begin(e, "k_cc_compress", 8.0 * nv, &slot);
+    hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->level);
     if (e->prof) mg_prof_end(e, slot);
     MG_HIP(hipGetLastError());
     MG_HIP(hipStreamSynchronize(s));

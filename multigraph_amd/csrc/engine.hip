@@ -52,7 +52,7 @@ static void release_all(mg_engine *e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) hipGraphExecDestroy(e->gexec);
-    void *ptrs[] = {e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->q, e->bm[0], e->bm[1],
+    void *ptrs[] = {e->part_n, e->part_mf, e->eoff, e->tsum, e->cstart, e->pcstart, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->q, e->bm[0], e->bm[1],
                     e->vis, e->chunks, e->lvl_off, e->aux, e->ctl, e->urow, e->ucol, e->rank, e->cval, e->deg, e->psum,
                     e->pb.a_src, e->pb.b_dst, e->pb.amap, e->pb.msg, e->pb.a_part, e->pb.b_part, e->pb.unitsA,
                     e->pb.unitsB, e->pb.split_parts, e->pb.unitsA64, e->pb.unitsB64};

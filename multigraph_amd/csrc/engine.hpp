@@ -26,6 +26,9 @@ enum { MODE_TD = 0, MODE_BU = 1, MODE_DONE = 2 };
 
 constexpr int kBlock = 256;          // threads per workgroup of the frontier kernels
 constexpr int kHugeChunk = 4096;     // edges per chunk of a split high-degree vertex
+constexpr int kTile = 1024;          // frontier entries per degree-scan tile
+constexpr int kEPB = 2048;           // edges per workgroup chunk of the edge-balanced push
+constexpr long long kSmallMF = 16384;  // frontiers with fewer edges use the single-pass push
 
 // Device-resident control block.  One small kernel advances it between levels, so
 // a sequence of levels needs no host round trip and can be captured in a hipGraph;
@@ -43,7 +46,11 @@ struct Ctl {
     int bm_cur;      // bitmap slot holding the current frontier
     int levels;      // levels produced so far (BC log length)
     int changed;     // dense-sweep "something changed" flag
-    int pad0;
+    int lb;          // 1: the push level uses the edge-balanced kernels
+    int lb_total;    // edges of the current frontier (edge-balanced path)
+    int lb_chunks;   // edge chunks of the current frontier
+    int need_queue;  // 1: build the new frontier's queue from its bitmap (after a pull level)
+    int qsrc;        // bitmap slot holding the frontier produced by the last pull level
     long long out_mf;  // degree sum of the appended vertices
     long long mf;      // degree sum of the current frontier
     long long mu;      // degree sum of unexplored vertices
@@ -53,6 +60,8 @@ struct Ctl {
 
 struct Params {
     int nv;
+    int sssp;        // 1: SSSP, where the dense level is an atomic-free pull relaxation
+    int need_log;    // 1: every level's vertices are kept in the queue log (BC)
     int log_queue;   // 1: frontiers appended to one log (BFS/BC); 0: ping-pong halves (SSSP)
     int forced;      // MG_MODE_*
     int huge;        // split threshold
@@ -78,9 +87,14 @@ __host__ __device__ inline void advance(Ctl &c, const Params &p) {
     int next;
     if (p.forced == MG_MODE_SPARSE) next = MODE_TD;
     else if (p.forced == MG_MODE_DENSE) next = MODE_BU;
+    else if (p.sssp) next = ((double)c.out_mf > (double)p.ne / (double)p.alpha) ? MODE_BU : MODE_TD;
     else if (prev == MODE_TD) next = ((double)c.out_mf > (double)c.mu / (double)p.alpha) ? MODE_BU : MODE_TD;
     else next = ((double)produced < (double)p.nv / (double)p.beta) ? MODE_TD : MODE_BU;
     c.convert = (next == MODE_BU && prev != MODE_BU) ? 1 : 0;
+    // a pull level leaves its frontier as a bitmap (slot bm_cur ^ 1, or 0 for SSSP);
+    // a queue is built from it when the next level pushes or when levels are logged
+    c.qsrc = p.sssp ? 0 : (c.bm_cur ^ 1);
+    c.need_queue = (prev == MODE_BU && (p.need_log || next == MODE_TD)) ? 1 : 0;
     if (prev == MODE_BU && next == MODE_BU) c.bm_cur ^= 1;
     c.in_begin = c.out_begin;
     c.in_n = produced;
@@ -90,6 +104,7 @@ __host__ __device__ inline void advance(Ctl &c, const Params &p) {
     c.out_n = 0;
     c.out_mf = 0;
     c.nchunks = 0;
+    c.lb = c.mf > kSmallMF ? 1 : 0;
     c.iter += 1;
     c.levels = c.iter;
     c.mode = next;
@@ -138,6 +153,13 @@ struct mg_engine {
     uint64_t *vis = nullptr;
     mg::HugeChunk *chunks = nullptr;
     long long chunk_cap = 0;
+    int *eoff = nullptr;       // frontier degree scan (tile-local exclusive offsets)
+    int *tsum = nullptr;       // per-tile degree sums, scanned in place
+    int *cstart = nullptr;     // first frontier entry of every edge chunk
+    int *pcstart = nullptr;    // first row of every chunk of the whole CSR (SSSP pull)
+    int pnch = 0;
+    int *part_n = nullptr;     // per-workgroup partial counts of a pull level
+    long long *part_mf = nullptr;
     int *lvl_off = nullptr;    // BC level log offsets (levels + 2)
     int *aux = nullptr;        // small device counters (BC split-vertex chunk count)
     mg::Ctl *ctl = nullptr;
@@ -146,6 +168,8 @@ struct mg_engine {
     // CC: upper-triangular CSR
     int *urow = nullptr, *ucol = nullptr;
     long long ne_u = 0;
+    bool cc_afforest = false;
+    double cc_giant_frac = 0.0;
     // PageRank
     float *rank = nullptr, *cval = nullptr;
     int *deg = nullptr;

@@ -2,10 +2,12 @@
 // the BC backward pass (the role of process2.cu), hand-written for gfx950.
 //
 // Per level the engine runs
-//   * a push level (sparse queue): expand + contract, one workgroup per 256 frontier
-//     vertices with a block-wide exclusive scan of their degrees so every lane takes
-//     edges of the block in turn (edge-balanced regardless of degree); vertices at or
-//     above `huge` edges are cut into 4096-edge chunks that the whole grid shares;
+//   * a push level (sparse queue): expand + contract.  Small frontiers (at most
+//     16K edges) go through one kernel, one workgroup per 256 frontier vertices with a
+//     block-wide exclusive scan of their degrees.  Larger frontiers are edge-balanced
+//     across the whole grid: a tiled scan of the frontier's degrees, the owner of the
+//     first edge of every 2048-edge chunk, then one workgroup per chunk, so a hub's
+//     edges spread over many workgroups and no workgroup owns more than 2048 edges;
 //     winners are appended with one atomic per wave (ballot + mbcnt);
 //   * or a pull level (dense bitmap): one wave per 64 vertices, the frontier bitmap
 //     word of a vertex's neighbour is tested per edge, and the wave's 64 results are
@@ -17,6 +19,7 @@
 // so a run of levels can be captured in a hipGraph and replayed with no host sync.
 #include "engine.hpp"
 #include "user_function.hpp"
+#include "lb.hpp"
 
 using namespace mg;
 
@@ -39,7 +42,8 @@ __global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int
     q[0] = src;
     Ctl c;
     c.iter = 1; c.mode = MODE_TD; c.in_begin = 0; c.in_n = 1; c.out_begin = out_begin; c.out_n = 0;
-    c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0; c.pad0 = 0;
+    c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0;
+    c.lb = deg_src > kSmallMF ? 1 : 0; c.lb_total = 0; c.lb_chunks = 0; c.need_queue = 0; c.qsrc = 0;
     c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0;
     *ctl = c;
 }
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
                                                  int *__restrict__ state, int *__restrict__ vfc,
                                                  double *__restrict__ sigma, int *__restrict__ q,
                                                  HugeChunk *__restrict__ chunks) {
-    if (ctl->mode != MODE_TD) return;
+    if (ctl->mode != MODE_TD || ctl->lb) return;
     const int iter = ctl->iter, in_begin = ctl->in_begin, n = ctl->in_n, out_begin = ctl->out_begin;
     __shared__ int s_off[kBlock];
     __shared__ int s_rb[kBlock];
@@ -123,12 +127,6 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
             deg = row[v + 1] - rb;
             val = state[v];
             if (sigma) val2 = sigma[v];
-            if (deg >= P.huge) {
-                int nch = (deg + kHugeChunk - 1) / kHugeChunk;
-                int c0 = atomicAdd(&ctl->nchunks, nch);
-                for (int j = 0; j < nch; ++j) chunks[c0 + j] = HugeChunk{v, j};
-                deg = 0;
-            }
         }
         int total;
         int off = block_excl_scan<kBlock / MG_WAVE>(deg, s_tmp, &total);
@@ -156,32 +154,136 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
     }
 }
 
-// Push level, split high-degree vertices: grid-stride over 4096-edge chunks.
-template <class H, bool kTrackDeg>
-__global__ __launch_bounds__(kBlock) void k_push_huge(Ctl *ctl, const int *__restrict__ row, const int *__restrict__ col,
-                                                      const int *__restrict__ wgt, int *__restrict__ state,
-                                                      int *__restrict__ vfc, double *__restrict__ sigma,
-                                                      int *__restrict__ q, const HugeChunk *__restrict__ chunks) {
-    if (ctl->mode != MODE_TD) return;
-    const int nch = ctl->nchunks;
-    const int iter = ctl->iter, out_begin = ctl->out_begin;
-    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
-        const HugeChunk hc = chunks[c];
-        const int v = hc.v;
-        const int vb = row[v], ve = row[v + 1];
-        const int b = vb + hc.j * kHugeChunk;
-        const int e = min(ve, b + kHugeChunk);
-        const int val = state[v];
-        const double val2 = sigma ? sigma[v] : 0.0;
-        for (int p0 = b; p0 < e; p0 += kBlock) {
-            const int p = p0 + threadIdx.x;
-            int win = 0, u = 0;
-            if (p < e) {
-                u = ld_stream(&col[p]);
-                win = Relax<H>::go(val, val2, u, p, wgt, state, vfc, sigma, iter);
-            }
-            append<kTrackDeg>(ctl, q, out_begin, row, win != 0, u);
+// ---- edge-balanced push (frontiers above kSmallMF edges) --------------------------
+// 1. degrees of the frontier, exclusive scan inside tiles of kTile entries
+__global__ __launch_bounds__(kBlock) void k_lb_deg(Ctl *ctl, const int *__restrict__ row, const int *__restrict__ q,
+                                                   int *__restrict__ eoff, int *__restrict__ tsum) {
+    if (ctl->mode != MODE_TD || !ctl->lb) return;
+    const int n = ctl->in_n, in_begin = ctl->in_begin;
+    __shared__ int s_tmp[kBlock / MG_WAVE + 1];
+    constexpr int kPer = kTile / kBlock;
+    for (int t = blockIdx.x; t * kTile < n; t += gridDim.x) {
+        const int i0 = t * kTile + threadIdx.x * kPer;
+        int d[kPer];
+        int local = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = i0 + k;
+            d[k] = 0;
+            if (i < n) { const int v = q[in_begin + i]; d[k] = row[v + 1] - row[v]; }
+            local += d[k];
         }
+        int total;
+        int pre = block_excl_scan<kBlock / MG_WAVE>(local, s_tmp, &total);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (i0 + k < n) eoff[i0 + k] = pre;
+            pre += d[k];
+        }
+        if (threadIdx.x == 0) tsum[t] = total;
+        __syncthreads();
+    }
+}
+
+// 2. one workgroup scans the tile sums in place and sizes the chunk grid
+__global__ __launch_bounds__(1024) void k_lb_tscan(Ctl *ctl, int *__restrict__ tsum) {
+    if (ctl->mode != MODE_TD || !ctl->lb) return;
+    const int T = (ctl->in_n + kTile - 1) / kTile;
+    __shared__ int s_tmp[1024 / MG_WAVE + 1];
+    __shared__ int s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int b = 0; b < T; b += 1024) {
+        const int i = b + threadIdx.x;
+        const int v = i < T ? tsum[i] : 0;
+        int total;
+        const int pre = block_excl_scan<1024 / MG_WAVE>(v, s_tmp, &total);
+        const int carry = s_carry;
+        if (i < T) tsum[i] = carry + pre;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = carry + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctl->lb_total = s_carry;
+        ctl->lb_chunks = (s_carry + kEPB - 1) / kEPB;
+    }
+}
+
+// global exclusive edge offset of frontier entry i
+__device__ __forceinline__ int lb_off(const int *eoff, const int *toff, int i) { return eoff[i] + toff[i / kTile]; }
+
+// largest frontier entry i with lb_off(i) <= e (the owner of virtual edge e)
+__device__ int lb_owner(const int *eoff, const int *toff, int n, int e) {
+    const int T = (n + kTile - 1) / kTile;
+    int lo = 0, hi = T - 1;
+    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (toff[mid] <= e) lo = mid; else hi = mid - 1; }
+    const int t = lo, base = toff[t];
+    int a = t * kTile, b = min(n, (t + 1) * kTile) - 1;
+    while (a < b) { const int mid = (a + b + 1) >> 1; if (base + eoff[mid] <= e) a = mid; else b = mid - 1; }
+    return a;
+}
+
+// 3. owner of the first edge of every chunk (cstart[chunks] = owner of the last edge)
+__global__ void k_lb_part(Ctl *ctl, const int *__restrict__ eoff, const int *__restrict__ toff, int *__restrict__ cstart) {
+    if (ctl->mode != MODE_TD || !ctl->lb) return;
+    const int n = ctl->in_n, nch = ctl->lb_chunks, total = ctl->lb_total;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= nch; c += gridDim.x * blockDim.x) {
+        const int e = c < nch ? c * kEPB : total - 1;
+        cstart[c] = lb_owner(eoff, toff, n, e);
+    }
+}
+
+// 4. the push: each workgroup takes kEPB consecutive edges of the frontier's edge list
+template <class H>
+__global__ __launch_bounds__(kBlock) void k_push_lb(Ctl *ctl, const int *__restrict__ row, const int *__restrict__ col,
+                                                    const int *__restrict__ wgt, int *__restrict__ state,
+                                                    int *__restrict__ vfc, double *__restrict__ sigma,
+                                                    int *__restrict__ q, const int *__restrict__ eoff,
+                                                    const int *__restrict__ toff, const int *__restrict__ cstart) {
+    if (ctl->mode != MODE_TD || !ctl->lb) return;
+    const int iter = ctl->iter, in_begin = ctl->in_begin, n = ctl->in_n, out_begin = ctl->out_begin;
+    const int nch = ctl->lb_chunks, total = ctl->lb_total;
+    constexpr int kCap = kEPB + 1;
+    __shared__ int s_off[kCap];
+    __shared__ int s_rb[kCap];
+    __shared__ int s_v[kCap];
+    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+        const int i0 = cstart[c];
+        const int i1 = cstart[c + 1];
+        const int cnt = i1 - i0 + 1;
+        const bool staged = cnt <= kCap;
+        if (staged) {
+            for (int k = threadIdx.x; k < cnt; k += kBlock) {
+                const int i = i0 + k;
+                const int v = q[in_begin + i];
+                s_off[k] = lb_off(eoff, toff, i);
+                s_rb[k] = row[v];
+                s_v[k] = v;
+            }
+        }
+        __syncthreads();
+        const int e0 = c * kEPB, e1 = min(total, e0 + kEPB);
+        for (int eb = e0; eb < e1; eb += kBlock) {
+            const int e = eb + threadIdx.x;
+            int win = 0, u = 0;
+            if (e < e1) {
+                int k, off, rb, v;
+                if (staged) {
+                    int lo = 0, hi = cnt - 1;
+                    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_off[mid] <= e) lo = mid; else hi = mid - 1; }
+                    k = lo; off = s_off[k]; rb = s_rb[k]; v = s_v[k];
+                } else {  // only reachable with zero-degree frontier entries
+                    const int i = lb_owner(eoff, toff, n, e);
+                    v = q[in_begin + i]; off = lb_off(eoff, toff, i); rb = row[v];
+                }
+                const int p = rb + (e - off);
+                u = ld_stream(&col[p]);
+                win = Relax<H>::go(state[v], sigma ? sigma[v] : 0.0, u, p, wgt, state, vfc, sigma, iter);
+            }
+            append<true>(ctl, q, out_begin, row, win != 0, u);
+        }
+        __syncthreads();
     }
 }
 
@@ -208,26 +310,32 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 
 // Pull level for BFS (early exit at the first parent) and for the BC forward pass
 // (all parents, summing their path counts).
+// The new frontier is the bitmap itself; its size and degree sum leave as one
+// partial per workgroup (no global atomics), summed by k_reduce_parts.
 template <bool kBC>
 __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__restrict__ row,
                                                  const int *__restrict__ col, int *__restrict__ level,
                                                  double *__restrict__ sigma, uint64_t *__restrict__ vis,
                                                  uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
-                                                 int *__restrict__ q) {
+                                                 int *__restrict__ part_n, long long *__restrict__ part_mf) {
     if (ctl->mode != MODE_BU) return;
-    const int iter = ctl->iter, out_begin = ctl->out_begin;
+    const int iter = ctl->iter;
     const uint64_t *fr = ctl->bm_cur ? bm1 : bm0;
     uint64_t *nx = ctl->bm_cur ? bm0 : bm1;
     const int words = (nv + 63) >> 6;
     const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * kBlock) >> 6;
+    int cnt = 0;
+    long long mf = 0;
     for (int w = wave; w < words; w += nwaves) {
         const uint64_t vw = vis[w];
         const int v = (w << 6) + lane();
         bool found = false;
         double s = 0.0;
+        int b = 0, e = 0;
         if (vw != ~0ull && !((vw >> lane()) & 1ull)) {
-            const int b = row[v], e = row[v + 1];
+            b = row[v];
+            e = row[v + 1];
             for (int p = b; p < e; ++p) {
                 const int u = ld_stream(&col[p]);
                 if (bm_test(fr, u)) {
@@ -242,9 +350,163 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
         if (found) {
             level[v] = iter;
             if (kBC) sigma[v] = s;
+            cnt += 1;
+            mf += e - b;
         }
-        append<true>(ctl, q, out_begin, row, found, v);
     }
+    __shared__ long long s_tmp[kBlock / MG_WAVE];
+    const long long tn = block_sum<kBlock / MG_WAVE>((long long)cnt, s_tmp);
+    const long long tm = block_sum<kBlock / MG_WAVE>(mf, s_tmp);
+    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_mf[blockIdx.x] = tm; }
+}
+
+// sum the per-workgroup partials of a pull level into the control block
+__global__ __launch_bounds__(256) void k_reduce_parts(Ctl *ctl, const int *__restrict__ part_n,
+                                                      const long long *__restrict__ part_mf, int nparts) {
+    if (ctl->mode != MODE_BU) return;
+    __shared__ long long s_tmp[256 / MG_WAVE];
+    long long n = 0, m = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) { n += part_n[i]; m += part_mf[i]; }
+    n = block_sum<256 / MG_WAVE>(n, s_tmp);
+    m = block_sum<256 / MG_WAVE>(m, s_tmp);
+    if (threadIdx.x == 0) { ctl->out_n = (int)n; ctl->out_mf = m; }
+}
+
+// Queue of the frontier bitmap bm[qsrc] (after a pull level), in vertex order, at
+// q[in_begin]: popcounts per workgroup range, one-workgroup scan, ordered write.
+__global__ __launch_bounds__(kBlock) void k_bmq_count(Ctl *ctl, int words, const uint64_t *__restrict__ bm0,
+                                                      const uint64_t *__restrict__ bm1, int *__restrict__ part) {
+    if (!ctl->need_queue) return;
+    const uint64_t *bm = ctl->qsrc ? bm1 : bm0;
+    const int per = (words + gridDim.x - 1) / gridDim.x;
+    const int w0 = blockIdx.x * per, w1 = min(words, w0 + per);
+    long long c = 0;
+    for (int w = w0 + threadIdx.x; w < w1; w += kBlock) c += __builtin_popcountll(bm[w]);
+    __shared__ long long s_tmp[kBlock / MG_WAVE];
+    c = block_sum<kBlock / MG_WAVE>(c, s_tmp);
+    if (threadIdx.x == 0) part[blockIdx.x] = (int)c;
+}
+
+__global__ __launch_bounds__(1024) void k_bmq_scan(Ctl *ctl, int *__restrict__ part, int nparts) {
+    if (!ctl->need_queue) return;
+    __shared__ int s_tmp[1024 / MG_WAVE + 1];
+    __shared__ int s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int b = 0; b < nparts; b += 1024) {
+        const int i = b + threadIdx.x;
+        const int v = i < nparts ? part[i] : 0;
+        int total;
+        const int pre = block_excl_scan<1024 / MG_WAVE>(v, s_tmp, &total);
+        const int carry = s_carry;
+        if (i < nparts) part[i] = carry + pre;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = carry + total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bmq_write(Ctl *ctl, int words, const uint64_t *__restrict__ bm0,
+                                                      const uint64_t *__restrict__ bm1, const int *__restrict__ part,
+                                                      int *__restrict__ q) {
+    if (!ctl->need_queue) return;
+    const uint64_t *bm = ctl->qsrc ? bm1 : bm0;
+    const int per = (words + gridDim.x - 1) / gridDim.x;
+    const int w0 = blockIdx.x * per, w1 = min(words, w0 + per);
+    int base = ctl->in_begin + part[blockIdx.x];
+    __shared__ int s_tmp[kBlock / MG_WAVE + 1];
+    for (int wb = w0; wb < w1; wb += kBlock) {
+        const int w = wb + threadIdx.x;
+        uint64_t m = w < w1 ? bm[w] : 0ull;
+        int total;
+        int pos = base + block_excl_scan<kBlock / MG_WAVE>((int)__builtin_popcountll(m), s_tmp, &total);
+        while (m) {
+            const int bit = __builtin_ctzll(m);
+            q[pos++] = (w << 6) + bit;
+            m &= m - 1;
+        }
+        base += total;
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------------- SSSP pull
+// Dense SSSP level without global atomics: every vertex takes the minimum of
+// dist[u] + w over neighbours u that changed in the previous iteration (frontier
+// bitmap), edge-balanced over the whole CSR.  Minima are reduced per owner in LDS;
+// an owner whose row lies inside the chunk is written with a plain store, the (at
+// most two) owners that straddle a chunk boundary with atomicMin.  This is the
+// reference's dense all-edge relaxation (process.cu:653-934) without its message
+// buffers: on a multi-die GPU device-scope atomics resolve beyond the die's L2, so a
+// push over a large frontier pays one such atomic per improving edge.
+__global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long long ne, const int *__restrict__ row,
+                                                      const int *__restrict__ col, const int *__restrict__ wgt,
+                                                      const int *__restrict__ cstart, const uint64_t *__restrict__ bm,
+                                                      int *__restrict__ dist, int *__restrict__ vfc) {
+    if (ctl->mode != MODE_BU) return;
+    const int iter = ctl->iter;
+    __shared__ int s_own[kEPB];
+    __shared__ int s_best[kEPB + 1];
+    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+        const int e0 = c * kEPB;
+        const int e1 = (int)min((long long)e0 + kEPB, ne);
+        const int u0 = cstart[c], u1 = cstart[c + 1];
+        const int nown = u1 - u0 + 1;
+        const bool fits = nown <= kEPB + 1;
+        for (int k = threadIdx.x; k < kEPB + 1; k += kBlock) s_best[k] = MG_INF;
+        chunk_owners(row, u0, u1, e0, e1, s_own);
+        for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
+            const int u = ld_stream(&col[e]);
+            if ((bm[u >> 6] >> (u & 63)) & 1ull) {
+                const int cand = dist[u] + ld_stream(&wgt[e]);
+                const int o = s_own[e - e0];
+                if (fits) atomicMin(&s_best[o - u0], cand);
+                else if (atomicMin(&dist[o], cand) > cand) vfc[o] = iter;
+            }
+        }
+        __syncthreads();
+        if (fits) {
+            for (int k = threadIdx.x; k < nown; k += kBlock) {
+                const int best = s_best[k];
+                const int o = u0 + k;
+                if (best == MG_INF || best >= dist[o]) continue;
+                if (row[o] < e0 || row[o + 1] > e1) {
+                    if (atomicMin(&dist[o], best) > best) vfc[o] = iter;
+                } else {
+                    dist[o] = best;
+                    vfc[o] = iter;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Frontier of stamp `iter + offset` from the stamps: a bitmap word per wave
+// (ballot) and per-workgroup partial counts (size, degree sum) for k_reduce_parts.
+// gate 0: always; 1: only on a pull level; 2: only right after a switch to pull
+__global__ __launch_bounds__(kBlock) void k_sssp_scan(Ctl *ctl, int nv, int offset, int gate, const int *__restrict__ vfc,
+                                                      const int *__restrict__ row, uint64_t *__restrict__ bm,
+                                                      int *__restrict__ part_n, long long *__restrict__ part_mf) {
+    if (gate >= 1 && ctl->mode != MODE_BU) return;
+    if (gate == 2 && !ctl->convert) return;
+    const int stamp = ctl->iter + offset;
+    const int words = (nv + 63) >> 6;
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * kBlock) >> 6;
+    int cnt = 0;
+    long long mf = 0;
+    for (int w = wave; w < words; w += nwaves) {
+        const int v = (w << 6) + lane();
+        const bool hit = v < nv && vfc[v] == stamp;
+        const uint64_t m = ballot(hit);
+        if (lane() == 0) bm[w] = m;
+        if (hit) { cnt += 1; mf += row[v + 1] - row[v]; }
+    }
+    __shared__ long long s_tmp[kBlock / MG_WAVE];
+    const long long tn = block_sum<kBlock / MG_WAVE>((long long)cnt, s_tmp);
+    const long long tm = block_sum<kBlock / MG_WAVE>(mf, s_tmp);
+    if (threadIdx.x == 0 && part_n) { part_n[blockIdx.x] = (int)tn; part_mf[blockIdx.x] = tm; }
 }
 
 // -------------------------------------------------------------------- BC backward
@@ -373,11 +635,23 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMalloc(&e->q, sizeof(int) * e->qcap));
     e->chunk_cap = 2 * (ne / kHugeChunk) + nv / 1024 + 64;
     MG_HIP(hipMalloc(&e->chunks, sizeof(HugeChunk) * e->chunk_cap));
+    MG_HIP(hipMalloc(&e->eoff, sizeof(int) * e->qcap));
+    MG_HIP(hipMalloc(&e->tsum, sizeof(int) * (e->qcap / kTile + 2)));
+    MG_HIP(hipMalloc(&e->cstart, sizeof(int) * (ne / kEPB + 4)));
+    MG_HIP(hipMalloc(&e->part_n, sizeof(int) * (max_grid(e) + 8)));
+    MG_HIP(hipMalloc(&e->part_mf, sizeof(long long) * (max_grid(e) + 8)));
     MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
     MG_HIP(hipMalloc(&e->aux, sizeof(int) * 16));
     MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
     if (e->algo == MG_SSSP) {
         MG_HIP(hipMalloc(&e->vfc, sizeof(int) * padv));
+        const long long words = (nv + 63) / 64 + 1;
+        MG_HIP(hipMalloc(&e->bm[0], sizeof(uint64_t) * words));
+        e->pnch = (int)((ne + kEPB - 1) / kEPB);
+        MG_HIP(hipMalloc(&e->pcstart, sizeof(int) * (e->pnch + 2)));
+        if (e->pnch > 0)
+            hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(e->pnch + 1, 256, 4096)), dim3(256), 0, e->stream, nv, ne, e->row,
+                               e->pnch, e->pcstart);
     } else {
         const long long words = (nv + 63) / 64 + 1;
         MG_HIP(hipMalloc(&e->bm[0], sizeof(uint64_t) * words));
@@ -395,7 +669,9 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     P.nv = nv;
     P.ne = ne;
     P.log_queue = (e->algo == MG_SSSP) ? 0 : 1;
-    P.forced = (e->algo == MG_SSSP) ? MG_MODE_SPARSE : e->opt.mode;
+    P.forced = e->opt.mode;
+    P.sssp = e->algo == MG_SSSP ? 1 : 0;
+    P.need_log = e->algo == MG_BC ? 1 : 0;
     P.huge = e->opt.huge_degree;
     P.alpha = e->opt.alpha;
     P.beta = e->opt.beta;
@@ -435,6 +711,8 @@ int mg_frontier_reset(mg_engine *e, int source) {
     return MG_OK;
 }
 
+static int launch_after_ctl(mg_engine *e);
+
 // launch the kernels of one level; `known` = host knows the control block (eager mode)
 static int launch_level(mg_engine *e, bool known) {
     const Ctl &c = *e->h_ctl;
@@ -442,59 +720,103 @@ static int launch_level(mg_engine *e, bool known) {
     const int G = max_grid(e);
     const int nv = e->nv;
     const bool td = !known || c.mode == MODE_TD;
-    const bool bu = !known || c.mode == MODE_BU;
+    bool bu = !known || c.mode == MODE_BU;
     int slot = -1;
-    if (bu && (!known || c.convert)) {
+    if (bu && e->algo != MG_SSSP && (!known || c.convert)) {
         const int gb = grid_for((nv + 63) / 64, kBlock / MG_WAVE, G);
         if (e->prof) mg_prof_begin(e, "k_level2bm", 4.0 * nv + nv / 4.0, &slot);
         hipLaunchKernelGGL(k_level2bm, dim3(gb), dim3(kBlock), 0, s, e->ctl, nv, e->level, e->vis, e->bm[0], e->bm[1]);
         if (e->prof) mg_prof_end(e, slot);
     }
     if (td) {
-        const int gt = known ? grid_for(c.in_n, kBlock, G) : G;
-        // algorithmic bytes: queue + row pair per frontier vertex, column + state per edge
-        const double bytes = known ? (12.0 * c.in_n + 8.0 * (double)c.mf + (e->algo == MG_SSSP ? 4.0 * c.mf : 0.0)) : 0.0;
-        if (e->prof) mg_prof_begin(e, "k_push", bytes, &slot);
-        switch (e->algo) {
-        case MG_BFS:
-            hipLaunchKernelGGL((k_push<BFSHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
-            break;
-        case MG_SSSP:
-            hipLaunchKernelGGL((k_push<SSSPHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
-            break;
-        default:
-            hipLaunchKernelGGL((k_push<BCHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
-            break;
-        }
-        if (e->prof) mg_prof_end(e, slot);
-        const int gh = known ? grid_for((long long)(c.mf / kHugeChunk) + 1, 1, G) : G;
-        if (!known || c.mf >= e->P.huge) {
-            if (e->prof) mg_prof_begin(e, "k_push_huge", 0.0, &slot);
+        // algorithmic bytes: queue entry, row pair and own state per frontier vertex;
+        // column and neighbour state per edge (+ weight for SSSP)
+        const double per_edge = e->algo == MG_SSSP ? 12.0 : 8.0;
+        const double bytes = known ? (16.0 * c.in_n + per_edge * (double)c.mf) : 0.0;
+        if (!known || !c.lb) {
+            const int gt = known ? grid_for(c.in_n, kBlock, G) : G;
+            if (e->prof) mg_prof_begin(e, "k_push", bytes, &slot);
             switch (e->algo) {
             case MG_BFS:
-                hipLaunchKernelGGL((k_push_huge<BFSHooks, true>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                hipLaunchKernelGGL((k_push<BFSHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
                 break;
             case MG_SSSP:
-                hipLaunchKernelGGL((k_push_huge<SSSPHooks, true>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                hipLaunchKernelGGL((k_push<SSSPHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
                 break;
             default:
-                hipLaunchKernelGGL((k_push_huge<BCHooks, true>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                hipLaunchKernelGGL((k_push<BCHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                break;
+            }
+            if (e->prof) mg_prof_end(e, slot);
+        }
+        if (!known || c.lb) {
+            const long long maxch = known ? (c.mf + kEPB - 1) / kEPB : (e->ne + kEPB - 1) / kEPB;
+            const int gd = known ? grid_for(c.in_n, kTile, G) : G;
+            if (e->prof) mg_prof_begin(e, "k_lb_scan", 12.0 * c.in_n, &slot);
+            hipLaunchKernelGGL(k_lb_deg, dim3(gd), dim3(kBlock), 0, s, e->ctl, e->row, e->q, e->eoff, e->tsum);
+            hipLaunchKernelGGL(k_lb_tscan, dim3(1), dim3(1024), 0, s, e->ctl, e->tsum);
+            hipLaunchKernelGGL(k_lb_part, dim3(grid_for(maxch + 1, 256, G)), dim3(256), 0, s, e->ctl, e->eoff, e->tsum, e->cstart);
+            if (e->prof) mg_prof_end(e, slot);
+            const int gp = known ? grid_for(maxch, 1, G) : G;
+            if (e->prof) mg_prof_begin(e, "k_push_lb", bytes, &slot);
+            switch (e->algo) {
+            case MG_BFS:
+                hipLaunchKernelGGL((k_push_lb<BFSHooks>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                break;
+            case MG_SSSP:
+                hipLaunchKernelGGL((k_push_lb<SSSPHooks>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                break;
+            default:
+                hipLaunchKernelGGL((k_push_lb<BCHooks>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
                 break;
             }
             if (e->prof) mg_prof_end(e, slot);
         }
     }
+    const int words = (nv + 63) / 64;
+    const int gsc = grid_for(words, kBlock / MG_WAVE, G);  // pull / scan grid = number of partials
+    if (bu && e->algo == MG_SSSP) {
+        // col + weight streamed per edge, bitmap per edge, dist + stamp per vertex
+        const double bytes = 8.0 * (double)e->ne + 4.0 * (nv + 1) + 8.0 * nv;
+        if (e->prof) mg_prof_begin(e, "k_pull_sssp", bytes, &slot);
+        hipLaunchKernelGGL(k_pull_sssp, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->row,
+                           e->col, e->w, e->pcstart, e->bm[0], e->level, e->vfc);
+        if (e->prof) mg_prof_end(e, slot);
+        hipLaunchKernelGGL(k_sssp_scan, dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, e->row, e->bm[0], e->part_n,
+                           e->part_mf);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
+        bu = false;
+    }
     if (bu) {
-        const int gb = grid_for((nv + 63) / 64, kBlock / MG_WAVE, G);
         const double bytes = known ? (8.0 * nv / 64.0 * 3 + 4.0 * (double)c.mu + 8.0 * nv) : 0.0;
         if (e->prof) mg_prof_begin(e, "k_pull", bytes, &slot);
         if (e->algo == MG_BC)
-            hipLaunchKernelGGL((k_pull<true>), dim3(gb), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->q);
+            hipLaunchKernelGGL((k_pull<true>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         else
-            hipLaunchKernelGGL((k_pull<false>), dim3(gb), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->q);
+            hipLaunchKernelGGL((k_pull<false>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         if (e->prof) mg_prof_end(e, slot);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
     }
     hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off);
+    if (!known) MG_TRY(launch_after_ctl(e));
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+// after the control kernel: the queue of a pull level's frontier (when the next level
+// pushes, or for the BC level log), and for SSSP the stamp bitmap a pull level reads
+static int launch_after_ctl(mg_engine *e) {
+    hipStream_t s = e->stream;
+    const int G = max_grid(e);
+    const int nv = e->nv;
+    const int words = (nv + 63) / 64;
+    const int gq = grid_for(words, 32, G);
+    hipLaunchKernelGGL(k_bmq_count, dim3(gq), dim3(kBlock), 0, s, e->ctl, words, e->bm[0], e->bm[1], e->part_n);
+    hipLaunchKernelGGL(k_bmq_scan, dim3(1), dim3(1024), 0, s, e->ctl, e->part_n, gq);
+    hipLaunchKernelGGL(k_bmq_write, dim3(gq), dim3(kBlock), 0, s, e->ctl, words, e->bm[0], e->bm[1], e->part_n, e->q);
+    if (e->algo == MG_SSSP)
+        hipLaunchKernelGGL(k_sssp_scan, dim3(grid_for(words, kBlock / MG_WAVE, G)), dim3(kBlock), 0, s, e->ctl, nv, -1, 2,
+                           e->vfc, e->row, e->bm[0], (int *)nullptr, (long long *)nullptr);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }
@@ -504,10 +826,13 @@ int mg_frontier_level(mg_engine *e) {
     MG_TRY(launch_level(e, true));
     MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
     MG_HIP(hipStreamSynchronize(e->stream));
-    e->iter = e->h_ctl->iter;
-    e->levels = e->h_ctl->levels;
-    e->edges_traversed = e->h_ctl->edges;
-    if (e->h_ctl->mode == MODE_DONE) e->done = true;
+    const Ctl &c = *e->h_ctl;
+    if (c.mode != MODE_DONE && (c.need_queue || (e->algo == MG_SSSP && c.mode == MODE_BU && c.convert)))
+        MG_TRY(launch_after_ctl(e));
+    e->iter = c.iter;
+    e->levels = c.levels;
+    e->edges_traversed = c.edges;
+    if (c.mode == MODE_DONE) e->done = true;
     return MG_OK;
 }
 

@@ -168,6 +168,7 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
         }
     }, 4096);
     g->flags = (sym ? MG_READ_SYM : 0) | (cc ? MG_READ_CC : 0);
+    g->symmetric = sym && !cc;
     return MG_OK;
 }
 
@@ -357,6 +358,8 @@ extern "C" int mg_graph_union(int count, const mg_graph *const *graphs, mg_graph
     g->valid[nv] = (int32_t)nv;
     g->start_p = 0;
     g->flags = graphs[0]->flags;
+    g->symmetric = true;
+    for (int i = 0; i < count; ++i) g->symmetric = g->symmetric && graphs[i]->symmetric;
     *out = g;
     return MG_OK;
 }

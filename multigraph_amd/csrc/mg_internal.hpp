@@ -20,6 +20,7 @@ struct mg_graph {
     int32_t start_p = 0;       // remapped source (read_input.cu:130)
     int flags = 0;
     bool weighted = true;
+    bool symmetric = true;     // every stored edge has its reverse stored too
     std::vector<int32_t> row;  // nv + 1
     std::vector<int32_t> col;  // ne
     std::vector<int32_t> w;    // ne

@@ -105,9 +105,17 @@ struct BCHooks {
 // holds throughout and the final root of every component is its minimum id.
 struct CCHooks {
     using V1T = int;
-    __device__ static int find(const V1T *parent, int v) {
+    // root of v with path halving: every visited vertex is re-pointed at its
+    // grandparent.  Concurrent writers only ever store an ancestor, and roots are
+    // changed only by the hook's CAS, so the halving writes need no atomics.
+    __device__ static int find(V1T *parent, int v) {
         int p = parent[v];
-        while (p != v) { v = p; p = parent[v]; }
+        while (p != v) {
+            const int gp = parent[p];
+            if (gp != p) parent[v] = gp;
+            v = p;
+            p = gp;
+        }
         return v;
     }
     // "hook": returns 1 when this call merged two trees
@@ -122,11 +130,20 @@ struct CCHooks {
             rv = find(parent, rv);
         }
     }
-    // pointer jumping: parent[v] = root(v); returns 1 when it changed
+    // root of v without writing: used by the final compression, where a halving
+    // write from one thread could re-point another thread's vertex, which already
+    // holds its root, back at an intermediate ancestor
+    __device__ static int find_ro(const V1T *parent, int v) {
+        int p = parent[v];
+        while (p != v) { v = p; p = parent[v]; }
+        return v;
+    }
+    // pointer jumping: parent[v] = root(v), written only by v's own thread;
+    // returns 1 when it changed
     __device__ static int update_condition(V1T *parent, int v) {
         int p = parent[v];
         if (p == v) return 0;
-        int r = find(parent, p);
+        int r = find_ro(parent, p);
         if (r != p) { parent[v] = r; return 1; }
         return 0;
     }

@@ -138,3 +138,17 @@ def test_pagerank_reference_rule(mg, orc, case):
     assert abs(e.iterations() - it) <= 2
     assert np.abs(r - ref).max() <= 0.2
     e.close()
+
+
+@pytest.mark.parametrize("n", [300, 1000, 5000])
+def test_cc_long_paths(mg, orc, n):
+    """long chains stress the concurrent hooks and the final compression: every label
+    must be its component's root, run after run"""
+    g = mg.Graph.from_coo(n, np.arange(n - 1), np.arange(1, n), np.full(n - 1, 3))
+    row, col, _ = g.csr()
+    uf, _ = orc.cc_unionfind(row, col)
+    e = mg.Engine(g, mg.CC)
+    for _ in range(5):
+        e.run(0)
+        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), uf)
+    e.close()
