@@ -481,16 +481,31 @@ int mg_cc_step(mg_engine *e, int *more) {
 }
 
 // ---------------------------------------------------------------------------- PR host
+int mg_pb_build_alloc(mg_engine *e, const mg_graph *g) {
+    MG_TRY(mg_pb_build(e, g));  // sets pr_nvi and the degrees of the internal ids
+    const long long padv = e->pr_nvi + 64;
+    MG_HIP(hipMalloc(&e->rank, sizeof(float) * padv));
+    MG_HIP(hipMalloc(&e->cval, sizeof(float) * padv));
+    MG_HIP(hipMalloc(&e->vfc, sizeof(int) * padv));
+    return MG_OK;
+}
+// PageRank: the partitioned layout (default, the reference's DENSE_MODE build) or a
+// pull over the CSR (MG_MODE_SPARSE)
 int mg_pr_build(mg_engine *e, const mg_graph *g) {
     const int nv = e->nv;
+    MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
+    MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    if (e->opt.mode != MG_MODE_SPARSE && nv > 0) {
+        e->pr_pb = true;
+        MG_TRY(mg_pb_build_alloc(e, g));  // internal ids are padded to whole partitions
+        return MG_OK;
+    }
     const long long padv = (long long)nv + 64;
     MG_HIP(hipMalloc(&e->rank, sizeof(float) * padv));
     MG_HIP(hipMalloc(&e->cval, sizeof(float) * padv));
     MG_HIP(hipMalloc(&e->deg, sizeof(int) * padv));
     MG_HIP(hipMalloc(&e->vfc, sizeof(int) * padv));
     MG_HIP(hipMalloc(&e->psum, sizeof(double) * padv));
-    MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
-    MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
     hipLaunchKernelGGL(k_degrees, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, e->stream, nv, e->row, e->deg);
     int nch = 0;
     MG_TRY(build_huge_chunks(e, g->row, nv, &e->chunks, &nch));
@@ -500,7 +515,8 @@ int mg_pr_build(mg_engine *e, const mg_graph *g) {
 }
 
 int mg_pr_reset(mg_engine *e) {
-    hipLaunchKernelGGL(k_pr_init, dim3(grid_for(e->nv, 256, 4096)), dim3(256), 0, e->stream, e->nv, e->rank, e->cval, e->vfc, e->ctl);
+    const long long n = e->pr_pb ? e->pr_nvi : e->nv;
+    hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, e->stream, (int)n, e->rank, e->cval, e->vfc, e->ctl);
     MG_HIP(hipGetLastError());
     MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
     MG_HIP(hipStreamSynchronize(e->stream));
@@ -512,6 +528,13 @@ int mg_pr_reset(mg_engine *e) {
 
 static int pr_launch_iteration(mg_engine *e) {
     hipStream_t s = e->stream;
+    if (e->pr_pb) {
+        hipLaunchKernelGGL(k_pr_begin, dim3(1), dim3(1), 0, s, e->ctl);
+        MG_TRY(mg_pb_iteration(e));
+        hipLaunchKernelGGL(k_pr_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->opt.pr_max_iter, e->ne);
+        MG_HIP(hipGetLastError());
+        return MG_OK;
+    }
     const int G = e->opt.grid_blocks > 0 ? e->opt.grid_blocks : 2048;
     const int variant = e->algo == MG_PR_T ? 0 : 1;
     int slot = -1;

@@ -23,7 +23,7 @@ extern "C" void mg_default_options(mg_options *o) {
     o->pr_damping = 0.85f;     // common.h:59
     o->pr_tolerance = 0.01f;   // common.h:60
     o->pr_max_iter = 0;        // the reference runs until no vertex changes
-    o->part_log2 = 14;
+    o->part_log2 = 0;          // automatic
 }
 
 static void fill_defaults(mg_options *o) {
@@ -36,7 +36,6 @@ static void fill_defaults(mg_options *o) {
     if (o->grid_blocks <= 0) o->grid_blocks = d.grid_blocks;
     if (o->pr_damping <= 0) o->pr_damping = d.pr_damping;
     if (o->pr_tolerance <= 0) o->pr_tolerance = d.pr_tolerance;
-    if (o->part_log2 <= 0) o->part_log2 = d.part_log2;
 }
 
 extern "C" int mg_device_count(int *n) {
@@ -249,6 +248,13 @@ extern "C" int mg_get_result(mg_engine *e, int which, void *dst, int64_t bytes) 
     }
     if (bytes < need) return mg_fail(MG_ERR_ARG, "buffer of %lld bytes, need %lld", (long long)bytes, (long long)need);
     MG_HIP(hipStreamSynchronize(e->stream));
+    if (which == MG_RES_RANK && e->pr_pb) {  // internal numbering -> compacted ids
+        std::vector<float> tmp(e->pr_nvi);
+        MG_HIP(hipMemcpy(tmp.data(), e->rank, sizeof(float) * e->pr_nvi, hipMemcpyDeviceToHost));
+        float *out = (float *)dst;
+        for (int v = 0; v < e->nv; ++v) out[v] = tmp[e->pr_perm_h[v]];
+        return MG_OK;
+    }
     if (need > 0) MG_HIP(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
     return MG_OK;
 }

@@ -114,7 +114,8 @@ __host__ __device__ inline void advance(Ctl &c, const Params &p) {
 
 // PageRank partitioned layout (the gen_structure.cu role, built on the GPU).
 struct mg_pb_layout {
-    int plog = 14;            // log2 partition size
+    int plog = 14;            // log2 partition size when forced by the options
+    int P = 0;                // ids per partition
     int np = 0;               // partitions
     int G = 16;               // entries per mapper group
     long long slots = 0;      // padded stream length (both streams)
@@ -175,6 +176,9 @@ struct mg_engine {
     int *deg = nullptr;
     double *psum = nullptr;
     mg_pb_layout pb;
+    bool pr_pb = false;               // PageRank over the partitioned layout
+    long long pr_nvi = 0;             // internal (renumbered, padded) vertex count
+    std::vector<int> pr_perm_h;       // compacted id -> internal id
     // host-side progress
     int iter = 1;
     bool ready = false;
@@ -210,4 +214,7 @@ int mg_pr_build(mg_engine *e, const mg_graph *g);
 int mg_pr_reset(mg_engine *e);
 int mg_pr_step(mg_engine *e, int *more);
 int mg_pr_capture(mg_engine *e, int iters, hipGraphExec_t *out);
+int mg_pb_build(mg_engine *e, const mg_graph *g);
+int mg_pb_build_alloc(mg_engine *e, const mg_graph *g);
+int mg_pb_iteration(mg_engine *e);
 void mg_free_dev(void *p);

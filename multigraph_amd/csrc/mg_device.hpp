@@ -129,4 +129,17 @@ __device__ __forceinline__ KSum wave_ksum(KSum k) {
 template <class T>
 __device__ __forceinline__ T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
 
+typedef unsigned int mg_u32x4 __attribute__((ext_vector_type(4)));
+typedef float mg_f32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte streaming loads (p must be 16-byte aligned)
+__device__ __forceinline__ uint4 ld_stream_u4(const void *p) {
+    const mg_u32x4 v = __builtin_nontemporal_load((const mg_u32x4 *)p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 ld_stream_f4(const void *p) {
+    const mg_f32x4 v = __builtin_nontemporal_load((const mg_f32x4 *)p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 }  // namespace mg
