@@ -98,7 +98,8 @@ int mg_graph_from_coo(int nv, int64_t nnz, const int32_t *src, const int32_t *ds
  * dropped, weights 1 + hash(weight_seed, min(u,v), max(u,v)) % 64; then read as above. */
 int mg_graph_gen_rmat(int scale, int edge_factor, double a, double b, double c,
                       uint64_t seed, uint64_t weight_seed, int flags, mg_graph **out);
-/* 2-D 4-neighbour grid, width x height, id = y*width + x. */
+/* 2-D 4-neighbour grid, width x height, id = y*width + x (edges +x and +y, both ways
+ * with MG_READ_SYM).  Generators honour MG_READ_SYM and MG_READ_CC as the reader does. */
 int mg_graph_gen_grid(int width, int height, uint64_t weight_seed, int flags, mg_graph **out);
 /* Disjoint union of graphs (vertex ids offset in order); used for uneven batches. */
 int mg_graph_union(int count, const mg_graph *const *graphs, mg_graph **out);
@@ -147,6 +148,8 @@ int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float *ms);
 /* levels / distances: int32 [nsrc][nv] (source-major); bc: float [nv] summed over sources */
 int mg_batch_get(mg_batch *b, int which, void *host_dst, int64_t bytes);
 int mg_batch_release(mg_batch *b);
+/* levels swept by the last mg_batch_run (BFS/BC: deepest level + 1; SSSP: sweeps) */
+int mg_batch_levels(const mg_batch *b, int *levels);
 
 /* ---- instrumentation -------------------------------------------------------- */
 /* Per-kernel accounting of the last mg_run: launches, summed device time (ms,

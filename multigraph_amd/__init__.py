@@ -85,6 +85,7 @@ _SIGS = {
     "mg_batch_run": (C.c_int, [_P, C.c_int, _I32P, C.POINTER(C.c_float)]),
     "mg_batch_get": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "mg_batch_release": (C.c_int, [_P]),
+    "mg_batch_levels": (C.c_int, [_P, _I32P]),
     "mg_set_profiling": (C.c_int, [_P, C.c_int]),
     "mg_kernel_stats": (C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
@@ -331,9 +332,20 @@ class Batch:
         return ms.value
 
     def levels(self) -> np.ndarray:
+        """[nsrc][nv] int32: BFS/BC levels, or SSSP distances for an SSSP batch"""
         out = np.empty((self.nsrc, self.nv), dtype=np.int32)
         _check(lib().mg_batch_get(self._h, RES_LEVEL, out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
+
+    def sigma(self) -> np.ndarray:
+        out = np.empty((self.nsrc, self.nv), dtype=np.float64)
+        _check(lib().mg_batch_get(self._h, RES_SIGMA, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def sweeps(self) -> int:
+        n = C.c_int32()
+        _check(lib().mg_batch_levels(self._h, C.byref(n)))
+        return n.value
 
     def bc(self) -> np.ndarray:
         out = np.empty(self.nv, dtype=np.float32)

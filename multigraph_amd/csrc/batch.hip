@@ -1,12 +1,703 @@
-// Batched multi-source runs (one bit per source packed per vertex).
+// Batched multi-source BFS / BC / SSSP: up to 64 sources run together, one bit per
+// source packed per vertex (visited word, frontier words) and one slot per source per
+// vertex for the outputs, stored vertex-major ([v][64]) so that the 64 lanes of a
+// wavefront are the 64 sources of one vertex and every per-vertex row is one
+// contiguous 256 B (int/float) or 512 B (double) access.
+//
+//   BFS / BC forward, per level:
+//     push  (few frontier vertices): a wave per frontier vertex u walks its edges;
+//           bits = F[u] & ~visited[v] are or-ed into the next frontier word of v
+//           (atomicOr), BC adds sigma[u][s] into sigma[v][s] for every bit s;
+//     pull  (many):  every vertex not yet reached by all sources or-s the frontier
+//           words of its neighbours (BFS stops as soon as the missing bits are
+//           covered); BC, lanes = sources, sums sigma of the parents of each new bit;
+//     the new frontier (vertex, word) is compacted into a level log (count, scan,
+//     ordered write: no contended counters).
+//   BC backward, per level d from the deepest: the words of level d+1 are scattered
+//     into a dense array, and a wave per logged (v, word) of level d sums, lane s,
+//     sigma[v][s] / sigma[w][s] * (1 + delta[w][s]) over neighbours w whose level-d+1
+//     word has bit s (Neumaier-compensated FP32, terms split in two floats).
+//   SSSP: Bellman-Ford by pull sweeps; changed[v] holds one bit per source whose
+//     distance at v dropped in the last sweep; lane s relaxes dist[v][s] from the
+//     neighbours whose changed bit s is set.
+#include <algorithm>
+#include <vector>
 #include "engine.hpp"
+#include "mg_device.hpp"
 
-struct mg_batch { int kind; };
+using namespace mg;
 
-extern "C" int mg_batch_build(const mg_graph *, int, int, const mg_options *, mg_batch **out) {
-    if (out) *out = nullptr;
-    return mg_fail(MG_ERR_STATE, "batched runs are not built yet");
+namespace {
+
+constexpr int kBB = 256;
+
+// order this wave's LDS stores before its loads of other lanes' slots (and the loads
+// of one chunk before the stores of the next)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-extern "C" int mg_batch_run(mg_batch *, int, const int32_t *, float *) { return mg_fail(MG_ERR_STATE, "batched runs are not built yet"); }
-extern "C" int mg_batch_get(mg_batch *, int, void *, int64_t) { return mg_fail(MG_ERR_STATE, "batched runs are not built yet"); }
-extern "C" int mg_batch_release(mg_batch *b) { delete b; return MG_OK; }
+
+int grid_for(long long items, int per_block, int cap) {
+    long long g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+__global__ void k_fill_u64(uint64_t *p, long long n, uint64_t v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_i32b(int *p, long long n, int v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f64b(double *p, long long n, double v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f32b(float *p, long long n, float v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// seed: source s at vertex src[s] (duplicates share the vertex)
+__global__ void k_ms_seed(int nsrc, const int *__restrict__ src, uint64_t *__restrict__ vis, uint64_t *__restrict__ F,
+                          int *__restrict__ lev, double *__restrict__ sigma, int *__restrict__ dist) {
+    const int s = threadIdx.x;
+    if (s >= nsrc) return;
+    const int v = src[s];
+    atomicOr((unsigned long long *)&vis[v], 1ull << s);
+    if (F) atomicOr((unsigned long long *)&F[v], 1ull << s);
+    if (lev) lev[(long long)v * 64 + s] = 0;
+    if (sigma) sigma[(long long)v * 64 + s] = 1.0;
+    if (dist) dist[(long long)v * 64 + s] = 0;
+}
+
+// ---- compaction of nonzero words into (vertex, word) entries -----------------------
+__global__ __launch_bounds__(kBB) void k_nz_count(int nv, const uint64_t *__restrict__ W, int *__restrict__ part) {
+    const int per = (nv + gridDim.x - 1) / gridDim.x;
+    const int v0 = blockIdx.x * per, v1 = min(nv, v0 + per);
+    long long c = 0;
+    for (int v = v0 + threadIdx.x; v < v1; v += kBB) c += W[v] != 0;
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    c = block_sum<kBB / MG_WAVE>(c, s_tmp);
+    if (threadIdx.x == 0) part[blockIdx.x] = (int)c;
+}
+
+__global__ __launch_bounds__(1024) void k_nz_scan(int *__restrict__ part, int nparts, long long *__restrict__ total) {
+    __shared__ int s_tmp[1024 / MG_WAVE + 1];
+    __shared__ int s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int b = 0; b < nparts; b += 1024) {
+        const int i = b + threadIdx.x;
+        const int v = i < nparts ? part[i] : 0;
+        int t;
+        const int pre = block_excl_scan<1024 / MG_WAVE>(v, s_tmp, &t);
+        const int carry = s_carry;
+        if (i < nparts) part[i] = carry + pre;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = carry + t;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = s_carry;
+}
+
+__global__ __launch_bounds__(kBB) void k_nz_write(int nv, const uint64_t *__restrict__ W, const int *__restrict__ part,
+                                                  long long base, int *__restrict__ outv, uint64_t *__restrict__ outw) {
+    const int per = (nv + gridDim.x - 1) / gridDim.x;
+    const int v0 = blockIdx.x * per, v1 = min(nv, v0 + per);
+    long long off = base + part[blockIdx.x];
+    __shared__ int s_tmp[kBB / MG_WAVE + 1];
+    for (int vb = v0; vb < v1; vb += kBB) {
+        const int v = vb + threadIdx.x;
+        const uint64_t w = v < v1 ? W[v] : 0ull;
+        int t;
+        const int pre = block_excl_scan<kBB / MG_WAVE>(w != 0 ? 1 : 0, s_tmp, &t);
+        if (w) { outv[off + pre] = v; if (outw) outw[off + pre] = w; }
+        off += t;
+        __syncthreads();
+    }
+}
+
+// ---- chunked work ------------------------------------------------------------------
+// Every kernel below walks a static list of edge chunks: a vertex's edges are cut
+// into chunks of at most kCH, so a hub is served by many wavefronts.  A vertex with
+// one chunk writes its results directly; the chunks of a split vertex merge theirs
+// with atomics (or, and, add, min) into state that starts the pass at its identity.
+struct Chunk { int v, p0, p1, multi; };
+constexpr int kCH = 1024;
+
+// push: a wave per chunk whose vertex u is in the frontier.  BFS: lanes over edges,
+// new bits or-ed into Fn.  BC: lanes = sources, edges in turn, sigma[u][s] added
+// into sigma[w][s] for every new bit s.
+template <bool kBC>
+__global__ __launch_bounds__(kBB) void k_mb_push(long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
+                                                 const uint64_t *__restrict__ F, const uint64_t *__restrict__ vis,
+                                                 uint64_t *__restrict__ Fn, double *__restrict__ sigma) {
+    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * kBB) >> 6;
+    for (long long c = wave; c < nch; c += nw) {
+        const Chunk k = ch[c];
+        const uint64_t fu = F[k.v];
+        if (!fu) continue;
+        if (!kBC) {
+            for (int p = k.p0 + lane(); p < k.p1; p += MG_WAVE) {
+                const int w = col[p];
+                const uint64_t m = fu & ~vis[w];
+                if (m) atomicOr((unsigned long long *)&Fn[w], (unsigned long long)m);
+            }
+        } else {
+            const double su = ((fu >> lane()) & 1ull) ? sigma[(long long)k.v * 64 + lane()] : 0.0;
+            for (int p = k.p0; p < k.p1; ++p) {
+                const int w = col[p];
+                const uint64_t m = fu & ~vis[w];
+                if (!m) continue;
+                if ((m >> lane()) & 1ull) atomicAdd(&sigma[(long long)w * 64 + lane()], su);
+                if (lane() == 0) atomicOr((unsigned long long *)&Fn[w], (unsigned long long)m);
+            }
+        }
+    }
+}
+
+// pull, BFS: a wave per chunk of a vertex still missing bits; lanes or the frontier
+// words of 64 neighbours at a time and stop once the missing bits are covered
+__global__ __launch_bounds__(kBB) void k_mb_pull_bfs(long long nch, const Chunk *__restrict__ ch, uint64_t all,
+                                                     const int *__restrict__ col, const uint64_t *__restrict__ F,
+                                                     const uint64_t *__restrict__ vis, uint64_t *__restrict__ Fn) {
+    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * kBB) >> 6;
+    for (long long c = wave; c < nch; c += nw) {
+        const Chunk k = ch[c];
+        const uint64_t need = all & ~vis[k.v];
+        if (!need) continue;
+        uint64_t acc = 0;
+        for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+            const int p = p0 + lane();
+            const uint64_t f = p < k.p1 ? (F[ld_stream(&col[p])] & need) : 0ull;
+            acc |= wave_or(f);
+            if (acc == need) break;
+        }
+        if (acc && lane() == 0) {
+            if (k.multi) atomicOr((unsigned long long *)&Fn[k.v], (unsigned long long)acc);
+            else Fn[k.v] = acc;
+        }
+    }
+}
+
+// pull, BC: lanes = sources; neighbours staged 64 at a time in LDS, then lane s sums
+// sigma[u][s] over neighbours u whose frontier word has bit s
+__global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *__restrict__ ch, uint64_t all,
+                                                    const int *__restrict__ col, const uint64_t *__restrict__ F,
+                                                    const uint64_t *__restrict__ vis, uint64_t *__restrict__ Fn,
+                                                    double *__restrict__ sigma) {
+    __shared__ int s_u[kBB];
+    __shared__ uint64_t s_f[kBB];
+    const int wl = threadIdx.x & ~(MG_WAVE - 1);
+    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * kBB) >> 6;
+    for (long long c = wave; c < nch; c += nw) {
+        const Chunk k = ch[c];
+        const uint64_t need = all & ~vis[k.v];
+        if (!need) continue;
+        uint64_t acc = 0;
+        double sg = 0.0;
+        for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+            const int p = p0 + lane();
+            int u = 0;
+            uint64_t f = 0;
+            if (p < k.p1) { u = ld_stream(&col[p]); f = F[u] & need; }
+            const uint64_t any = wave_or(f);
+            if (!any) continue;
+            acc |= any;
+            wave_sync();
+            s_u[threadIdx.x] = u;
+            s_f[threadIdx.x] = f;
+            wave_sync();
+            const int nn = min(MG_WAVE, k.p1 - p0);
+            for (int j = 0; j < nn; ++j) {
+                if ((s_f[wl + j] >> lane()) & 1ull) sg += sigma[(long long)s_u[wl + j] * 64 + lane()];
+            }
+        }
+        if (!acc) continue;
+        if ((acc >> lane()) & 1ull) {
+            const long long kv = (long long)k.v * 64 + lane();
+            if (k.multi) atomicAdd(&sigma[kv], sg);
+            else sigma[kv] = sg;
+        }
+        if (lane() == 0) {
+            if (k.multi) atomicOr((unsigned long long *)&Fn[k.v], (unsigned long long)acc);
+            else Fn[k.v] = acc;
+        }
+    }
+}
+
+// after a push or pull level: keep only new bits, mark them visited, record levels,
+// and leave the new frontier's size and degree sum as per-workgroup partials
+__global__ __launch_bounds__(kBB) void k_mb_apply(int nv, const int *__restrict__ row, uint64_t *__restrict__ vis,
+                                                  uint64_t *__restrict__ Fn, int *__restrict__ lev, int it,
+                                                  int *__restrict__ part_n, long long *__restrict__ part_e) {
+    int cnt = 0;
+    long long mf = 0;
+    for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB) {
+        const uint64_t f = Fn[v];
+        if (!f) continue;
+        const uint64_t vv = vis[v];
+        const uint64_t nw = f & ~vv;
+        if (nw != f) Fn[v] = nw;
+        if (!nw) continue;
+        vis[v] = vv | nw;
+        ++cnt;
+        mf += row[v + 1] - row[v];
+        uint64_t m = nw;
+        while (m) { const int s = __builtin_ctzll(m); lev[(long long)v * 64 + s] = it; m &= m - 1; }
+    }
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
+    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
+    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+}
+
+// size and degree sum of the nonzero words of W
+__global__ __launch_bounds__(kBB) void k_mb_count(int nv, const int *__restrict__ row, const uint64_t *__restrict__ W,
+                                                  int *__restrict__ part_n, long long *__restrict__ part_e) {
+    int cnt = 0;
+    long long mf = 0;
+    for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB)
+        if (W[v]) { ++cnt; mf += row[v + 1] - row[v]; }
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
+    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
+    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+}
+
+__global__ void k_sum_parts(const int *__restrict__ pn, const long long *__restrict__ pe, int n, long long *__restrict__ out) {
+    __shared__ long long s_tmp[256 / MG_WAVE];
+    long long a = 0, b = 0;
+    for (int i = threadIdx.x; i < n; i += 256) { a += pn[i]; if (pe) b += pe[i]; }
+    a = block_sum<256 / MG_WAVE>(a, s_tmp);
+    b = block_sum<256 / MG_WAVE>(b, s_tmp);
+    if (threadIdx.x == 0) { out[0] = a; out[1] = b; }
+}
+
+// ---- BC backward -------------------------------------------------------------------
+__global__ void k_ms_scatter_words(long long n, const int *__restrict__ lv, const uint64_t *__restrict__ lw,
+                                   uint64_t *__restrict__ D, int clear) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        D[lv[i]] = clear ? 0ull : lw[i];
+}
+
+__device__ __forceinline__ void ms_term(KSum &acc, double sv, double sw, float dw) {
+    const double t = (sv / sw) * (1.0 + (double)dw);
+    const float hi = (float)t;
+    acc.add(hi);
+    acc.add((float)(t - (double)hi));
+}
+
+// E = words of level d, D = words of level d+1; a wave per chunk of a vertex in E
+__global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
+                                                 const uint64_t *__restrict__ E, const uint64_t *__restrict__ D,
+                                                 const double *__restrict__ sigma, float *__restrict__ delta) {
+    __shared__ int s_u[kBB];
+    __shared__ uint64_t s_f[kBB];
+    const int wl = threadIdx.x & ~(MG_WAVE - 1);
+    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * kBB) >> 6;
+    for (long long c = wave; c < nch; c += nw) {
+        const Chunk k = ch[c];
+        const uint64_t word = E[k.v];
+        if (!word) continue;
+        const bool mine = (word >> lane()) & 1ull;
+        const double sv = mine ? sigma[(long long)k.v * 64 + lane()] : 0.0;
+        KSum acc;
+        for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+            const int p = p0 + lane();
+            int w = 0;
+            uint64_t f = 0;
+            if (p < k.p1) { w = ld_stream(&col[p]); f = D[w] & word; }
+            if (!wave_or(f)) continue;
+            wave_sync();
+            s_u[threadIdx.x] = w;
+            s_f[threadIdx.x] = f;
+            wave_sync();
+            const int nn = min(MG_WAVE, k.p1 - p0);
+            for (int j = 0; j < nn; ++j) {
+                if ((s_f[wl + j] >> lane()) & 1ull) {
+                    const long long kk = (long long)s_u[wl + j] * 64 + lane();
+                    ms_term(acc, sv, sigma[kk], delta[kk]);
+                }
+            }
+        }
+        if (mine) {
+            const long long kv = (long long)k.v * 64 + lane();
+            if (k.multi) atomicAdd(&delta[kv], acc.value());
+            else delta[kv] = acc.value();
+        }
+    }
+}
+
+// bc[v] += sum over sources s != v of delta[v][s]
+__global__ void k_ms_bc_sum(int nv, int nsrc, const int *__restrict__ src, const float *__restrict__ delta, float *__restrict__ bc) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int v = wave; v < nv; v += nw) {
+        KSum k;
+        if (lane() < nsrc && src[lane()] != v) k.add(delta[(long long)v * 64 + lane()]);
+        k = wave_ksum(k);
+        if (lane() == 0) bc[v] += k.value();
+    }
+}
+
+// ---- SSSP --------------------------------------------------------------------------
+// a wave per chunk, lanes = sources: relax dist[v][s] from neighbours whose changed
+// bit s is set; the vertex's new changed word is the ballot of improvements
+__global__ __launch_bounds__(kBB) void k_mb_sssp(long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
+                                                 const int *__restrict__ wgt, const uint64_t *__restrict__ C,
+                                                 uint64_t *__restrict__ Cn, int *__restrict__ dist) {
+    __shared__ int s_u[kBB], s_w[kBB];
+    __shared__ uint64_t s_f[kBB];
+    const int wl = threadIdx.x & ~(MG_WAVE - 1);
+    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * kBB) >> 6;
+    for (long long c = wave; c < nch; c += nw) {
+        const Chunk k = ch[c];
+        const long long kv = (long long)k.v * 64 + lane();
+        const int cur = dist[kv];
+        int best = cur;
+        for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+            const int p = p0 + lane();
+            int u = 0, wu = 0;
+            uint64_t f = 0;
+            if (p < k.p1) { u = ld_stream(&col[p]); f = C[u]; }
+            if (!wave_or(f)) continue;
+            if (p < k.p1) wu = ld_stream(&wgt[p]);
+            wave_sync();
+            s_u[threadIdx.x] = u;
+            s_w[threadIdx.x] = wu;
+            s_f[threadIdx.x] = f;
+            wave_sync();
+            const int nn = min(MG_WAVE, k.p1 - p0);
+            for (int j = 0; j < nn; ++j) {
+                if ((s_f[wl + j] >> lane()) & 1ull) best = min(best, dist[(long long)s_u[wl + j] * 64 + lane()] + s_w[wl + j]);
+            }
+        }
+        bool imp;
+        if (k.multi) imp = best < cur && atomicMin(&dist[kv], best) > best;
+        else { imp = best < cur; if (imp) dist[kv] = best; }
+        const uint64_t m = ballot(imp);
+        if (lane() == 0 && m) {
+            if (k.multi) atomicOr((unsigned long long *)&Cn[k.v], (unsigned long long)m);
+            else Cn[k.v] = m;
+        }
+    }
+}
+
+// vertex-major [v][64] -> source-major [s][v]
+template <class T>
+__global__ void k_ms_transpose(int nv, int nsrc, const T *__restrict__ in, T *__restrict__ out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)nv * nsrc;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(i / nv), v = (int)(i % nv);
+        out[i] = in[(long long)v * 64 + s];
+    }
+}
+
+}  // namespace
+
+struct mg_batch {
+    int kind = 0;
+    int nv = 0;
+    long long ne = 0;
+    int max_sources = 64;
+    mg_options opt{};
+    hipStream_t stream = nullptr;
+    int *row = nullptr, *col = nullptr, *w = nullptr;
+    uint64_t *vis = nullptr, *F[2] = {nullptr, nullptr}, *D = nullptr, *E = nullptr;
+    int *lev = nullptr, *dist = nullptr, *src = nullptr;
+    double *sigma = nullptr;
+    float *delta = nullptr, *bc = nullptr;
+    Chunk *chunks = nullptr;      // static edge chunks
+    long long nch = 0;
+    int *logv = nullptr;          // (vertex, word) entries of every level (BC)
+    uint64_t *logw = nullptr;
+    long long log_cap = 0;
+    int *part_n = nullptr;
+    long long *part_e = nullptr, *tot = nullptr, *h_tot = nullptr;
+    int nsrc = 0, levels = 0;
+    bool symmetric = true;        // pull levels need every edge stored both ways
+};
+
+static void batch_free(mg_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    void *ptrs[] = {b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->sigma,
+                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_e, b->tot};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    if (b->h_tot) (void)hipHostFree(b->h_tot);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, const mg_options *opt, mg_batch **out) {
+    if (!g || !out) return mg_fail(MG_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (kind < MG_BATCH_BFS || kind > MG_BATCH_SSSP) return mg_fail(MG_ERR_ARG, "unknown batch kind %d", kind);
+    if (max_sources < 1 || max_sources > 64) return mg_fail(MG_ERR_LIMIT, "a batch holds 1..64 sources (asked %d)", max_sources);
+    if (kind == MG_BATCH_SSSP && !g->symmetric)
+        return mg_fail(MG_ERR_ARG, "batched SSSP sweeps pull over in-edges and needs a symmetric graph");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return mg_fail(MG_ERR_DEVICE, "no HIP device available");
+    mg_batch *b = new mg_batch();
+    b->kind = kind;
+    b->nv = g->nv;
+    b->ne = g->ne;
+    b->max_sources = max_sources;
+    b->symmetric = g->symmetric;
+    if (opt) b->opt = *opt; else mg_default_options(&b->opt);
+    if (b->opt.grid_blocks > 4096) b->opt.grid_blocks = 4096;  // per-workgroup partial arrays hold 4096
+    auto fail = [&](int rc) { batch_free(b); return rc; };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "stream"));
+    // static edge chunks (a hub's edges are cut into pieces of kCH)
+    std::vector<Chunk> ch;
+    ch.reserve((size_t)g->nv + (size_t)(g->ne / kCH) + 1);
+    for (int v = 0; v < g->nv; ++v) {
+        const int p0 = g->row[v], p1 = g->row[v + 1];
+        if (p1 == p0) continue;
+        const int multi = (p1 - p0) > kCH ? 1 : 0;
+        for (int p = p0; p < p1; p += kCH) ch.push_back(Chunk{v, p, std::min(p1, p + kCH), multi});
+    }
+    b->nch = (long long)ch.size();
+    const long long nv = b->nv, ne = std::max<long long>(1, b->ne), slots = nv * 64 + 64;
+    bool ok = hipMalloc(&b->row, sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->col, sizeof(int) * ne) == hipSuccess &&
+              hipMalloc(&b->chunks, sizeof(Chunk) * std::max<long long>(1, b->nch)) == hipSuccess &&
+              hipMalloc(&b->vis, sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
+              hipMalloc(&b->F[0], sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
+              hipMalloc(&b->F[1], sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
+              hipMalloc(&b->src, sizeof(int) * 64) == hipSuccess && hipMalloc(&b->part_n, sizeof(int) * 4096) == hipSuccess &&
+              hipMalloc(&b->part_e, sizeof(long long) * 4096) == hipSuccess &&
+              hipMalloc(&b->tot, sizeof(long long) * 4) == hipSuccess &&
+              hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess;
+    if (ok && kind != MG_BATCH_SSSP) ok = hipMalloc(&b->lev, sizeof(int) * slots) == hipSuccess;
+    if (ok && kind == MG_BATCH_SSSP)
+        ok = hipMalloc(&b->w, sizeof(int) * ne) == hipSuccess && hipMalloc(&b->dist, sizeof(int) * slots) == hipSuccess;
+    if (ok && kind == MG_BATCH_BC) {
+        b->log_cap = nv * 8 + 1024;  // grown on demand
+        ok = hipMalloc(&b->sigma, sizeof(double) * slots) == hipSuccess && hipMalloc(&b->delta, sizeof(float) * slots) == hipSuccess &&
+             hipMalloc(&b->bc, sizeof(float) * (nv + 1)) == hipSuccess && hipMalloc(&b->D, sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
+             hipMalloc(&b->E, sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
+             hipMalloc(&b->logv, sizeof(int) * b->log_cap) == hipSuccess &&
+             hipMalloc(&b->logw, sizeof(uint64_t) * b->log_cap) == hipSuccess;
+    }
+    if (!ok) return fail(mg_fail(MG_ERR_NOMEM, "batch: device allocation failed"));
+    if (hipMemcpy(b->row, g->row.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (g->ne > 0 && hipMemcpy(b->col, g->col.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess) ||
+        (b->w && g->ne > 0 && hipMemcpy(b->w, g->w.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess) ||
+        (b->nch > 0 && hipMemcpy(b->chunks, ch.data(), sizeof(Chunk) * b->nch, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(mg_fail(MG_ERR_DEVICE, "batch: upload failed"));
+    *out = b;
+    return MG_OK;
+}
+
+extern "C" int mg_batch_release(mg_batch *b) {
+    if (!b) return mg_fail(MG_ERR_ARG, "null batch");
+    batch_free(b);
+    return MG_OK;
+}
+
+// compact the nonzero words of W into the log at `base`; returns the count
+static int compact(mg_batch *b, const uint64_t *W, long long base, long long *count) {
+    hipStream_t s = b->stream;
+    const int gq = grid_for(b->nv, 2048, 2048);
+    hipLaunchKernelGGL(k_nz_count, dim3(gq), dim3(kBB), 0, s, b->nv, W, b->part_n);
+    hipLaunchKernelGGL(k_nz_scan, dim3(1), dim3(1024), 0, s, b->part_n, gq, b->tot + 2);
+    MG_HIP(hipMemcpyAsync(b->h_tot + 2, b->tot + 2, sizeof(long long), hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+    const long long n = b->h_tot[2];
+    if (base + n > b->log_cap) {
+        long long cap = std::max(b->log_cap * 2, base + n + 1024);
+        int *nv2 = nullptr;
+        uint64_t *nw2 = nullptr;
+        MG_HIP(hipMalloc(&nv2, sizeof(int) * cap));
+        MG_HIP(hipMalloc(&nw2, sizeof(uint64_t) * cap));
+        if (base > 0) {
+            MG_HIP(hipMemcpyAsync(nv2, b->logv, sizeof(int) * base, hipMemcpyDeviceToDevice, s));
+            MG_HIP(hipMemcpyAsync(nw2, b->logw, sizeof(uint64_t) * base, hipMemcpyDeviceToDevice, s));
+        }
+        MG_HIP(hipStreamSynchronize(s));
+        hipFree(b->logv);
+        hipFree(b->logw);
+        b->logv = nv2;
+        b->logw = nw2;
+        b->log_cap = cap;
+    }
+    hipLaunchKernelGGL(k_nz_write, dim3(gq), dim3(kBB), 0, s, b->nv, W, b->part_n, base, b->logv, b->logw);
+    MG_HIP(hipGetLastError());
+    *count = n;
+    return MG_OK;
+}
+
+// frontier size and degree sum from the per-workgroup partials
+static int read_counts(mg_batch *b, int nparts, long long *n, long long *mf) {
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, b->stream, b->part_n, b->part_e, nparts, b->tot);
+    MG_HIP(hipMemcpyAsync(b->h_tot, b->tot, sizeof(long long) * 2, hipMemcpyDeviceToHost, b->stream));
+    MG_HIP(hipStreamSynchronize(b->stream));
+    *n = b->h_tot[0];
+    if (mf) *mf = b->h_tot[1];
+    return MG_OK;
+}
+
+static int run_bfs_bc(mg_batch *b, int nsrc) {
+    hipStream_t s = b->stream;
+    const int nv = b->nv;
+    const bool bc = b->kind == MG_BATCH_BC;
+    const uint64_t all = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
+    const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
+    const float alpha = b->opt.alpha > 0 ? b->opt.alpha : 15.0f;
+    const long long slots = (long long)nv * 64 + 64;
+    const int gv = grid_for(nv, kBB, G);
+    const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[0], (long long)nv + 1, 0ull);
+    hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->lev, slots, (int)MG_INF);
+    if (bc) {
+        hipLaunchKernelGGL(k_fill_f64b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->sigma, slots, 0.0);
+        hipLaunchKernelGGL(k_fill_f32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->delta, slots, 0.0f);
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->D, (long long)nv + 1, 0ull);
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->E, (long long)nv + 1, 0ull);
+    }
+    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, nsrc, b->src, b->vis, b->F[0], b->lev, bc ? b->sigma : nullptr, (int *)nullptr);
+    std::vector<long long> off{0};  // BC level log offsets
+    long long n = 0, mf = 0;
+    hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, nv, b->row, b->F[0], b->part_n, b->part_e);
+    MG_TRY(read_counts(b, gv, &n, &mf));
+    if (bc) { long long c0 = 0; MG_TRY(compact(b, b->F[0], 0, &c0)); off.push_back(c0); }
+    int cur = 0, it = 1;
+    while (n > 0) {
+        uint64_t *F = b->F[cur], *Fn = b->F[cur ^ 1];
+        bool push = (double)mf < (double)b->ne / alpha;
+        if (b->opt.mode == MG_MODE_SPARSE) push = true;
+        if (b->opt.mode == MG_MODE_DENSE) push = false;
+        if (!b->symmetric) push = true;
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, Fn, (long long)nv + 1, 0ull);
+        if (push) {
+            if (bc) hipLaunchKernelGGL((k_mb_push<true>), dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
+            else hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
+        } else {
+            if (bc) hipLaunchKernelGGL(k_mb_pull_bc, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, all, b->col, F, b->vis, Fn, b->sigma);
+            else hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, all, b->col, F, b->vis, Fn);
+        }
+        hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, nv, b->row, b->vis, Fn, b->lev, it, b->part_n, b->part_e);
+        MG_TRY(read_counts(b, gv, &n, &mf));
+        if (bc && n > 0) { long long c = 0; MG_TRY(compact(b, Fn, off.back(), &c)); off.push_back(off.back() + c); }
+        cur ^= 1;
+        ++it;
+    }
+    b->levels = it - 1;
+    if (!bc) return MG_OK;
+    // backward: levels 0 .. L-1, entries off[l] .. off[l+1]
+    const int L = (int)off.size() - 1;
+    for (int l = L - 1; l >= 0; --l) {
+        const long long n0 = off[l + 1] - off[l];
+        const long long n1 = (l + 1 < L) ? off[l + 2] - off[l + 1] : 0;
+        if (n0 <= 0) continue;
+        hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->E, 0);
+        if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], b->D, 0);
+        hipLaunchKernelGGL(k_mb_back, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, b->E, b->D, b->sigma, b->delta);
+        hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->E, 1);
+        if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], b->D, 1);
+    }
+    hipLaunchKernelGGL(k_ms_bc_sum, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, nsrc, b->src, b->delta, b->bc);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+static int run_sssp(mg_batch *b, int nsrc) {
+    hipStream_t s = b->stream;
+    const int nv = b->nv;
+    const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
+    const long long slots = (long long)nv * 64 + 64;
+    const int gv = grid_for(nv, kBB, G);
+    const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
+    hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->dist, slots, (int)MG_INF);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[0], (long long)nv + 1, 0ull);
+    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, nsrc, b->src, b->vis, b->F[0], (int *)nullptr, (double *)nullptr, b->dist);
+    int cur = 0, it = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[cur ^ 1], (long long)nv + 1, 0ull);
+        hipLaunchKernelGGL(k_mb_sssp, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, b->w, b->F[cur], b->F[cur ^ 1], b->dist);
+        hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, nv, b->row, b->F[cur ^ 1], b->part_n, b->part_e);
+        long long n = 0;
+        MG_TRY(read_counts(b, gv, &n, nullptr));
+        cur ^= 1;
+        ++it;
+        if (n == 0) break;
+    }
+    b->levels = it;
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+extern "C" int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float *ms) {
+    if (!b || (nsrc > 0 && !sources)) return mg_fail(MG_ERR_ARG, "null argument");
+    if (nsrc < 1 || nsrc > b->max_sources) return mg_fail(MG_ERR_LIMIT, "batch of %d sources (1..%d)", nsrc, b->max_sources);
+    for (int i = 0; i < nsrc; ++i)
+        if (sources[i] < 0 || sources[i] >= b->nv) return mg_fail(MG_ERR_ARG, "source %d outside [0,%d)", sources[i], b->nv);
+    MG_HIP(hipMemcpy(b->src, sources, sizeof(int) * nsrc, hipMemcpyHostToDevice));
+    b->nsrc = nsrc;
+    if (b->kind == MG_BATCH_BC)
+        hipLaunchKernelGGL(k_fill_f32b, dim3(grid_for(b->nv + 1, 256, 4096)), dim3(256), 0, b->stream, b->bc, (long long)b->nv + 1, 0.0f);
+    MG_HIP(hipStreamSynchronize(b->stream));
+    hipEvent_t e0, e1;
+    MG_HIP(hipEventCreate(&e0));
+    MG_HIP(hipEventCreate(&e1));
+    MG_HIP(hipEventRecord(e0, b->stream));
+    int rc = b->kind == MG_BATCH_SSSP ? run_sssp(b, nsrc) : run_bfs_bc(b, nsrc);
+    hipEventRecord(e1, b->stream);
+    hipError_t se = hipEventSynchronize(e1);
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != MG_OK) return rc;
+    if (se != hipSuccess) return mg_fail(MG_ERR_DEVICE, "batch run failed: %s", hipGetErrorString(se));
+    if (ms) *ms = t;
+    return MG_OK;
+}
+
+extern "C" int mg_batch_get(mg_batch *b, int which, void *dst, int64_t bytes) {
+    if (!b || !dst) return mg_fail(MG_ERR_ARG, "null argument");
+    const int nv = b->nv, ns = b->nsrc;
+    hipStream_t s = b->stream;
+    if (which == MG_RES_LEVEL || which == MG_RES_SIGMA) {
+        const bool sig = which == MG_RES_SIGMA;
+        if (sig && b->kind != MG_BATCH_BC) return mg_fail(MG_ERR_ARG, "sigma needs a BC batch");
+        const size_t esz = sig ? sizeof(double) : sizeof(int);
+        const int64_t need = (int64_t)esz * nv * ns;
+        if (bytes < need) return mg_fail(MG_ERR_ARG, "buffer of %lld bytes, need %lld", (long long)bytes, (long long)need);
+        void *tmp = nullptr;
+        MG_HIP(hipMalloc(&tmp, std::max<int64_t>(need, 8)));
+        const long long n = (long long)nv * ns;
+        if (sig) hipLaunchKernelGGL((k_ms_transpose<double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, nv, ns, b->sigma, (double *)tmp);
+        else hipLaunchKernelGGL((k_ms_transpose<int>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, nv, ns,
+                                b->kind == MG_BATCH_SSSP ? b->dist : b->lev, (int *)tmp);
+        hipError_t e1 = hipStreamSynchronize(s);
+        hipError_t e2 = need > 0 ? hipMemcpy(dst, tmp, need, hipMemcpyDeviceToHost) : hipSuccess;
+        hipFree(tmp);
+        if (e1 != hipSuccess || e2 != hipSuccess) return mg_fail(MG_ERR_DEVICE, "batch result copy failed");
+        return MG_OK;
+    }
+    if (which == MG_RES_DELTA) {
+        if (b->kind != MG_BATCH_BC) return mg_fail(MG_ERR_ARG, "bc needs a BC batch");
+        if (bytes < (int64_t)sizeof(float) * nv) return mg_fail(MG_ERR_ARG, "buffer too small");
+        MG_HIP(hipStreamSynchronize(s));
+        MG_HIP(hipMemcpy(dst, b->bc, sizeof(float) * nv, hipMemcpyDeviceToHost));
+        return MG_OK;
+    }
+    return mg_fail(MG_ERR_ARG, "unknown batch result %d", which);
+}
+
+extern "C" int mg_batch_levels(const mg_batch *b, int *levels) {
+    if (!b || !levels) return mg_fail(MG_ERR_ARG, "null argument");
+    *levels = b->levels;
+    return MG_OK;
+}

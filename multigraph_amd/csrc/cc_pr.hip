@@ -109,18 +109,20 @@ namespace {
 // ------------------------------------------------------------------ CC
 // Upper-triangular CSR (edges u < v), the role of the reference's src<dst COO
 // (read_input.cu:111-113, gen_structure.cu:34-139).
-__global__ void k_cc_count(int nv, const int *__restrict__ row, const int *__restrict__ col, int *__restrict__ cnt) {
+// the hooked edge list: (u, v) with v > u of a symmetric CSR (each undirected edge once);
+// every non-loop edge of a directed one (`all`), whose components are the weak ones
+__global__ void k_cc_count(int nv, const int *__restrict__ row, const int *__restrict__ col, int all, int *__restrict__ cnt) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nw = (gridDim.x * blockDim.x) >> 6;
     for (int u = wave; u < nv; u += nw) {
         int c = 0;
-        for (int p = row[u] + lane(); p < row[u + 1]; p += MG_WAVE) c += col[p] > u;
+        for (int p = row[u] + lane(); p < row[u + 1]; p += MG_WAVE) c += all ? col[p] != u : col[p] > u;
         c = wave_sum(c);
         if (lane() == 0) cnt[u] = c;
     }
 }
 
-__global__ void k_cc_fill(int nv, const int *__restrict__ row, const int *__restrict__ col, const long long *__restrict__ uoff,
+__global__ void k_cc_fill(int nv, const int *__restrict__ row, const int *__restrict__ col, int all, const long long *__restrict__ uoff,
                           int *__restrict__ urow, int *__restrict__ ucol) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nw = (gridDim.x * blockDim.x) >> 6;
@@ -129,7 +131,7 @@ __global__ void k_cc_fill(int nv, const int *__restrict__ row, const int *__rest
         if (lane() == 0) urow[u] = (int)o;
         for (int p0 = row[u]; p0 < row[u + 1]; p0 += MG_WAVE) {
             const int p = p0 + lane();
-            const bool keep = p < row[u + 1] && col[p] > u;
+            const bool keep = p < row[u + 1] && (all ? col[p] != u : col[p] > u);
             const uint64_t m = ballot(keep);
             if (keep) ucol[o + mbcnt(m)] = col[p];
             o += __builtin_popcountll(m);
@@ -388,14 +390,14 @@ int mg_cc_build(mg_engine *e, const mg_graph *g) {
     long long *uoff = nullptr;
     MG_HIP(hipMalloc(&cnt, sizeof(int) * ((long long)nv + 1)));
     MG_HIP(hipMalloc(&uoff, sizeof(long long) * ((long long)nv + 1)));
-    hipLaunchKernelGGL(k_cc_count, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, e->row, e->col, cnt);
+    hipLaunchKernelGGL(k_cc_count, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, e->row, e->col, g->symmetric ? 0 : 1, cnt);
     MG_TRY(mg_scan64(s, nullptr, cnt, nv, uoff));
     long long ne_u = 0;
     MG_HIP(hipMemcpy(&ne_u, uoff + nv, sizeof(long long), hipMemcpyDeviceToHost));
     e->ne_u = ne_u;
     MG_HIP(hipMalloc(&e->urow, sizeof(int) * ((long long)nv + 1)));
     MG_HIP(hipMalloc(&e->ucol, sizeof(int) * (ne_u > 0 ? ne_u : 1)));
-    hipLaunchKernelGGL(k_cc_fill, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, e->row, e->col, uoff, e->urow, e->ucol);
+    hipLaunchKernelGGL(k_cc_fill, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, e->row, e->col, g->symmetric ? 0 : 1, uoff, e->urow, e->ucol);
     if (nv == 0) {
         int z = 0;
         MG_HIP(hipMemcpy(e->urow, &z, sizeof(int), hipMemcpyHostToDevice));
@@ -495,7 +497,9 @@ int mg_pr_build(mg_engine *e, const mg_graph *g) {
     const int nv = e->nv;
     MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
     MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
-    if (e->opt.mode != MG_MODE_SPARSE && nv > 0) {
+    // the pull kernels read a CSR row as in-edges, so a directed graph always takes the
+    // partitioned layout, which scatters along out-edges and gathers by destination
+    if ((e->opt.mode != MG_MODE_SPARSE || !g->symmetric) && nv > 0) {
         e->pr_pb = true;
         MG_TRY(mg_pb_build_alloc(e, g));  // internal ids are padded to whole partitions
         return MG_OK;

@@ -669,13 +669,14 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     P.nv = nv;
     P.ne = ne;
     P.log_queue = (e->algo == MG_SSSP) ? 0 : 1;
-    P.forced = e->opt.mode;
+    // pull levels read a vertex's CSR row as its in-edges, which only a symmetric CSR
+    // holds; a directed graph runs push levels only
+    P.forced = g->symmetric ? e->opt.mode : MG_MODE_SPARSE;
     P.sssp = e->algo == MG_SSSP ? 1 : 0;
     P.need_log = e->algo == MG_BC ? 1 : 0;
     P.huge = e->opt.huge_degree;
     P.alpha = e->opt.alpha;
     P.beta = e->opt.beta;
-    (void)g;
     MG_HIP(hipGetLastError());
     return MG_OK;
 }

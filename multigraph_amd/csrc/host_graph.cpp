@@ -312,7 +312,8 @@ extern "C" int mg_graph_gen_rmat(int scale, int edge_factor, double a, double b,
         s[m] = s[k]; d[m] = d[k]; w[m] = w[k]; ++m;
     }
     mg_graph *g = new mg_graph();
-    int rc = mg_build_from_entries((int32_t)nv, m, s.data(), d.data(), w.data(), true, (flags & MG_READ_CC) != 0, 0, g);
+    int rc = mg_build_from_entries((int32_t)nv, m, s.data(), d.data(), w.data(), (flags & MG_READ_SYM) != 0,
+                                   (flags & MG_READ_CC) != 0, 0, g);
     if (rc != MG_OK) { delete g; return rc; }
     *out = g;
     return MG_OK;
@@ -332,7 +333,8 @@ extern "C" int mg_graph_gen_grid(int width, int height, uint64_t weight_seed, in
             if (y + 1 < height) { s.push_back(id); d.push_back(id + width); w.push_back(mg_edge_weight(weight_seed, id, id + width)); }
         }
     mg_graph *g = new mg_graph();
-    int rc = mg_build_from_entries((int32_t)nv, (int64_t)s.size(), s.data(), d.data(), w.data(), true, (flags & MG_READ_CC) != 0, 0, g);
+    int rc = mg_build_from_entries((int32_t)nv, (int64_t)s.size(), s.data(), d.data(), w.data(), (flags & MG_READ_SYM) != 0,
+                                   (flags & MG_READ_CC) != 0, 0, g);
     if (rc != MG_OK) { delete g; return rc; }
     *out = g;
     return MG_OK;

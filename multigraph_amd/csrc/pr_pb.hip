@@ -345,12 +345,32 @@ int mg_pb_build(mg_engine *e, const mg_graph *g) {
     e->pr_nvi = nvi;
     MG_HIP(hipMalloc(&e->psum, sizeof(double) * (nvi + 64)));
     if ((long long)np * np > (1ll << 31)) return mg_fail(MG_ERR_LIMIT, "%d partitions: tile table too large", np);
-    // degree round-robin numbering (host: one sort of nv degrees)
+    // The internal rows are in-edges (a destination gathers from its sources).  A
+    // symmetric CSR is its own transpose; a directed one is transposed here on the host.
+    std::vector<int> trow, tcol;
+    const int *hr = g->row.data();
+    const int *crow = e->row, *ccol = e->col;
+    int *d_trow = nullptr, *d_tcol = nullptr, *ilen = nullptr;
+    if (!g->symmetric) {
+        trow.assign((size_t)nv + 1, 0);
+        tcol.resize((size_t)std::max<long long>(1, e->ne));
+        for (long long p = 0; p < e->ne; ++p) trow[(size_t)g->col[p] + 1]++;
+        for (int v = 0; v < nv; ++v) trow[v + 1] += trow[v];
+        std::vector<int> cur(trow.begin(), trow.end() - 1);
+        for (int u = 0; u < nv; ++u)
+            for (int p = g->row[u]; p < g->row[u + 1]; ++p) tcol[cur[g->col[p]]++] = u;
+        MG_HIP(hipMalloc(&d_trow, sizeof(int) * ((size_t)nv + 1)));
+        MG_HIP(hipMalloc(&d_tcol, sizeof(int) * tcol.size()));
+        MG_HIP(hipMemcpy(d_trow, trow.data(), sizeof(int) * ((size_t)nv + 1), hipMemcpyHostToDevice));
+        MG_HIP(hipMemcpy(d_tcol, tcol.data(), sizeof(int) * tcol.size(), hipMemcpyHostToDevice));
+        hr = trow.data();
+        crow = d_trow;
+        ccol = d_tcol;
+    }
+    // degree round-robin numbering (host: one sort of nv in-degrees)
     std::vector<int> order(nv);
     for (int v = 0; v < nv; ++v) order[v] = v;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return (g->row[a + 1] - g->row[a]) > (g->row[b + 1] - g->row[b]);
-    });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return (hr[a + 1] - hr[a]) > (hr[b + 1] - hr[b]); });
     e->pr_perm_h.assign(nv, 0);
     for (int r = 0; r < nv; ++r) e->pr_perm_h[order[r]] = (r % np) * P + r / np;
     int *perm = nullptr, *irow = nullptr, *icol = nullptr;
@@ -359,18 +379,27 @@ int mg_pb_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMemcpy(perm, e->pr_perm_h.data(), sizeof(int) * nv, hipMemcpyHostToDevice));
     MG_HIP(hipMalloc(&e->deg, sizeof(int) * (nvi + 64)));
     MG_HIP(hipMemsetAsync(e->deg, 0, sizeof(int) * (nvi + 64), s));
+    // out-degrees (the divisor of every message) and internal row lengths
     hipLaunchKernelGGL(k_pb_ideg, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->row, perm, e->deg);
+    if (crow != e->row) {
+        MG_HIP(hipMalloc(&ilen, sizeof(int) * (nvi + 64)));
+        MG_HIP(hipMemsetAsync(ilen, 0, sizeof(int) * (nvi + 64), s));
+        hipLaunchKernelGGL(k_pb_ideg, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, crow, perm, ilen);
+    }
     MG_HIP(hipMalloc(&irow64, sizeof(long long) * (nvi + 1)));
-    MG_TRY(mg_scan64(s, nullptr, e->deg, nvi, irow64));
+    MG_TRY(mg_scan64(s, nullptr, ilen ? ilen : e->deg, nvi, irow64));
     MG_HIP(hipMalloc(&irow, sizeof(int) * (nvi + 1)));
     MG_HIP(hipMalloc(&icol, sizeof(int) * std::max<long long>(1, e->ne)));
     hipLaunchKernelGGL(k_pb_narrow, dim3(grid_for(nvi + 1, 256, 8192)), dim3(256), 0, s, nvi, irow64, irow);
-    hipLaunchKernelGGL(k_pb_relabel, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, e->row, e->col, perm, irow64, icol);
+    hipLaunchKernelGGL(k_pb_relabel, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, crow, ccol, perm, irow64, icol);
     std::vector<int> hrow(nvi + 1);
     MG_HIP(hipMemcpyAsync(hrow.data(), irow, sizeof(int) * (nvi + 1), hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));
     hipFree(irow64);
     hipFree(perm);
+    if (ilen) hipFree(ilen);
+    if (d_trow) hipFree(d_trow);
+    if (d_tcol) hipFree(d_tcol);
     int rc = mg_pb_layout_build(e, hrow, irow, icol);
     hipFree(irow);
     hipFree(icol);

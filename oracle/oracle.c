@@ -194,7 +194,7 @@ int orc_cc_unionfind(int nv, const int32_t *row, const int32_t *col, int32_t *la
 
 /* validate.cu:149-189 — Brandes from `src`: forward FIFO BFS counting shortest
  * paths (sigma), backward over the queue in reverse accumulating dependencies
- * delta[k] += sigma[k]/sigma[i] * (1 + delta[i]) for predecessors k of i.
+ * delta[i] = sum sigma[i]/sigma[k] * (1 + delta[k]) over successors k of i.
  * The reference computes sigma in int and delta in float; here both are FP64, the
  * precision the GPU result is judged against. Returns the number of reached vertices. */
 int orc_bc_fp64(int nv, const int32_t *row, const int32_t *col, int src, int32_t *level, double *sigma,
@@ -213,12 +213,18 @@ int orc_bc_fp64(int nv, const int32_t *row, const int32_t *col, int src, int32_t
             if (level[i] + 1 == level[k]) sigma[k] += sigma[i];
         }
     }
+    /* Dependencies in successor form: delta[i] sums over out-edges i->k one level deeper.
+     * validate.cu:170-185 pushes from k to the predecessors found in k's own row, which
+     * is the same sum when every edge is stored both ways; on a directed CSR only the
+     * successor form is Brandes' recurrence, so the oracle uses it throughout. */
     for (int t = tail - 1; t >= 0; --t) {
         int i = q[t];
+        double acc = 0.0;
         for (int32_t j = row[i]; j < row[i + 1]; ++j) {
             int k = col[j];
-            if (level[k] == level[i] - 1) delta[k] += (sigma[k] / sigma[i]) * (1.0 + delta[i]);
+            if (level[k] == level[i] + 1) acc += (sigma[i] / sigma[k]) * (1.0 + delta[k]);
         }
+        delta[i] = acc;
     }
     free(q);
     return tail;

@@ -1,0 +1,120 @@
+"""Batched multi-source runs (one bit per source per vertex) against the oracle run
+once per source: BFS levels and SSSP distances bit-exact, BC path counts bit-exact,
+summed BC dependencies within FP32 tolerance of the FP64 Brandes sums."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _hubs(mg):
+    """Two hubs of 3000 and 1500 leaves joined by a path, plus leaf-to-leaf edges:
+    hub rows span several 1024-edge chunks, whose partial results must merge."""
+    rng = np.random.default_rng(21)
+    src = [0] * 3000 + [1] * 1500 + [0]
+    dst = list(range(2, 3002)) + list(range(3002, 4502)) + [1]
+    a = rng.integers(2, 4502, size=4000)
+    b = rng.integers(2, 4502, size=4000)
+    keep = a != b
+    src += a[keep].tolist()
+    dst += b[keep].tolist()
+    return mg.Graph.from_coo(4502, np.array(src), np.array(dst))
+
+
+def _graphs(mg):
+    return {
+        "rmat13": mg.Graph.rmat(13, 16, seed=11),
+        "rmat11x2": mg.Graph.rmat(11, 2, seed=12),
+        "grid": mg.Graph.grid(30, 45),
+        "hubs": _hubs(mg),
+    }
+
+
+GRAPHS = ["rmat13", "rmat11x2", "grid", "hubs"]
+
+
+def _sources(g, k, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, g.nv, size=k).astype(np.int32)
+
+
+MODES = [dict(), dict(mode=1), dict(mode=2)]
+
+
+@pytest.mark.parametrize("gname", GRAPHS)
+@pytest.mark.parametrize("k", [1, 7, 64])
+@pytest.mark.parametrize("opts", MODES, ids=["hybrid", "push", "pull"])
+def test_batch_bfs(mg, orc, gname, k, opts):
+    g = _graphs(mg)[gname]
+    row, col, _ = g.csr()
+    src = _sources(g, k, 3 + k)
+    b = mg.Batch(g, mg.BATCH_BFS, **opts)
+    b.run(src)
+    lv = b.levels()
+    for i, s in enumerate(src):
+        np.testing.assert_array_equal(lv[i], orc.bfs(row, col, int(s)), err_msg=f"source {i}={s}")
+    b.close()
+
+
+@pytest.mark.parametrize("gname", GRAPHS)
+@pytest.mark.parametrize("opts", MODES, ids=["hybrid", "push", "pull"])
+def test_batch_bc(mg, orc, gname, opts):
+    g = _graphs(mg)[gname]
+    row, col, _ = g.csr()
+    src = _sources(g, 64, 5)
+    src[7] = src[3]  # duplicate source: both slots must be served
+    b = mg.Batch(g, mg.BATCH_BC, **opts)
+    b.run(src)
+    lv = b.levels()
+    sg = b.sigma()
+    bc = b.bc().astype(np.float64)
+    ref_bc = np.zeros(g.nv)
+    for i, s in enumerate(src):
+        l, sig, dl = orc.bc_fp64(row, col, int(s))
+        np.testing.assert_array_equal(lv[i], l)
+        np.testing.assert_array_equal(sg[i], sig)
+        dl = dl.copy()
+        dl[int(s)] = 0.0
+        ref_bc += dl
+    err = np.abs(bc - ref_bc) / np.maximum(1.0, np.abs(ref_bc))
+    print(f"bc {gname} max rel err {err.max():.2e}")
+    assert err.max() <= 1e-5
+    b.close()
+
+
+@pytest.mark.parametrize("gname", GRAPHS)
+def test_batch_sssp(mg, orc, gname):
+    g = _graphs(mg)[gname]
+    row, col, w = g.csr()
+    src = _sources(g, 64, 9)
+    b = mg.Batch(g, mg.BATCH_SSSP)
+    b.run(src)
+    d = b.levels()
+    for i, s in enumerate(src):
+        ref, _ = orc.sssp_bf(row, col, w, int(s))
+        np.testing.assert_array_equal(d[i], ref, err_msg=f"source {i}={s}")
+    b.close()
+
+
+@pytest.mark.parametrize("kind", ["bfs", "bc"])
+def test_batch_directed(mg, orc, kind):
+    """A directed CSR (no MG_READ_SYM) runs push levels only; levels follow out-edges."""
+    g = mg.Graph.rmat(11, 8, seed=13, flags=0)
+    row, col, _ = g.csr()
+    src = _sources(g, 64, 17)
+    b = mg.Batch(g, mg.BATCH_BFS if kind == "bfs" else mg.BATCH_BC)
+    b.run(src)
+    lv = b.levels()
+    for i, s in enumerate(src):
+        if kind == "bfs":
+            np.testing.assert_array_equal(lv[i], orc.bfs(row, col, int(s)))
+        else:
+            l, _, _ = orc.bc_fp64(row, col, int(s))
+            np.testing.assert_array_equal(lv[i], l)
+    b.close()
+
+
+def test_batch_sssp_needs_symmetric(mg):
+    g = mg.Graph.rmat(8, 4, seed=2, flags=0)
+    with pytest.raises(mg.MGError):
+        mg.Batch(g, mg.BATCH_SSSP)

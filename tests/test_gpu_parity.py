@@ -152,3 +152,49 @@ def test_cc_long_paths(mg, orc, n):
         e.run(0)
         np.testing.assert_array_equal(e.result(mg.RES_LEVEL), uf)
     e.close()
+
+
+@pytest.fixture(scope="module")
+def directed(mg):
+    """RMAT read without symmetrization: every level follows out-edges only"""
+    g = mg.Graph.rmat(12, 8, seed=6, flags=0)
+    row, col, w = g.csr()
+    return g, row, col, w
+
+
+@pytest.mark.parametrize("mode", MODES, ids=[m[0] for m in MODES])
+def test_directed_bfs_sssp(mg, orc, directed, mode):
+    g, row, col, w = directed
+    eb = mg.Engine(g, mg.BFS, **mode[1])
+    es = mg.Engine(g, mg.SSSP, **mode[1])
+    for s in sources(g):
+        print(f"directed bfs/sssp mode={mode[0]} src={s}", flush=True)
+        eb.run(s)
+        np.testing.assert_array_equal(eb.result(mg.RES_LEVEL), orc.bfs(row, col, s))
+        es.run(s)
+        np.testing.assert_array_equal(es.result(mg.RES_LEVEL), orc.sssp_bf(row, col, w, s)[0])
+    eb.close()
+    es.close()
+
+
+def test_directed_bc_cc_pr(mg, orc, directed):
+    g, row, col, w = directed
+    e = mg.Engine(g, mg.BC)
+    for s in sources(g):
+        e.run(s)
+        lv, sg, dl = orc.bc_fp64(row, col, s)
+        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), lv)
+        np.testing.assert_array_equal(e.result(mg.RES_SIGMA), sg)
+        np.testing.assert_allclose(e.result(mg.RES_DELTA).astype(np.float64), dl, rtol=1e-6, atol=1e-6)
+    e.close()
+    e = mg.Engine(g, mg.CC)
+    e.run(0)
+    np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.cc_unionfind(row, col)[0])
+    e.close()
+    for mode in (dict(), dict(mode=1)):
+        e = mg.Engine(g, mg.PR_T, pr_tolerance=1e-7, pr_max_iter=100, **mode)
+        e.run(0)
+        r = e.result(mg.RES_RANK).astype(np.float64)
+        ref, _ = orc.pr_fp64(row, col, 0, 0.85, 1e-7, -e.iterations())
+        assert np.abs(r - ref).sum() / np.abs(ref).sum() <= 1e-6
+        e.close()
