@@ -27,6 +27,7 @@ MODE_HYBRID, MODE_SPARSE, MODE_DENSE = 0, 1, 2
 RES_LEVEL, RES_SIGMA, RES_DELTA, RES_RANK = 0, 1, 2, 3
 BATCH_BFS, BATCH_BC, BATCH_SSSP = 0, 1, 2
 INF = 0x64646464
+OK, ERR_ARG, ERR_IO, ERR_RANGE, ERR_NOMEM, ERR_DEVICE, ERR_LIMIT, ERR_STATE = 0, -1, -2, -3, -4, -5, -6, -7
 
 
 class MGError(RuntimeError):

@@ -88,7 +88,8 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
     std::vector<uint8_t> keep((size_t)n, 0);
     std::vector<uint8_t> mark((size_t)nv + 1, 0);
     int64_t first_nonempty = -1;
-    for (int32_t v = 0; v < nv; ++v) if (start[v + 1] > start[v]) { first_nonempty = start[v]; break; }
+    int32_t first_row = -1;
+    for (int32_t v = 0; v < nv; ++v) if (start[v + 1] > start[v]) { first_nonempty = start[v]; first_row = v; break; }
     mg_parallel_for(nv, [&](int64_t b, int64_t e) {
         std::vector<uint64_t> key;
         std::vector<int32_t> td, tw;
@@ -128,6 +129,12 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
     for (int32_t v = 0; v < nv; ++v) if (mark[v]) g->orig[g->valid[v]] = v;
     if (source < 0 || (nv > 0 && source >= nv)) return mg_fail(MG_ERR_ARG, "source %d outside [0,%d)", source, nv);
     g->start_p = g->valid[source];
+    // Entry 0 of the sorted order never marks its endpoints valid; when no other entry
+    // names them, the reference maps them past the last compacted id and writes out of
+    // its arrays.  Such input is rejected here instead.
+    if (first_nonempty >= 0 && (g->valid[first_row] >= cnv || g->valid[sd[first_nonempty]] >= cnv))
+        return mg_fail(MG_ERR_RANGE, "entry (%d,%d) names a vertex no other entry does; its compacted id would be out of range",
+                       first_row + 1, sd[first_nonempty] + 1);
     // CSR of kept edges in compacted ids (read_input.cu:135-171)
     g->nv = cnv;
     g->row.assign((size_t)cnv + 1, 0);
