@@ -7,10 +7,21 @@ plus connected components, on a symmetrized RMAT scale-22 graph (a,b,c,d =
 vertices compacted).  value = (E + E_u) / step time, the reference's TEPS conventions
 (SSSP counts directed edges E, CC the src<dst edges E_u; code/main.cu:46,124).
 
-The JSON line also carries the dominant kernel's roofline (algorithmic bytes per launch
-over its HIP-event-timed average duration, on the engine stream), the CPU baseline
-(the oracle's single-thread restatement of code/validate.cu on the same graph) and
-per-algorithm figures.  Inputs are resident in HBM before the timed region.
+The JSON line also carries
+  * roofline: the dominant kernel of the step, algorithmic bytes per launch over its
+    average duration from HIP events recorded around every launch on the engine
+    stream;
+  * cpu_baseline: the oracle's single-thread restatement of code/validate.cu (SSSP
+    Bellman-Ford + CC label propagation) on the same graph, timed on this host;
+  * per_algo: BFS, BC, PR_T, PR_D on RMAT-22 with GTEPS and the fraction of HBM peak
+    their algorithmic bytes imply;
+  * batched: 64 sources (uniform over the vertices, seed 3) run as one batch vs 64
+    single-source runs, BFS, BC and SSSP, on RMAT-22;
+  * pagerank_rmat24: PR_T and PR_D at damping 0.85, tolerance 1e-7, at most 100
+    iterations on RMAT-24 (their error against FP64 is covered by the parity tests);
+  * uneven_batch: 32 graphs (RMAT-14..20, grids 256^2..2048^2) in one batched run,
+    4 BFS / SSSP sources each, default options vs autotuned.
+Inputs are resident in HBM before every timed region.
 """
 import argparse
 import json
@@ -51,14 +62,153 @@ def cpu_model():
     return platform.processor()
 
 
+def frac(bytes_, ms):
+    return bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+
+
+def gteps(edges, ms):
+    return edges / (ms * 1e-3) / 1e9
+
+
+def rmat(mg, scale):
+    t0 = time.time()
+    g = mg.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=1, weight_seed=7)
+    log(f"RMAT-{scale}: nv={g.nv} ne={g.ne} built on the host in {time.time() - t0:.1f}s")
+    return g
+
+
+def per_algo_extras(mg, np, g, src, nv, ne):
+    out = {}
+    e = mg.Engine(g, mg.BFS)
+    e.run(src)
+    t = float(np.median([e.run(src) for _ in range(5)]))
+    out["bfs"] = {"ms": t, "gteps": gteps(ne, t), "levels": e.iterations(), "hbm_frac": frac(8.0 * ne + 8.0 * nv, t)}
+    e.close()
+    e = mg.Engine(g, mg.BC)
+    e.run(src)
+    t = float(np.median([e.run(src) for _ in range(3)]))
+    out["bc"] = {"ms": t, "gteps": gteps(2 * ne, t), "hbm_frac": frac(36.0 * ne + 24.0 * nv, t)}
+    e.close()
+    for name, algo in (("pr_t", mg.PR_T), ("pr_d", mg.PR_D)):
+        e = mg.Engine(g, algo, pr_tolerance=1e-7, pr_max_iter=100)
+        e.run(0)
+        ms = e.run(0)
+        its = max(e.iterations(), 1)
+        out[name] = {"ms": ms, "iterations": its, "ms_per_iter": ms / its, "gteps": gteps(ne, ms),
+                     "gteps_per_iter": gteps(ne, ms / its), "hbm_frac_per_iter": frac(8.0 * ne + 16.0 * nv, ms / its)}
+        e.close()
+    return out
+
+
+def batched(mg, np, g, nv, ne):
+    """64 sources (seed 3) as one batch vs 64 single-source runs"""
+    rng = np.random.default_rng(3)
+    src = rng.integers(0, nv, size=64).astype(np.int32)
+    out = {"sources": "64 uniform over the compacted vertices, numpy default_rng(3)"}
+    for name, kind, algo, te in (("bfs", mg.BATCH_BFS, mg.BFS, 1), ("bc", mg.BATCH_BC, mg.BC, 2),
+                                 ("sssp", mg.BATCH_SSSP, mg.SSSP, 1)):
+        b = mg.Batch(g, kind)
+        b.run(src)
+        tb = float(np.median([b.run(src) for _ in range(3)]))
+        levels = b.sweeps()
+        lv = b.levels()
+        e = mg.Engine(g, algo)
+        e.run(int(src[0]))
+        ts = []
+        ok = True
+        for i, s in enumerate(src):
+            ts.append(e.run(int(s)))
+            if i < 4:  # the batch's rows match single-source runs, bit for bit
+                ok = ok and bool(np.array_equal(e.result(mg.RES_LEVEL), lv[i]))
+        tsum = float(np.sum(ts))
+        rec = {"batch_ms": tb, "singles_ms": tsum, "batched_gteps": gteps(64 * te * ne, tb),
+               "per_source_gteps": gteps(64 * te * ne, tsum), "speedup": tsum / tb, "levels": levels,
+               "rows_match_singles": ok}
+        if name == "bfs":
+            rec["hbm_frac"] = frac((12.0 * ne + 20.0 * nv) * levels, tb)
+        b.close()
+        e.close()
+        out[name] = rec
+        log(f"batched {name}: {tb:.2f} ms vs {tsum:.2f} ms of singles ({tsum / tb:.2f}x)")
+    return out
+
+
+def uneven(mg, np):
+    """32 graphs in one batch: RMAT-14..20 (4 seeds each) and 4-neighbour grids 256^2 ..
+    2048^2 joined as a disjoint union, 4 sources per graph sharing 4 slots of one run;
+    default options vs the autotuner's"""
+    t0 = time.time()
+    graphs = [mg.Graph.rmat(sc, 16, seed=100 + 10 * k + sc, weight_seed=7) for k in range(4) for sc in range(14, 21)]
+    graphs += [mg.Graph.grid(side, side, weight_seed=7) for side in (256, 512, 1024, 2048)]
+    u = mg.Graph.union(graphs)
+    rng = np.random.default_rng(5)
+    src, slots, off = [], [], 0
+    for g in graphs:
+        src += (rng.integers(0, g.nv, 4) + off).tolist()
+        slots += [0, 1, 2, 3]
+        off += g.nv
+    ne = u.ne
+    log(f"uneven batch: {len(graphs)} graphs, nv={u.nv} ne={ne} ({time.time() - t0:.1f}s)")
+    out = {"graphs": "RMAT-14..20 x 4 seeds + grids 256^2, 512^2, 1024^2, 2048^2", "nv": u.nv, "ne": ne,
+           "sources": "4 per graph, numpy default_rng(5), slots 0..3 shared across graphs"}
+    for name, kind in (("bfs", mg.BATCH_BFS), ("sssp", mg.BATCH_SSSP)):
+        b = mg.Batch(u, kind)
+        b.run(src, slots)
+        t_def = float(np.median([b.run(src, slots) for _ in range(3)]))
+        lv_def = b.levels()
+        levels = b.sweeps()
+        b.close()
+        t1 = time.time()
+        opt, _, _ = mg.batch_autotune(u, kind, src, slots, reps=2)
+        tune_s = time.time() - t1
+        b = mg.Batch(u, kind, options=opt)
+        b.run(src, slots)
+        t_tun = float(np.median([b.run(src, slots) for _ in range(3)]))
+        same = bool(np.array_equal(b.levels(), lv_def))
+        b.close()
+        out[name] = {"untuned_ms": t_def, "tuned_ms": t_tun, "untuned_gteps": gteps(4 * ne, t_def),
+                     "tuned_gteps": gteps(4 * ne, t_tun), "levels": levels, "tuning_s": tune_s,
+                     "results_equal": same,
+                     "tuned_options": {"alpha": opt.alpha, "batch_chunk": opt.batch_chunk,
+                                       "grid_blocks": opt.grid_blocks, "graph_levels": opt.graph_levels}}
+        log(f"uneven {name}: {t_def:.2f} ms untuned, {t_tun:.2f} ms tuned ({tune_s:.1f}s of tuning)")
+    return out
+
+
+def pagerank_big(mg, np, scale):
+    """timing only: the ranks' error against FP64 is measured by the PageRank parity tests"""
+    g = rmat(mg, scale)
+    nv, ne = g.nv, g.ne
+    out = {"graph": f"RMAT-{scale}", "nv": nv, "ne": ne}
+    for name, algo in (("pr_t", mg.PR_T), ("pr_d", mg.PR_D)):
+        e = mg.Engine(g, algo, pr_tolerance=1e-7, pr_max_iter=100)
+        e.run(0)
+        e.set_profiling(True)
+        ms_prof = e.run(0)
+        ks = e.kernel_stats()
+        e.set_profiling(False)
+        ms = e.run(0)
+        its = max(e.iterations(), 1)
+        rec = {"ms": ms, "iterations": its, "ms_per_iter": ms / its, "gteps_per_iter": gteps(ne, ms / its),
+               "hbm_frac_per_iter": frac(8.0 * ne + 16.0 * nv, ms / its),
+               "kernels": {k: {"avg_ms": v["ms"] / max(v["launches"], 1), "launches": v["launches"]}
+                           for k, v in ks.items()}, "profiled_ms": ms_prof}
+        e.close()
+        out[name] = rec
+        log(f"{name} RMAT-{scale}: {ms / its:.3f} ms/iter over {its} iterations")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--pr-scale", type=int, default=24, help="graph scale of the PageRank figures (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the per-algorithm extras")
+    ap.add_argument("--no-extra", action="store_true", help="skip per-algorithm, batched and PageRank figures")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 32-graph uneven batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU path",
@@ -73,13 +223,11 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
-    t0 = time.time()
-    g = mg.Graph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, weight_seed=7)
+    g = rmat(mg, args.scale)
     nv, ne, _ = g.info()
     row, col, w = g.csr()
     ne_u = int(np.count_nonzero(col > np.repeat(np.arange(nv, dtype=np.int32), np.diff(row))))
     src = g.max_degree_vertex()
-    log(f"RMAT-{args.scale}: nv={nv} ne={ne} ne_u={ne_u} src={src} ({time.time() - t0:.1f}s)")
 
     sssp = mg.Engine(g, mg.SSSP)
     cc = mg.Engine(g, mg.CC)
@@ -118,41 +266,22 @@ def main():
     sssp.set_profiling(False)
     cc.set_profiling(False)
 
+    t_sssp, t_cc = float(np.mean(ms_sssp)), float(np.mean(ms_cc))
     per_algo = {
-        "sssp": {"ms": float(np.mean(ms_sssp)), "gteps": ne / (np.mean(ms_sssp) * 1e-3) / 1e9,
-                 "iterations": sssp_iters},
-        "cc": {"ms": float(np.mean(ms_cc)), "gteps": ne_u / (np.mean(ms_cc) * 1e-3) / 1e9},
+        "sssp": {"ms": t_sssp, "gteps": gteps(ne, t_sssp), "iterations": sssp_iters,
+                 "hbm_frac_one_pass": frac(12.0 * ne + 8.0 * nv, t_sssp)},
+        "cc": {"ms": t_cc, "gteps": gteps(ne_u, t_cc), "hbm_frac_one_round": frac(16.0 * ne_u + 8.0 * nv, t_cc)},
     }
     sssp.close()
     cc.close()
 
+    batch = None
+    pr_big = None
+    uneven_batch = None
     if not args.no_extra:
-        e = mg.Engine(g, mg.BFS)
-        e.run(src)
-        t = [e.run(src) for _ in range(5)]
-        per_algo["bfs"] = {"ms": float(np.median(t)), "gteps": ne / (np.median(t) * 1e-3) / 1e9,
-                           "levels": e.iterations()}
-        e.close()
-        e = mg.Engine(g, mg.BC)
-        e.run(src)
-        t = [e.run(src) for _ in range(3)]
-        per_algo["bc"] = {"ms": float(np.median(t)), "gteps": 2 * ne / (np.median(t) * 1e-3) / 1e9}
-        e.close()
-        for name, algo in (("pr_t", mg.PR_T), ("pr_d", mg.PR_D)):
-            e = mg.Engine(g, algo, pr_tolerance=1e-7, pr_max_iter=100)
-            e.set_profiling(True)
-            ms = e.run(0)
-            its = e.iterations()
-            ks = e.kernel_stats()
-            e.set_profiling(False)
-            gk = ks.get("k_pr_gather", {"ms": 0.0, "bytes": 0.0, "launches": 1})
-            alg_bytes_iter = 8.0 * ne + 16.0 * nv
-            per_algo[name] = {"ms": ms, "iterations": its, "ms_per_iter": ms / max(its, 1),
-                              "gteps_per_iter": ne / (ms / max(its, 1) * 1e-3) / 1e9,
-                              "hbm_frac_per_iter": alg_bytes_iter / (ms / max(its, 1) * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                              "gather_kernel_gbs": gk["bytes"] / max(gk["ms"], 1e-9) / 1e6}
-            e.close()
-        log("extras done")
+        per_algo.update(per_algo_extras(mg, np, g, src, nv, ne))
+        log("per-algorithm figures done")
+        batch = batched(mg, np, g, nv, ne)
 
     cpu = None
     if not args.no_cpu:
@@ -169,6 +298,12 @@ def main():
                          f"{t2 - t1:.2f}s) + label-propagation CC ({t3 - t2:.2f}s), code/validate.cu restated, "
                          f"1 thread on {cpu_model()}"}
         log("cpu baseline done")
+
+    del g, row, col, w, srcs, d, lab
+    if not args.no_extra and args.pr_scale > 0:
+        pr_big = pagerank_big(mg, np, args.pr_scale)
+    if not args.no_extra and not args.no_uneven:
+        uneven_batch = uneven(mg, np)
 
     line = {
         "metric": METRIC,
@@ -191,6 +326,9 @@ def main():
                      "bytes_per_launch": avg_bytes, "launches": dom["launches"]},
         "cpu_baseline": cpu,
         "per_algo": per_algo,
+        "batched": batch,
+        "pagerank_rmat24": pr_big,
+        "uneven_batch": uneven_batch,
         "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1),
                         "gbs": v["bytes"] / max(v["ms"], 1e-9) / 1e6} for k, v in kt.items()},
         "verified": ok,

@@ -73,11 +73,13 @@ typedef struct mg_options {
     float  beta;            /* pull->push when frontier vertices < nv / beta            */
     int    huge_degree;     /* vertices at or above this degree are split across blocks */
     int    grid_blocks;     /* workgroups of the grid-stride frontier kernels           */
-    int    graph_levels;    /* levels captured per hipGraph launch (0: eager, host step)*/
+    int    graph_levels;    /* levels / iterations captured per hipGraph launch (0: BFS,
+                               SSSP, BC step eagerly; PageRank replays 8 per host check) */
     float  pr_damping;      /* PageRank damping (0.85, common.h:59)                     */
     float  pr_tolerance;    /* PageRank tolerance (0.01, common.h:60)                   */
     int    pr_max_iter;     /* PageRank iteration cap (0: unbounded, as the reference)  */
     int    part_log2;       /* log2 of the vertex-partition size of the tiled layout    */
+    int    batch_chunk;     /* batched runs: edges per static chunk (0: 1024)           */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */
@@ -130,7 +132,8 @@ int mg_step(mg_engine *e, int *iter);
 /* BC dependency pass over the logged levels (MultiGraph_V_E_V_pull). */
 int mg_bc_backward(mg_engine *e, int iter);
 /* reset + all steps (+ backward for BC), timed with HIP events on the engine's
- * stream; uses captured hipGraphs when opt.graph_levels > 0. */
+ * stream; uses captured hipGraphs when opt.graph_levels > 0 (always for PageRank on the
+ * partitioned layout, unless profiling is on). */
 int mg_run(mg_engine *e, int source, float *ms);
 int mg_sync(mg_engine *e);
 int mg_iterations(const mg_engine *e, int *iters);
@@ -145,11 +148,31 @@ typedef struct mg_batch mg_batch;
 int mg_batch_build(const mg_graph *g, int kind, int max_sources, const mg_options *opt,
                    mg_batch **out);
 int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float *ms);
-/* levels / distances: int32 [nsrc][nv] (source-major); bc: float [nv] summed over sources */
+/* Sources with explicit slots (bits): slot[i] in [0, max_sources).  Sources that share a
+ * slot run from their set; on a disjoint union (mg_graph_union) sources of different
+ * member graphs can share slots, so 32 graphs x 4 sources fill 4 slots of one run. */
+int mg_batch_run_slots(mg_batch *b, int n, const int32_t *sources, const int32_t *slots, float *ms);
+/* levels / distances: int32 [nslots][nv] (slot-major); sigma: double [nslots][nv];
+ * bc (MG_RES_DELTA): float [nv], dependencies summed over slots, each vertex's own slots left out */
 int mg_batch_get(mg_batch *b, int which, void *host_dst, int64_t bytes);
 int mg_batch_release(mg_batch *b);
 /* levels swept by the last mg_batch_run (BFS/BC: deepest level + 1; SSSP: sweeps) */
 int mg_batch_levels(const mg_batch *b, int *levels);
+
+/* ---- autotuner ------------------------------------------------------------- */
+/* Pick the tunables of mg_options (push/pull thresholds alpha and beta, grid shape,
+ * levels per captured graph, PageRank partition size, batched chunk size) for one
+ * graph and algorithm by timing candidates on the graph itself, one coordinate at a
+ * time (each candidate: a build, a warm-up run, the median of `reps` timed runs).
+ * Replaces the reference's per-run wall-clock mode sampling (process.cu:1161-1255)
+ * with a one-time search.  *best starts from *best's own values (zero fields: the
+ * defaults); ms_default / ms_tuned receive the timed cost before and after. */
+int mg_autotune(const mg_graph *g, int algo, int nsrc, const int32_t *sources, int reps,
+                mg_options *best, float *ms_default, float *ms_tuned);
+/* Same for a batch kind over one run of `sources` (with `slots`, or NULL). */
+int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32_t *sources,
+                      const int32_t *slots, int reps, mg_options *best, float *ms_default,
+                      float *ms_tuned);
 
 /* ---- instrumentation -------------------------------------------------------- */
 /* Per-kernel accounting of the last mg_run: launches, summed device time (ms,

@@ -48,6 +48,7 @@ class Options(C.Structure):
         ("pr_tolerance", C.c_float),
         ("pr_max_iter", C.c_int),
         ("part_log2", C.c_int),
+        ("batch_chunk", C.c_int),
     ]
 
 
@@ -84,9 +85,14 @@ _SIGS = {
     "mg_get_result": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "mg_batch_build": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Options), C.POINTER(_P)]),
     "mg_batch_run": (C.c_int, [_P, C.c_int, _I32P, C.POINTER(C.c_float)]),
+    "mg_batch_run_slots": (C.c_int, [_P, C.c_int, _I32P, _I32P, C.POINTER(C.c_float)]),
     "mg_batch_get": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "mg_batch_release": (C.c_int, [_P]),
     "mg_batch_levels": (C.c_int, [_P, _I32P]),
+    "mg_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, C.c_int, C.POINTER(Options), C.POINTER(C.c_float),
+                              C.POINTER(C.c_float)]),
+    "mg_batch_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, _I32P, C.c_int, C.POINTER(Options),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mg_set_profiling": (C.c_int, [_P, C.c_int]),
     "mg_kernel_stats": (C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
@@ -325,11 +331,19 @@ class Batch:
         except Exception:
             pass
 
-    def run(self, sources) -> float:
+    def run(self, sources, slots=None) -> float:
+        """one run; source i takes bit i, or bit slots[i] when a slot list is given"""
         s = _i32(sources)
         ms = C.c_float(0.0)
-        _check(lib().mg_batch_run(self._h, len(s), _ptr(s), C.byref(ms)))
-        self.nsrc = len(s)
+        if slots is None:
+            _check(lib().mg_batch_run(self._h, len(s), _ptr(s), C.byref(ms)))
+            self.nsrc = len(s)
+        else:
+            k = _i32(slots)
+            if len(k) != len(s):
+                raise ValueError("one slot per source")
+            _check(lib().mg_batch_run_slots(self._h, len(s), _ptr(s), _ptr(k), C.byref(ms)))
+            self.nsrc = int(k.max()) + 1
         return ms.value
 
     def levels(self) -> np.ndarray:
@@ -352,3 +366,23 @@ class Batch:
         out = np.empty(self.nv, dtype=np.float32)
         _check(lib().mg_batch_get(self._h, RES_DELTA, out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
+
+
+def autotune(graph: Graph, algo: int, sources, reps: int = 3, **kw):
+    """(best Options, ms with the defaults, ms tuned) for an engine algorithm"""
+    src = _i32(sources)
+    o = default_options(**kw)
+    a, b = C.c_float(), C.c_float()
+    _check(lib().mg_autotune(graph._h, algo, len(src), _ptr(src), reps, C.byref(o), C.byref(a), C.byref(b)))
+    return o, a.value, b.value
+
+
+def batch_autotune(graph: Graph, kind: int, sources, slots=None, reps: int = 3, **kw):
+    """(best Options, ms with the defaults, ms tuned) for one batched run"""
+    src = _i32(sources)
+    sl = _i32(slots) if slots is not None else None
+    o = default_options(**kw)
+    a, b = C.c_float(), C.c_float()
+    _check(lib().mg_batch_autotune(graph._h, kind, len(src), _ptr(src), _ptr(sl) if sl is not None else None, reps,
+                                   C.byref(o), C.byref(a), C.byref(b)))
+    return o, a.value, b.value

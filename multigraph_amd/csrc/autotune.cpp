@@ -1,0 +1,120 @@
+// Autotuner: picks mg_options for one graph and algorithm by timing candidates on the
+// graph itself.  The reference re-decides push vs pull every level from wall-clock
+// samples of both (process.cu:1161-1255, pseudo_* kernels); here the decision rule is
+// deterministic and its thresholds, with the grid shape, the levels per captured
+// graph, the PageRank partition size and the batched chunk size, are searched once.
+//
+// Search: coordinate descent from the starting options, one pass over the
+// coordinates in the order listed; each candidate is built, run once to warm up, then
+// timed `reps` times (median, HIP events inside mg_run / mg_batch_run).
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "mg_internal.hpp"
+
+namespace {
+
+struct Coord {
+    std::function<void(mg_options &, double)> set;
+    std::vector<double> values;
+};
+
+float median(std::vector<float> v) {
+    std::sort(v.begin(), v.end());
+    return v.empty() ? 0.f : v[v.size() / 2];
+}
+
+// cost of one option set: summed median times over the sources (engines) or of the
+// batch run; < 0 on error
+using CostFn = std::function<int(const mg_options &, float *)>;
+
+int descend(const std::vector<Coord> &coords, const CostFn &cost, mg_options *best, float *ms_default, float *ms_tuned) {
+    float base = 0.f;
+    int rc = cost(*best, &base);
+    if (rc != MG_OK) return rc;
+    if (ms_default) *ms_default = base;
+    float cur = base;
+    for (const Coord &c : coords) {
+        mg_options keep = *best;
+        for (double v : c.values) {
+            mg_options o = *best;
+            c.set(o, v);
+            if (memcmp(&o, best, sizeof(o)) == 0) continue;
+            float t = 0.f;
+            if (cost(o, &t) != MG_OK) continue;  // a candidate that does not fit is skipped
+            if (t < cur) { cur = t; keep = o; }
+        }
+        *best = keep;
+    }
+    if (ms_tuned) *ms_tuned = cur;
+    return MG_OK;
+}
+
+}  // namespace
+
+extern "C" int mg_autotune(const mg_graph *g, int algo, int nsrc, const int32_t *sources, int reps, mg_options *best,
+                           float *ms_default, float *ms_tuned) {
+    if (!g || !best || (nsrc > 0 && !sources)) return mg_fail(MG_ERR_ARG, "null argument");
+    if (algo < MG_BFS || algo > MG_PR_D) return mg_fail(MG_ERR_ARG, "unknown algorithm %d", algo);
+    if (reps < 1) reps = 1;
+    std::vector<int> src(sources, sources + std::max(0, nsrc));
+    if (src.empty()) src.push_back(g->start_p);
+    CostFn cost = [&](const mg_options &o, float *out) -> int {
+        mg_engine *e = nullptr;
+        int rc = mg_build(g, algo, &o, &e);
+        if (rc != MG_OK) return rc;
+        float total = 0.f;
+        for (int s : src) {
+            float ms = 0.f;
+            if ((rc = mg_run(e, s, &ms)) != MG_OK) break;  // warm-up (graph capture, caches)
+            std::vector<float> t;
+            for (int r = 0; r < reps && rc == MG_OK; ++r) { rc = mg_run(e, s, &ms); t.push_back(ms); }
+            total += median(t);
+            if (algo == MG_CC || algo == MG_PR_T || algo == MG_PR_D) break;  // no source
+        }
+        mg_release(e);
+        *out = total;
+        return rc;
+    };
+    std::vector<Coord> coords;
+    auto grid = Coord{[](mg_options &o, double v) { o.grid_blocks = (int)v; }, {1024, 2048, 4096}};
+    if (algo == MG_BFS || algo == MG_SSSP || algo == MG_BC) {
+        coords.push_back({[](mg_options &o, double v) { o.alpha = (float)v; }, {4, 8, 15, 30, 60}});
+        coords.push_back({[](mg_options &o, double v) { o.beta = (float)v; }, {6, 12, 18, 36, 72}});
+        coords.push_back(grid);
+        coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {0, 4, 16}});
+    } else if (algo == MG_CC) {
+        coords.push_back(grid);
+    } else {
+        coords.push_back({[](mg_options &o, double v) { o.part_log2 = (int)v; }, {0, 11, 12, 13}});
+        coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {4, 8, 16}});
+    }
+    return descend(coords, cost, best, ms_default, ms_tuned);
+}
+
+extern "C" int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32_t *sources, const int32_t *slots, int reps,
+                                 mg_options *best, float *ms_default, float *ms_tuned) {
+    if (!g || !best || n < 1 || !sources) return mg_fail(MG_ERR_ARG, "null argument");
+    if (reps < 1) reps = 1;
+    CostFn cost = [&](const mg_options &o, float *out) -> int {
+        mg_batch *b = nullptr;
+        int rc = mg_batch_build(g, kind, 64, &o, &b);
+        if (rc != MG_OK) return rc;
+        float ms = 0.f;
+        auto run = [&]() { return slots ? mg_batch_run_slots(b, n, sources, slots, &ms) : mg_batch_run(b, n, sources, &ms); };
+        rc = run();  // warm-up (graph capture)
+        std::vector<float> t;
+        for (int r = 0; r < reps && rc == MG_OK; ++r) { rc = run(); t.push_back(ms); }
+        mg_batch_release(b);
+        *out = median(t);
+        return rc;
+    };
+    std::vector<Coord> coords;
+    coords.push_back({[](mg_options &o, double v) { o.alpha = (float)v; }, {4, 8, 15, 30, 60, 120}});
+    coords.push_back({[](mg_options &o, double v) { o.batch_chunk = (int)v; }, {256, 512, 1024, 4096}});
+    coords.push_back({[](mg_options &o, double v) { o.grid_blocks = (int)v; }, {1024, 2048, 4096}});
+    if (kind != MG_BATCH_BC) coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {2, 8, 32}});
+    return descend(coords, cost, best, ms_default, ms_tuned);
+}

@@ -59,12 +59,15 @@ __global__ void k_fill_f32b(float *p, long long n, float v) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// seed: source s at vertex src[s] (duplicates share the vertex)
-__global__ void k_ms_seed(int nsrc, const int *__restrict__ src, uint64_t *__restrict__ vis, uint64_t *__restrict__ F,
-                          int *__restrict__ lev, double *__restrict__ sigma, int *__restrict__ dist) {
-    const int s = threadIdx.x;
-    if (s >= nsrc) return;
-    const int v = src[s];
+// seed: source i at vertex src[i] in slot (bit) slot[i], or bit i without a slot list.
+// Several sources may share a vertex or a slot (a slot with several sources runs from
+// their set, which for sources in disjoint components is one run per component).
+__global__ void k_ms_seed(int n, const int *__restrict__ src, const int *__restrict__ slot, uint64_t *__restrict__ vis,
+                          uint64_t *__restrict__ F, int *__restrict__ lev, double *__restrict__ sigma, int *__restrict__ dist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = slot ? slot[i] : i;
+    const int v = src[i];
     atomicOr((unsigned long long *)&vis[v], 1ull << s);
     if (F) atomicOr((unsigned long long *)&F[v], 1ull << s);
     if (lev) lev[(long long)v * 64 + s] = 0;
@@ -125,15 +128,46 @@ __global__ __launch_bounds__(kBB) void k_nz_write(int nv, const uint64_t *__rest
 // one chunk writes its results directly; the chunks of a split vertex merge theirs
 // with atomics (or, and, add, min) into state that starts the pass at its identity.
 struct Chunk { int v, p0, p1, multi; };
-constexpr int kCH = 1024;
+constexpr int kCH = 1024;  // default edges per chunk (mg_options.batch_chunk)
+
+// Device-side level control (BFS and SSSP runs): the level kernels read the current
+// level from here, so K levels can be captured in one hipGraph and replayed past the
+// end of the run (every kernel returns at once when `done` is set).
+struct BCtl {
+    int it;       // level (BFS) or sweep (SSSP) being processed
+    int done;
+    int push;     // BFS: this level pushes (1) or pulls (0)
+    int pad;
+    long long n, mf;  // size and degree sum of the last produced frontier
+    unsigned long long all;  // the slots in use
+};
+
+// A level kernel gets two frontier buffers (A, B).  Without a control block A is the
+// current frontier and B the next one; with one they are F[0], F[1] and the level's
+// parity picks: current = F[(it - 1) & 1], next = F[it & 1].
+__device__ __forceinline__ void cur_next(const BCtl *ctl, uint64_t *A, uint64_t *B, uint64_t **cur, uint64_t **next) {
+    if (!ctl) { *cur = A; *next = B; return; }
+    const int it = ctl->it;
+    *cur = ((it - 1) & 1) ? B : A;
+    *next = (it & 1) ? B : A;
+}
+// want: -1 any level, 1 push levels only, 0 pull levels only
+__device__ __forceinline__ bool level_runs(const BCtl *ctl, int want) {
+    if (!ctl) return true;
+    if (ctl->done) return false;
+    return want < 0 || ctl->push == want;
+}
 
 // push: a wave per chunk whose vertex u is in the frontier.  BFS: lanes over edges,
 // new bits or-ed into Fn.  BC: lanes = sources, edges in turn, sigma[u][s] added
 // into sigma[w][s] for every new bit s.
 template <bool kBC>
-__global__ __launch_bounds__(kBB) void k_mb_push(long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
-                                                 const uint64_t *__restrict__ F, const uint64_t *__restrict__ vis,
-                                                 uint64_t *__restrict__ Fn, double *__restrict__ sigma) {
+__global__ __launch_bounds__(kBB) void k_mb_push(const BCtl *ctl, long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
+                                                 uint64_t *FA, const uint64_t *__restrict__ vis,
+                                                 uint64_t *FB, double *__restrict__ sigma) {
+    if (!level_runs(ctl, 1)) return;
+    uint64_t *F, *Fn;
+    cur_next(ctl, FA, FB, &F, &Fn);
     const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * kBB) >> 6;
     for (long long c = wave; c < nch; c += nw) {
@@ -161,9 +195,13 @@ __global__ __launch_bounds__(kBB) void k_mb_push(long long nch, const Chunk *__r
 
 // pull, BFS: a wave per chunk of a vertex still missing bits; lanes or the frontier
 // words of 64 neighbours at a time and stop once the missing bits are covered
-__global__ __launch_bounds__(kBB) void k_mb_pull_bfs(long long nch, const Chunk *__restrict__ ch, uint64_t all,
-                                                     const int *__restrict__ col, const uint64_t *__restrict__ F,
-                                                     const uint64_t *__restrict__ vis, uint64_t *__restrict__ Fn) {
+__global__ __launch_bounds__(kBB) void k_mb_pull_bfs(const BCtl *ctl, long long nch, const Chunk *__restrict__ ch, uint64_t all,
+                                                     const int *__restrict__ col, uint64_t *FA,
+                                                     const uint64_t *__restrict__ vis, uint64_t *FB) {
+    if (!level_runs(ctl, 0)) return;
+    uint64_t *F, *Fn;
+    cur_next(ctl, FA, FB, &F, &Fn);
+    if (ctl) all = ctl->all;
     const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * kBB) >> 6;
     for (long long c = wave; c < nch; c += nw) {
@@ -233,9 +271,13 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
 
 // after a push or pull level: keep only new bits, mark them visited, record levels,
 // and leave the new frontier's size and degree sum as per-workgroup partials
-__global__ __launch_bounds__(kBB) void k_mb_apply(int nv, const int *__restrict__ row, uint64_t *__restrict__ vis,
-                                                  uint64_t *__restrict__ Fn, int *__restrict__ lev, int it,
+__global__ __launch_bounds__(kBB) void k_mb_apply(const BCtl *ctl, int nv, const int *__restrict__ row, uint64_t *__restrict__ vis,
+                                                  uint64_t *FA, uint64_t *FB, int *__restrict__ lev, int it,
                                                   int *__restrict__ part_n, long long *__restrict__ part_e) {
+    if (!level_runs(ctl, -1)) return;
+    uint64_t *F, *Fn;
+    cur_next(ctl, FA, FB, &F, &Fn);
+    if (ctl) it = ctl->it;
     int cnt = 0;
     long long mf = 0;
     for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB) {
@@ -257,9 +299,13 @@ __global__ __launch_bounds__(kBB) void k_mb_apply(int nv, const int *__restrict_
     if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
 }
 
-// size and degree sum of the nonzero words of W
-__global__ __launch_bounds__(kBB) void k_mb_count(int nv, const int *__restrict__ row, const uint64_t *__restrict__ W,
+// size and degree sum of the nonzero words of the next frontier (without a control
+// block: of WB)
+__global__ __launch_bounds__(kBB) void k_mb_count(const BCtl *ctl, int nv, const int *__restrict__ row, uint64_t *WA, uint64_t *WB,
                                                   int *__restrict__ part_n, long long *__restrict__ part_e) {
+    if (!level_runs(ctl, -1)) return;
+    uint64_t *cur, *W;
+    cur_next(ctl, WA, WB, &cur, &W);
     int cnt = 0;
     long long mf = 0;
     for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB)
@@ -268,6 +314,48 @@ __global__ __launch_bounds__(kBB) void k_mb_count(int nv, const int *__restrict_
     const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
     const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
     if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+}
+
+// zero the next frontier (a fill kernel: graphs record no memset nodes)
+__global__ void k_mb_fill_next(const BCtl *ctl, uint64_t *FA, uint64_t *FB, long long n) {
+    if (!level_runs(ctl, -1)) return;
+    uint64_t *cur, *next;
+    cur_next(ctl, FA, FB, &cur, &next);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) next[i] = 0ull;
+}
+
+__global__ void k_mb_ctl_init(BCtl *ctl, unsigned long long all) {
+    BCtl c{};
+    c.all = all;
+    *ctl = c;
+}
+
+// between levels: total the partials of the last produced frontier; stop when it is
+// empty, else open the next level and choose its direction (BFS: push while the
+// frontier's degree sum is below E / alpha)
+__global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe, int nparts,
+                         long long ne, float alpha, int forced, int symmetric) {
+    if (ctl->done) return;
+    __shared__ long long s_tmp[256 / MG_WAVE];
+    long long a = 0, b = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) { a += pn[i]; b += pe[i]; }
+    a = block_sum<256 / MG_WAVE>(a, s_tmp);
+    b = block_sum<256 / MG_WAVE>(b, s_tmp);
+    if (threadIdx.x != 0) return;
+    BCtl c = *ctl;
+    c.n = a;
+    c.mf = b;
+    if (a == 0) {
+        c.done = 1;
+    } else {
+        c.it += 1;
+        int push = (double)b < (double)ne / alpha;
+        if (forced == MG_MODE_SPARSE) push = 1;
+        if (forced == MG_MODE_DENSE) push = 0;
+        if (!symmetric) push = 1;
+        c.push = push;
+    }
+    *ctl = c;
 }
 
 __global__ void k_sum_parts(const int *__restrict__ pn, const long long *__restrict__ pe, int n, long long *__restrict__ out) {
@@ -335,13 +423,14 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
     }
 }
 
-// bc[v] += sum over sources s != v of delta[v][s]
-__global__ void k_ms_bc_sum(int nv, int nsrc, const int *__restrict__ src, const float *__restrict__ delta, float *__restrict__ bc) {
+// bc[v] += sum over slots s of delta[v][s], except the slots v is a source of (level 0)
+__global__ void k_ms_bc_sum(int nv, int nslots, const int *__restrict__ lev, const float *__restrict__ delta, float *__restrict__ bc) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nw = (gridDim.x * blockDim.x) >> 6;
     for (int v = wave; v < nv; v += nw) {
         KSum k;
-        if (lane() < nsrc && src[lane()] != v) k.add(delta[(long long)v * 64 + lane()]);
+        const long long kv = (long long)v * 64 + lane();
+        if (lane() < nslots && lev[kv] != 0) k.add(delta[kv]);
         k = wave_ksum(k);
         if (lane() == 0) bc[v] += k.value();
     }
@@ -350,9 +439,12 @@ __global__ void k_ms_bc_sum(int nv, int nsrc, const int *__restrict__ src, const
 // ---- SSSP --------------------------------------------------------------------------
 // a wave per chunk, lanes = sources: relax dist[v][s] from neighbours whose changed
 // bit s is set; the vertex's new changed word is the ballot of improvements
-__global__ __launch_bounds__(kBB) void k_mb_sssp(long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
-                                                 const int *__restrict__ wgt, const uint64_t *__restrict__ C,
-                                                 uint64_t *__restrict__ Cn, int *__restrict__ dist) {
+__global__ __launch_bounds__(kBB) void k_mb_sssp(const BCtl *ctl, long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
+                                                 const int *__restrict__ wgt, uint64_t *CA,
+                                                 uint64_t *CB, int *__restrict__ dist) {
+    if (!level_runs(ctl, 0)) return;
+    uint64_t *C, *Cn;
+    cur_next(ctl, CA, CB, &C, &Cn);
     __shared__ int s_u[kBB], s_w[kBB];
     __shared__ uint64_t s_f[kBB];
     const int wl = threadIdx.x & ~(MG_WAVE - 1);
@@ -391,6 +483,38 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp(long long nch, const Chunk *__r
     }
 }
 
+// push sweep (few changed vertices): a wave per chunk of a vertex u with changed bits,
+// lanes = sources; along each edge (u, w) lane s lowers dist[w][s] to
+// dist[u][s] + weight (read first, atomicMin only when it improves) and the
+// improvements are or-ed into w's next changed word
+__global__ __launch_bounds__(kBB) void k_mb_sssp_push(const BCtl *ctl, long long nch, const Chunk *__restrict__ ch,
+                                                      const int *__restrict__ col, const int *__restrict__ wgt, uint64_t *CA,
+                                                      uint64_t *CB, int *__restrict__ dist) {
+    if (!level_runs(ctl, 1)) return;
+    uint64_t *C, *Cn;
+    cur_next(ctl, CA, CB, &C, &Cn);
+    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * kBB) >> 6;
+    for (long long c = wave; c < nch; c += nw) {
+        const Chunk k = ch[c];
+        const uint64_t cu = C[k.v];
+        if (!cu) continue;
+        const bool mine = (cu >> lane()) & 1ull;
+        const int du = mine ? dist[(long long)k.v * 64 + lane()] : (int)MG_INF;
+        for (int p = k.p0; p < k.p1; ++p) {
+            const int w = col[p];
+            const int nd = du + wgt[p];
+            bool imp = false;
+            if (mine) {
+                int *dw = &dist[(long long)w * 64 + lane()];
+                if (nd < *dw) imp = atomicMin(dw, nd) > nd;
+            }
+            const uint64_t m = ballot(imp);
+            if (lane() == 0 && m) atomicOr((unsigned long long *)&Cn[w], (unsigned long long)m);
+        }
+    }
+}
+
 // vertex-major [v][64] -> source-major [s][v]
 template <class T>
 __global__ void k_ms_transpose(int nv, int nsrc, const T *__restrict__ in, T *__restrict__ out) {
@@ -412,7 +536,10 @@ struct mg_batch {
     hipStream_t stream = nullptr;
     int *row = nullptr, *col = nullptr, *w = nullptr;
     uint64_t *vis = nullptr, *F[2] = {nullptr, nullptr}, *D = nullptr, *E = nullptr;
-    int *lev = nullptr, *dist = nullptr, *src = nullptr;
+    int *lev = nullptr, *dist = nullptr, *src = nullptr, *slot = nullptr;
+    int seed_cap = 0, nseeds = 0;  // device capacity of src/slot, sources of the last run
+    bool slotted = false;          // the last run gave an explicit slot list
+    uint64_t all = 0;              // the slots in use
     double *sigma = nullptr;
     float *delta = nullptr, *bc = nullptr;
     Chunk *chunks = nullptr;      // static edge chunks
@@ -422,17 +549,22 @@ struct mg_batch {
     long long log_cap = 0;
     int *part_n = nullptr;
     long long *part_e = nullptr, *tot = nullptr, *h_tot = nullptr;
-    int nsrc = 0, levels = 0;
+    int nsrc = 0, levels = 0;     // slots of the last run, levels it swept
     bool symmetric = true;        // pull levels need every edge stored both ways
+    BCtl *ctl = nullptr, *h_ctl = nullptr;  // device level control (BFS, SSSP)
+    hipGraphExec_t gexec = nullptr;         // K captured levels
+    int gexec_levels = 0;
 };
 
 static void batch_free(mg_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    void *ptrs[] = {b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->sigma,
-                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_e, b->tot};
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    void *ptrs[] = {b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
+                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_e, b->tot, b->ctl};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
+    if (b->h_ctl) (void)hipHostFree(b->h_ctl);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -456,14 +588,15 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     if (b->opt.grid_blocks > 4096) b->opt.grid_blocks = 4096;  // per-workgroup partial arrays hold 4096
     auto fail = [&](int rc) { batch_free(b); return rc; };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "stream"));
-    // static edge chunks (a hub's edges are cut into pieces of kCH)
+    // static edge chunks (a hub's edges are cut into pieces of at most `chunk`)
+    const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;
     std::vector<Chunk> ch;
-    ch.reserve((size_t)g->nv + (size_t)(g->ne / kCH) + 1);
+    ch.reserve((size_t)g->nv + (size_t)(g->ne / chunk) + 1);
     for (int v = 0; v < g->nv; ++v) {
         const int p0 = g->row[v], p1 = g->row[v + 1];
         if (p1 == p0) continue;
-        const int multi = (p1 - p0) > kCH ? 1 : 0;
-        for (int p = p0; p < p1; p += kCH) ch.push_back(Chunk{v, p, std::min(p1, p + kCH), multi});
+        const int multi = (p1 - p0) > chunk ? 1 : 0;
+        for (int p = p0; p < p1; p += chunk) ch.push_back(Chunk{v, p, std::min(p1, p + chunk), multi});
     }
     b->nch = (long long)ch.size();
     const long long nv = b->nv, ne = std::max<long long>(1, b->ne), slots = nv * 64 + 64;
@@ -472,10 +605,12 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
               hipMalloc(&b->vis, sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
               hipMalloc(&b->F[0], sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
               hipMalloc(&b->F[1], sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
-              hipMalloc(&b->src, sizeof(int) * 64) == hipSuccess && hipMalloc(&b->part_n, sizeof(int) * 4096) == hipSuccess &&
+              hipMalloc(&b->part_n, sizeof(int) * 4096) == hipSuccess &&
               hipMalloc(&b->part_e, sizeof(long long) * 4096) == hipSuccess &&
               hipMalloc(&b->tot, sizeof(long long) * 4) == hipSuccess &&
-              hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess &&
+              hipMalloc(&b->ctl, sizeof(BCtl)) == hipSuccess &&
+              hipHostMalloc(&b->h_ctl, sizeof(BCtl), hipHostMallocDefault) == hipSuccess;
     if (ok && kind != MG_BATCH_SSSP) ok = hipMalloc(&b->lev, sizeof(int) * slots) == hipSuccess;
     if (ok && kind == MG_BATCH_SSSP)
         ok = hipMalloc(&b->w, sizeof(int) * ne) == hipSuccess && hipMalloc(&b->dist, sizeof(int) * slots) == hipSuccess;
@@ -545,11 +680,11 @@ static int read_counts(mg_batch *b, int nparts, long long *n, long long *mf) {
     return MG_OK;
 }
 
-static int run_bfs_bc(mg_batch *b, int nsrc) {
+static int run_bfs_bc(mg_batch *b) {
     hipStream_t s = b->stream;
     const int nv = b->nv;
     const bool bc = b->kind == MG_BATCH_BC;
-    const uint64_t all = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
+    const uint64_t all = b->all;
     const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
     const float alpha = b->opt.alpha > 0 ? b->opt.alpha : 15.0f;
     const long long slots = (long long)nv * 64 + 64;
@@ -564,10 +699,11 @@ static int run_bfs_bc(mg_batch *b, int nsrc) {
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->D, (long long)nv + 1, 0ull);
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->E, (long long)nv + 1, 0ull);
     }
-    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, nsrc, b->src, b->vis, b->F[0], b->lev, bc ? b->sigma : nullptr, (int *)nullptr);
+    hipLaunchKernelGGL(k_ms_seed, dim3(grid_for(b->nseeds, 256, 4096)), dim3(256), 0, s, b->nseeds, b->src,
+                       b->slotted ? b->slot : nullptr, b->vis, b->F[0], b->lev, bc ? b->sigma : nullptr, (int *)nullptr);
     std::vector<long long> off{0};  // BC level log offsets
     long long n = 0, mf = 0;
-    hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, nv, b->row, b->F[0], b->part_n, b->part_e);
+    hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->F[1], b->F[0], b->part_n, b->part_e);
     MG_TRY(read_counts(b, gv, &n, &mf));
     if (bc) { long long c0 = 0; MG_TRY(compact(b, b->F[0], 0, &c0)); off.push_back(c0); }
     int cur = 0, it = 1;
@@ -579,13 +715,13 @@ static int run_bfs_bc(mg_batch *b, int nsrc) {
         if (!b->symmetric) push = true;
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, Fn, (long long)nv + 1, 0ull);
         if (push) {
-            if (bc) hipLaunchKernelGGL((k_mb_push<true>), dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
-            else hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
+            if (bc) hipLaunchKernelGGL((k_mb_push<true>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
+            else hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
         } else {
             if (bc) hipLaunchKernelGGL(k_mb_pull_bc, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, all, b->col, F, b->vis, Fn, b->sigma);
-            else hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, all, b->col, F, b->vis, Fn);
+            else hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, all, b->col, F, b->vis, Fn);
         }
-        hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, nv, b->row, b->vis, Fn, b->lev, it, b->part_n, b->part_e);
+        hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->vis, F, Fn, b->lev, it, b->part_n, b->part_e);
         MG_TRY(read_counts(b, gv, &n, &mf));
         if (bc && n > 0) { long long c = 0; MG_TRY(compact(b, Fn, off.back(), &c)); off.push_back(off.back() + c); }
         cur ^= 1;
@@ -605,45 +741,116 @@ static int run_bfs_bc(mg_batch *b, int nsrc) {
         hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->E, 1);
         if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], b->D, 1);
     }
-    hipLaunchKernelGGL(k_ms_bc_sum, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, nsrc, b->src, b->delta, b->bc);
+    hipLaunchKernelGGL(k_ms_bc_sum, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, b->nsrc, b->lev, b->delta, b->bc);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }
 
-static int run_sssp(mg_batch *b, int nsrc) {
+// BFS (without BC) and SSSP: levels run under the device control block, K at a time
+// from a captured hipGraph; the host reads the control block once per K levels
+static int capture_levels(mg_batch *b, int K, hipGraphExec_t *out) {
     hipStream_t s = b->stream;
     const int nv = b->nv;
     const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
-    const long long slots = (long long)nv * 64 + 64;
     const int gv = grid_for(nv, kBB, G);
     const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
-    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
-    hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->dist, slots, (int)MG_INF);
-    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[0], (long long)nv + 1, 0ull);
-    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, nsrc, b->src, b->vis, b->F[0], (int *)nullptr, (double *)nullptr, b->dist);
-    int cur = 0, it = 0;
-    for (;;) {
-        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[cur ^ 1], (long long)nv + 1, 0ull);
-        hipLaunchKernelGGL(k_mb_sssp, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, b->w, b->F[cur], b->F[cur ^ 1], b->dist);
-        hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, nv, b->row, b->F[cur ^ 1], b->part_n, b->part_e);
-        long long n = 0;
-        MG_TRY(read_counts(b, gv, &n, nullptr));
-        cur ^= 1;
-        ++it;
-        if (n == 0) break;
+    const float alpha = b->opt.alpha > 0 ? b->opt.alpha : 15.0f;
+    const bool sssp = b->kind == MG_BATCH_SSSP;
+    hipGraph_t g = nullptr;
+    MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < K; ++k) {
+        hipLaunchKernelGGL(k_mb_ctl, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, gv, b->ne, alpha, b->opt.mode,
+                           b->symmetric ? 1 : 0);
+        hipLaunchKernelGGL(k_mb_fill_next, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, (const BCtl *)b->ctl, b->F[0],
+                           b->F[1], (long long)nv + 1);
+        if (sssp) {
+            hipLaunchKernelGGL(k_mb_sssp_push, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col, b->w,
+                               b->F[0], b->F[1], b->dist);
+            hipLaunchKernelGGL(k_mb_sssp, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col, b->w, b->F[0],
+                               b->F[1], b->dist);
+            hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->F[0], b->F[1], b->part_n,
+                               b->part_e);
+        } else {
+            hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col,
+                               b->F[0], b->vis, b->F[1], (double *)nullptr);
+            hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, 0ull, b->col,
+                               b->F[0], b->vis, b->F[1]);
+            hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->vis, b->F[0], b->F[1],
+                               b->lev, 0, b->part_n, b->part_e);
+        }
     }
-    b->levels = it;
-    MG_HIP(hipGetLastError());
+    hipError_t ce = hipStreamEndCapture(s, &g);
+    if (ce != hipSuccess) return mg_fail(MG_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) return mg_fail(MG_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
     return MG_OK;
 }
 
-extern "C" int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float *ms) {
-    if (!b || (nsrc > 0 && !sources)) return mg_fail(MG_ERR_ARG, "null argument");
-    if (nsrc < 1 || nsrc > b->max_sources) return mg_fail(MG_ERR_LIMIT, "batch of %d sources (1..%d)", nsrc, b->max_sources);
-    for (int i = 0; i < nsrc; ++i)
+static int run_levels(mg_batch *b) {
+    hipStream_t s = b->stream;
+    const int nv = b->nv;
+    const bool sssp = b->kind == MG_BATCH_SSSP;
+    const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
+    const int gv = grid_for(nv, kBB, G);
+    const long long slots = (long long)nv * 64 + 64;
+    const int K = b->opt.graph_levels > 0 ? b->opt.graph_levels : 8;
+    if (!b->gexec || b->gexec_levels != K) {  // captured once per batch, on an idle stream
+        if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        MG_HIP(hipStreamSynchronize(s));
+        MG_TRY(capture_levels(b, K, &b->gexec));
+        b->gexec_levels = K;
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[0], (long long)nv + 1, 0ull);
+    if (sssp) hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->dist, slots, (int)MG_INF);
+    else hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->lev, slots, (int)MG_INF);
+    hipLaunchKernelGGL(k_ms_seed, dim3(grid_for(b->nseeds, 256, 4096)), dim3(256), 0, s, b->nseeds, b->src,
+                       b->slotted ? b->slot : nullptr, b->vis, b->F[0], sssp ? (int *)nullptr : b->lev, (double *)nullptr,
+                       sssp ? b->dist : (int *)nullptr);
+    hipLaunchKernelGGL(k_mb_ctl_init, dim3(1), dim3(1), 0, s, b->ctl, (unsigned long long)b->all);
+    hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->F[1], b->F[0], b->part_n,
+                       b->part_e);
+    MG_HIP(hipGetLastError());
+    for (;;) {
+        MG_HIP(hipGraphLaunch(b->gexec, s));
+        MG_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s));
+        MG_HIP(hipStreamSynchronize(s));
+        if (b->h_ctl->done) break;
+    }
+    b->levels = b->h_ctl->it;
+    return MG_OK;
+}
+
+static int batch_run(mg_batch *b, int n, const int32_t *sources, const int32_t *slots, float *ms) {
+    if (!b || (n > 0 && !sources)) return mg_fail(MG_ERR_ARG, "null argument");
+    if (n < 1) return mg_fail(MG_ERR_ARG, "a batch needs at least one source");
+    if (!slots && n > b->max_sources) return mg_fail(MG_ERR_LIMIT, "batch of %d sources (1..%d)", n, b->max_sources);
+    int nslots = slots ? 0 : n;
+    uint64_t all = 0;
+    for (int i = 0; i < n; ++i) {
         if (sources[i] < 0 || sources[i] >= b->nv) return mg_fail(MG_ERR_ARG, "source %d outside [0,%d)", sources[i], b->nv);
-    MG_HIP(hipMemcpy(b->src, sources, sizeof(int) * nsrc, hipMemcpyHostToDevice));
-    b->nsrc = nsrc;
+        const int k = slots ? slots[i] : i;
+        if (k < 0 || k >= b->max_sources) return mg_fail(MG_ERR_LIMIT, "slot %d outside [0,%d)", k, b->max_sources);
+        nslots = std::max(nslots, k + 1);
+        all |= 1ull << k;
+    }
+    if (n > b->seed_cap) {
+        if (b->src) hipFree(b->src);
+        if (b->slot) hipFree(b->slot);
+        b->src = b->slot = nullptr;
+        b->seed_cap = 0;
+        const int cap = std::max(64, n);
+        MG_HIP(hipMalloc(&b->src, sizeof(int) * cap));
+        MG_HIP(hipMalloc(&b->slot, sizeof(int) * cap));
+        b->seed_cap = cap;
+    }
+    MG_HIP(hipMemcpy(b->src, sources, sizeof(int) * n, hipMemcpyHostToDevice));
+    if (slots) MG_HIP(hipMemcpy(b->slot, slots, sizeof(int) * n, hipMemcpyHostToDevice));
+    b->slotted = slots != nullptr;
+    b->nseeds = n;
+    b->nsrc = nslots;
+    b->all = all;
     if (b->kind == MG_BATCH_BC)
         hipLaunchKernelGGL(k_fill_f32b, dim3(grid_for(b->nv + 1, 256, 4096)), dim3(256), 0, b->stream, b->bc, (long long)b->nv + 1, 0.0f);
     MG_HIP(hipStreamSynchronize(b->stream));
@@ -651,7 +858,7 @@ extern "C" int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float
     MG_HIP(hipEventCreate(&e0));
     MG_HIP(hipEventCreate(&e1));
     MG_HIP(hipEventRecord(e0, b->stream));
-    int rc = b->kind == MG_BATCH_SSSP ? run_sssp(b, nsrc) : run_bfs_bc(b, nsrc);
+    int rc = b->kind == MG_BATCH_BC ? run_bfs_bc(b) : run_levels(b);
     hipEventRecord(e1, b->stream);
     hipError_t se = hipEventSynchronize(e1);
     float t = 0.f;
@@ -662,6 +869,15 @@ extern "C" int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float
     if (se != hipSuccess) return mg_fail(MG_ERR_DEVICE, "batch run failed: %s", hipGetErrorString(se));
     if (ms) *ms = t;
     return MG_OK;
+}
+
+extern "C" int mg_batch_run(mg_batch *b, int nsrc, const int32_t *sources, float *ms) {
+    return batch_run(b, nsrc, sources, nullptr, ms);
+}
+
+extern "C" int mg_batch_run_slots(mg_batch *b, int n, const int32_t *sources, const int32_t *slots, float *ms) {
+    if (!slots) return mg_fail(MG_ERR_ARG, "null slot list");
+    return batch_run(b, n, sources, slots, ms);
 }
 
 extern "C" int mg_batch_get(mg_batch *b, int which, void *dst, int64_t bytes) {

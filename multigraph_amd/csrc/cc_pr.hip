@@ -345,7 +345,7 @@ __global__ void k_pr_ctl(Ctl *ctl, int max_iter, long long ne) {
     c.edges += ne;
     c.levels = c.iter;
     if (!c.changed || (max_iter > 0 && c.iter >= max_iter)) c.mode = MODE_DONE;
-    else { c.iter += 1; c.changed = 0; c.mf = 0; }
+    else { c.iter += 1; c.changed = 0; c.mf = 0; c.resid = 0.0; }
     *ctl = c;
 }
 
@@ -521,6 +521,7 @@ int mg_pr_build(mg_engine *e, const mg_graph *g) {
 int mg_pr_reset(mg_engine *e) {
     const long long n = e->pr_pb ? e->pr_nvi : e->nv;
     hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, e->stream, (int)n, e->rank, e->cval, e->vfc, e->ctl);
+    if (e->pr_pb) MG_TRY(mg_pb_reset(e));
     MG_HIP(hipGetLastError());
     MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
     MG_HIP(hipStreamSynchronize(e->stream));
@@ -532,8 +533,7 @@ int mg_pr_reset(mg_engine *e) {
 
 static int pr_launch_iteration(mg_engine *e) {
     hipStream_t s = e->stream;
-    if (e->pr_pb) {
-        hipLaunchKernelGGL(k_pr_begin, dim3(1), dim3(1), 0, s, e->ctl);
+    if (e->pr_pb) {  // the control kernel clears the flags for the next iteration
         MG_TRY(mg_pb_iteration(e));
         hipLaunchKernelGGL(k_pr_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->opt.pr_max_iter, e->ne);
         MG_HIP(hipGetLastError());
@@ -573,7 +573,20 @@ int mg_pr_step(mg_engine *e, int *more) {
     return MG_OK;
 }
 
+// K iterations of the partitioned layout in one hipGraph: every kernel of an
+// iteration returns at once when the control block says the run is over, so the graph
+// can be replayed past convergence and the host checks the control block once per K
 int mg_pr_capture(mg_engine *e, int iters, hipGraphExec_t *out) {
-    (void)e; (void)iters; (void)out;
-    return mg_fail(MG_ERR_STATE, "PageRank graph capture is not available in this build");
+    if (!e->pr_pb) return mg_fail(MG_ERR_STATE, "only the partitioned PageRank layout runs from a captured graph");
+    hipGraph_t g = nullptr;
+    MG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = MG_OK;
+    for (int i = 0; i < iters && rc == MG_OK; ++i) rc = pr_launch_iteration(e);
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc != MG_OK) { if (g) hipGraphDestroy(g); return rc; }
+    if (ce != hipSuccess) return mg_fail(MG_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) return mg_fail(MG_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    return MG_OK;
 }

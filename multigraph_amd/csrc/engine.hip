@@ -53,8 +53,8 @@ static void release_all(mg_engine *e) {
     if (e->gexec) hipGraphExecDestroy(e->gexec);
     void *ptrs[] = {e->part_n, e->part_mf, e->eoff, e->tsum, e->cstart, e->pcstart, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->q, e->bm[0], e->bm[1],
                     e->vis, e->chunks, e->lvl_off, e->aux, e->ctl, e->urow, e->ucol, e->rank, e->cval, e->deg, e->psum,
-                    e->pb.a_src, e->pb.b_dst, e->pb.amap, e->pb.msg, e->pb.a_part, e->pb.b_part, e->pb.unitsA,
-                    e->pb.unitsB, e->pb.split_parts, e->pb.unitsA64, e->pb.unitsB64};
+                    e->pb.a_src, e->pb.b_dst, e->pb.amap, e->pb.msg, e->pb.gbase, e->pb.unitsA,
+                    e->pb.unitsB, e->pb.split_parts, e->pb.apart};
     for (void *p : ptrs) mg_free_dev(p);
     if (e->h_ctl) (void)hipHostFree(e->h_ctl);
     for (auto &ev : e->ev_pool) hipEventDestroy(ev);
@@ -173,12 +173,15 @@ extern "C" int mg_run(mg_engine *e, int source, float *ms) {
     MG_HIP(hipEventCreate(&b));
     MG_HIP(hipEventRecord(a, e->stream));
     const bool frontier = e->algo == MG_BFS || e->algo == MG_SSSP || e->algo == MG_BC;
-    const int K = e->opt.graph_levels;
+    // PageRank over the partitioned layout always replays captured iterations (8 per
+    // host check unless graph_levels says otherwise); profiling times launches eagerly
+    const bool pr_graph = (e->algo == MG_PR_T || e->algo == MG_PR_D) && e->pr_pb;
+    const int K = frontier ? e->opt.graph_levels : (pr_graph ? (e->opt.graph_levels > 0 ? e->opt.graph_levels : 8) : 0);
     int rc = MG_OK;
-    if (frontier && K > 0 && !e->prof) {
+    if ((frontier || pr_graph) && K > 0 && !e->prof) {
         if (!e->gexec || e->gexec_levels != K) {
             if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
-            rc = mg_frontier_capture(e, K, &e->gexec);
+            rc = frontier ? mg_frontier_capture(e, K, &e->gexec) : mg_pr_capture(e, K, &e->gexec);
             e->gexec_levels = K;
         }
         while (rc == MG_OK) {

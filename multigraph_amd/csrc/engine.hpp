@@ -118,19 +118,19 @@ struct mg_pb_layout {
     int P = 0;                // ids per partition
     int np = 0;               // partitions
     int G = 16;               // entries per mapper group
-    long long slots = 0;      // padded stream length (both streams)
-    uint16_t *a_src = nullptr;   // src-major stream: local source id (0xFFFF = padding)
-    uint16_t *b_dst = nullptr;   // dst-major stream: local destination id (0xFFFF = padding)
-    int *amap = nullptr;         // per group of G A-slots: base of the matching B-slots
-    float *msg = nullptr;        // per-slot message, written in A order into B positions
-    long long *a_part = nullptr; // np+1: start of each source partition in stream A
-    long long *b_part = nullptr; // np+1: start of each destination partition in stream B
-    // work units (chunks of the streams), built on the host from a_part / b_part
+    long long slots = 0;      // edge slots of the destination-major stream (tiles padded to G)
+    long long pairs = 0;      // message slots: one per (source vertex, destination partition)
+    uint16_t *a_src = nullptr;   // per message slot, source-major order: local source id (0xFFFF = padding)
+    int *amap = nullptr;         // per group of G source-major message slots: their destination-major position
+    float *msg = nullptr;        // per message slot, destination-major order
+    uint16_t *b_dst = nullptr;   // per edge slot: local destination id | (starts a new message) << 15
+    int *gbase = nullptr;        // per group of G edge slots (+1): message index of the group's first slot
+    // work units (chunks of the streams), built on the host from the partition starts
     int nA = 0, nB = 0, nApply = 0;
-    int4 *unitsA = nullptr;      // {partition, begin_lo, end_lo, begin_hi/end_hi packed} see pr.hip
-    int4 *unitsB = nullptr;
+    int4 *unitsA = nullptr;      // {partition, first message slot, end, 0}
+    int4 *unitsB = nullptr;      // {partition, first edge slot, end, split}
     int *split_parts = nullptr;  // destination partitions processed by more than one unit
-    long long *unitsA64 = nullptr, *unitsB64 = nullptr;
+    int *apart = nullptr;        // np + 1: first source-major message slot of every partition
 };
 
 struct mg_engine {
@@ -214,6 +214,7 @@ int mg_pr_build(mg_engine *e, const mg_graph *g);
 int mg_pr_reset(mg_engine *e);
 int mg_pr_step(mg_engine *e, int *more);
 int mg_pr_capture(mg_engine *e, int iters, hipGraphExec_t *out);
+int mg_pb_reset(mg_engine *e);
 int mg_pb_build(mg_engine *e, const mg_graph *g);
 int mg_pb_build_alloc(mg_engine *e, const mg_graph *g);
 int mg_pb_iteration(mg_engine *e);

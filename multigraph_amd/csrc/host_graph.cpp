@@ -65,32 +65,84 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
         if (src[i] < 0 || src[i] >= nv || dst[i] < 0 || dst[i] >= nv)
             return mg_fail(MG_ERR_RANGE, "A vertex id is out of range %d %d", src[i], dst[i]);
     }
-    // counting sort by src, stable in expanded-entry order
-    std::vector<int64_t> start((size_t)nv + 1, 0);
-    for (int64_t i = 0; i < nnz; ++i) {
-        start[(size_t)src[i] + 1]++;
-        if (sym) start[(size_t)dst[i] + 1]++;
-    }
-    for (int32_t v = 0; v < nv; ++v) start[v + 1] += start[v];
-    std::vector<int32_t> sd((size_t)n), sw((size_t)n);
+    // Counting sort by src, stable in expanded-entry order, in two parallel passes:
+    // entries are first split by src into buckets of consecutive vertices (each thread
+    // takes a contiguous slice of the input and writes its bucket runs in order), then
+    // every bucket is counting-sorted on its own vertex range.
+    const int T = std::max(1, std::min<int>(mg_host_threads(), (int)std::min<int64_t>(64, (nnz >> 16) + 1)));
+    int shift = 0;
+    while (((int64_t)std::max(nv - 1, 0) >> shift) >= 4096) ++shift;
+    const int nb = (int)(((int64_t)std::max(nv - 1, 0) >> shift) + 1);
+    std::vector<int64_t> bcnt((size_t)T * nb + 1, 0);  // [bucket][thread]
+    auto slice = [&](int t, int64_t *lo, int64_t *hi) {
+        const int64_t per = (nnz + T - 1) / T;
+        *lo = std::min<int64_t>(nnz, per * t);
+        *hi = std::min<int64_t>(nnz, *lo + per);
+    };
+    auto run_threads = [&](auto fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < T; ++t) th.emplace_back([&, t]() { fn(t); });
+        fn(0);
+        for (auto &x : th) x.join();
+    };
+    run_threads([&](int t) {
+        int64_t lo, hi;
+        slice(t, &lo, &hi);
+        for (int64_t i = lo; i < hi; ++i) {
+            bcnt[(size_t)(src[i] >> shift) * T + t]++;
+            if (sym) bcnt[(size_t)(dst[i] >> shift) * T + t]++;
+        }
+    });
     {
-        std::vector<int64_t> pos(start.begin(), start.end() - 1);
-        for (int64_t i = 0; i < nnz; ++i) {
-            int64_t p = pos[src[i]]++;
-            sd[p] = dst[i]; sw[p] = val ? val[i] : 1;
+        int64_t run = 0;
+        for (size_t k = 0; k < (size_t)T * nb; ++k) { const int64_t c = bcnt[k]; bcnt[k] = run; run += c; }
+        bcnt[(size_t)T * nb] = run;
+    }
+    std::vector<int32_t> xs((size_t)n), xd((size_t)n), xw((size_t)n);
+    run_threads([&](int t) {
+        int64_t lo, hi;
+        slice(t, &lo, &hi);
+        std::vector<int64_t> cur((size_t)nb);
+        for (int k = 0; k < nb; ++k) cur[k] = bcnt[(size_t)k * T + t];
+        for (int64_t i = lo; i < hi; ++i) {
+            const int32_t wv = val ? val[i] : 1;
+            int64_t q = cur[src[i] >> shift]++;
+            xs[q] = src[i]; xd[q] = dst[i]; xw[q] = wv;
             if (sym) {
-                int64_t q = pos[dst[i]]++;
-                sd[q] = src[i]; sw[q] = val ? val[i] : 1;
+                q = cur[dst[i] >> shift]++;
+                xs[q] = dst[i]; xd[q] = src[i]; xw[q] = wv;
             }
         }
-    }
+    });
+    std::vector<int64_t> start((size_t)nv + 1, 0);
+    // buckets of low RMAT ids hold far more entries: hand them out dynamically
+    auto each_bucket = [&](auto fn) {
+        std::atomic<int64_t> next{0};
+        run_threads([&](int) { for (int64_t bk; (bk = next.fetch_add(1)) < nb;) fn(bk); });
+    };
+    each_bucket([&](int64_t bk) {
+        for (int64_t q = bcnt[(size_t)bk * T]; q < bcnt[(size_t)(bk + 1) * T]; ++q) start[(size_t)xs[q] + 1]++;
+    });
+    for (int32_t v = 0; v < nv; ++v) start[v + 1] += start[v];
+    std::vector<int32_t> sd((size_t)n), sw((size_t)n);
+    each_bucket([&](int64_t bk) {
+        const int64_t v0 = bk << shift, v1 = std::min<int64_t>(nv, (bk + 1) << shift);
+        std::vector<int64_t> pos(start.begin() + v0, start.begin() + v1);
+        for (int64_t q = bcnt[(size_t)bk * T]; q < bcnt[(size_t)(bk + 1) * T]; ++q) {
+            const int64_t p = pos[xs[q] - v0]++;
+            sd[p] = xd[q]; sw[p] = xw[q];
+        }
+    });
+    std::vector<int32_t>().swap(xs);
+    std::vector<int32_t>().swap(xd);
+    std::vector<int32_t>().swap(xw);
     // stable sort of each row by dst: sort (dst, position) pairs
     std::vector<uint8_t> keep((size_t)n, 0);
     std::vector<uint8_t> mark((size_t)nv + 1, 0);
     int64_t first_nonempty = -1;
     int32_t first_row = -1;
     for (int32_t v = 0; v < nv; ++v) if (start[v + 1] > start[v]) { first_nonempty = start[v]; first_row = v; break; }
-    mg_parallel_for(nv, [&](int64_t b, int64_t e) {
+    mg_parallel_dyn(nv, [&](int64_t b, int64_t e) {
         std::vector<uint64_t> key;
         std::vector<int32_t> td, tw;
         for (int64_t v = b; v < e; ++v) {
@@ -108,7 +160,7 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
         }
     }, 1024);
     // keep / valid marks (read_input.cu:105-114)
-    mg_parallel_for(nv, [&](int64_t b, int64_t e) {
+    mg_parallel_dyn(nv, [&](int64_t b, int64_t e) {
         for (int64_t v = b; v < e; ++v) {
             for (int64_t p = start[v]; p < start[v + 1]; ++p) {
                 if (p == first_nonempty) { keep[p] = 1; continue; }
@@ -139,7 +191,7 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
     g->nv = cnv;
     g->row.assign((size_t)cnv + 1, 0);
     std::vector<int64_t> rowcnt((size_t)nv, 0);
-    mg_parallel_for(nv, [&](int64_t b, int64_t e) {
+    mg_parallel_dyn(nv, [&](int64_t b, int64_t e) {
         for (int64_t v = b; v < e; ++v) {
             int64_t c = 0;
             for (int64_t p = start[v]; p < start[v + 1]; ++p) c += keep[p];
@@ -164,7 +216,7 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
     // fill rows of compacted vertices without kept edges (read_input.cu:167-170)
     for (int32_t v = 1; v <= cnv; ++v) if (g->row[v] < g->row[v - 1]) g->row[v] = g->row[v - 1];
     g->row[cnv] = (int32_t)ne;
-    mg_parallel_for(nv, [&](int64_t b, int64_t e) {
+    mg_parallel_dyn(nv, [&](int64_t b, int64_t e) {
         for (int64_t v = b; v < e; ++v) {
             int64_t o = rstart[v];
             for (int64_t p = start[v]; p < start[v + 1]; ++p) if (keep[p]) {
@@ -290,20 +342,21 @@ extern "C" int mg_graph_gen_rmat(int scale, int edge_factor, double a, double b,
     if (draws * 2 > (int64_t)INT32_MAX) return mg_fail(MG_ERR_LIMIT, "RMAT too large for the int32 CSR");
     std::vector<int32_t> s((size_t)draws), d((size_t)draws), w((size_t)draws);
     std::vector<uint8_t> self((size_t)draws, 0);
-    const double ab = a + b, abc = a + b + c;
+    // quadrant choice per level from 16 random bits (4 levels per 64-bit draw), with the
+    // probabilities quantised to 1/65536
+    const uint32_t ta = (uint32_t)std::llround(a * 65536.0), tab = (uint32_t)std::llround((a + b) * 65536.0),
+                   tabc = (uint32_t)std::llround((a + b + c) * 65536.0);
     const uint64_t base = mg_splitmix64(seed ^ 0x2545F4914F6CDD1DULL);
     mg_parallel_for(draws, [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k) {
             uint64_t st = mg_splitmix64(base ^ (uint64_t)k * 0x9E3779B97F4A7C15ULL);
             uint32_t u = 0, v = 0;
+            uint64_t bits = 0;
             for (int bit = 0; bit < scale; ++bit) {
-                st = mg_splitmix64(st);
-                double r = (double)(st >> 11) * (1.0 / 9007199254740992.0);
-                uint32_t qu, qv;
-                if (r < a) { qu = 0; qv = 0; }
-                else if (r < ab) { qu = 0; qv = 1; }
-                else if (r < abc) { qu = 1; qv = 0; }
-                else { qu = 1; qv = 1; }
+                if ((bit & 3) == 0) { st = mg_splitmix64(st); bits = st; }
+                const uint32_t r = (uint32_t)(bits & 0xffff);
+                bits >>= 16;
+                const uint32_t qu = r >= tab, qv = (r >= ta) & ((r < tab) | (r >= tabc));
                 u = (u << 1) | qu;
                 v = (v << 1) | qv;
             }

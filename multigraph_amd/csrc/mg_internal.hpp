@@ -3,6 +3,7 @@
 #include <stdint.h>
 #include <string>
 #include <vector>
+#include <atomic>
 #include <thread>
 #include <algorithm>
 #include <cstdlib>
@@ -60,6 +61,24 @@ void mg_parallel_for(int64_t n, F f, int64_t grain = 1 << 16) {
         if (b >= e) break;
         th.emplace_back([=, &f]() { f(b, e); });
     }
+    for (auto &x : th) x.join();
+}
+
+// Same contract, but [0, n) is handed out in pieces of `grain` from a shared counter:
+// for loops over vertices whose cost follows a skewed degree distribution.
+template <class F>
+void mg_parallel_dyn(int64_t n, F f, int64_t grain = 4096) {
+    int T = mg_host_threads();
+    const int64_t chunks = (n + grain - 1) / grain;
+    if (T > chunks) T = (int)chunks;
+    if (T <= 1) { if (n > 0) f((int64_t)0, n); return; }
+    std::atomic<int64_t> next{0};
+    auto body = [&]() {
+        for (int64_t c; (c = next.fetch_add(1)) < chunks;) f(c * grain, std::min<int64_t>(n, (c + 1) * grain));
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(body);
+    body();
     for (auto &x : th) x.join();
 }
 

@@ -3,22 +3,28 @@
 //
 // Layout.  Vertices are cut into np partitions of P ids.  An edge of row x with
 // column y carries rank mass from y (source) to x (destination); it belongs to tile
-// (s, d) = (y / P, x / P).  Every tile is stored twice, with the same slot order:
-//   stream A (source-major: s, then d) holds the local source id (y mod P, 16 bit),
-//   stream B (destination-major: d, then s) holds the local destination id (x mod P).
-// Tiles are padded to a multiple of G = 16 slots (padding id 0xFFFF), and amap[g]
-// gives, for the g-th group of 16 A-slots, the position of the matching 16 B-slots.
+// (s, d) = (y / P, x / P).  Inside a tile the edges are sorted by source, and every
+// run of edges with one source shares one message: a message slot stands for a
+// (source vertex, destination partition) pair.  Streams, each tile padded to G = 16:
+//   a_src (message slots, source-major: s, then d): local source id (16 bit);
+//   amap[g]: for the g-th group of 16 a_src slots, the position of their messages
+//            in the destination-major message array msg;
+//   b_dst (edge slots, destination-major: d, then s): local destination id, with bit
+//            15 set where a new message (source) starts;
+//   gbase[g]: message index of the first edge slot of the g-th group of 16.
 //
 // One iteration = two streaming passes, no random HBM access and no global atomics:
 //   scatter: per source partition, stage c = rank / degree of its P vertices in LDS,
-//            stream A and write msg[B position] = c[local source] (16-slot runs);
-//   gather:  per destination partition, accumulate msg into P FP64 LDS accumulators
-//            (native LDS FP64 atomic add) while streaming B, then apply the PR_T /
-//            PR_D rule to the partition's vertices and write rank and the next c.
+//            stream a_src and write msg[amap position] = c[local source];
+//   gather:  per destination partition, stage the messages of 8192 edge slots at a
+//            time in LDS, stream b_dst and add each edge's message into P FP64 LDS
+//            accumulators (native LDS FP64 atomic add), then apply the PR_T / PR_D
+//            rule to the partition's vertices and write the rank.
 // Work is cut into units of about equal slot counts; a destination partition whose
 // stream is cut into several units adds its partial sums into an FP64 vector instead
-// and is applied by a separate pass.  Traffic: 2 + 4 + 0.25 B per slot in the scatter,
-// 2 + 4 B per slot in the gather, plus 20 B per vertex.
+// and is applied by a separate pass.  Traffic per iteration, with M message slots and
+// S edge slots: 6.25 B * M in the scatter, 4 B * M + 2.25 B * S in the gather, plus
+// 20 B per vertex.  On RMAT graphs M is about half of S.
 #include <algorithm>
 #include <vector>
 #include "engine.hpp"
@@ -28,9 +34,17 @@ using namespace mg;
 
 namespace {
 
-constexpr int kG = 16;          // slots per mapper group
+constexpr int kG = 8;           // slots per group of either stream (one 16-byte vector of ids)
 constexpr int kBT = 512;        // threads per workgroup of the layout and phase kernels
 constexpr int kBuildEdges = 1 << 16;
+
+// order this wave's LDS stores before its loads of other lanes' slots, and its loads
+// before the next stores
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 int grid_for(long long items, int per_block, int cap) {
     long long g = (items + per_block - 1) / per_block;
@@ -39,8 +53,11 @@ int grid_for(long long items, int per_block, int cap) {
     return (int)g;
 }
 
-__global__ void k_fill_u16(uint16_t *p, long long n) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0xFFFF;
+__global__ void k_fill_u16(uint16_t *p, long long n, uint16_t v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, long long n, uint32_t v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // build work item: rows [x0, x1) of destination partition d
@@ -71,24 +88,30 @@ __global__ __launch_bounds__(kBT) void k_pb_count(const int *__restrict__ col, c
     }
 }
 
-// padded tile sizes in destination-major (pb) and source-major (pa) order
-__global__ void k_pb_pad(const int *__restrict__ count, int np, long long *__restrict__ pb, long long *__restrict__ pa) {
+// padded edge slots per tile (destination-major)
+__global__ void k_pb_pad(const int *__restrict__ count, long long nt, long long *__restrict__ pb) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (long long)gridDim.x * blockDim.x)
+        pb[t] = ((count[t] + kG - 1) / kG) * kG;
+}
+
+// padded message slots per tile, in destination-major (pb) and source-major (pa) order
+__global__ void k_pb_pad_pairs(const int *__restrict__ npair, int np, long long *__restrict__ pb, long long *__restrict__ pa) {
     const long long n = (long long)np * np;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-        const long long c = ((count[t] + kG - 1) / kG) * kG;
+        const long long c = ((npair[t] + kG - 1) / kG) * kG;
         const long long d = t / np, s = t % np;
         pb[t] = c;
         pa[s * np + d] = c;
     }
 }
 
-// pass 2: place every edge; ranks inside a tile come from a per-tile cursor reserved
-// once per (item, tile) and LDS counters inside the item
-__global__ __launch_bounds__(kBT) void k_pb_scatter(const int *__restrict__ col, const int *__restrict__ row,
-                                                    const BItem *__restrict__ items, int nitems, int P, int np,
-                                                    int *__restrict__ cursor, const long long *__restrict__ boff,
-                                                    const long long *__restrict__ aoff, uint16_t *__restrict__ a_src,
-                                                    uint16_t *__restrict__ b_dst, int *__restrict__ amap) {
+// pass 2: place every edge of an item as key (local source << 16 | local destination)
+// in its tile; ranks inside a tile come from a per-tile cursor reserved once per
+// (item, tile) and LDS counters inside the item
+__global__ __launch_bounds__(kBT) void k_pb_place(const int *__restrict__ col, const int *__restrict__ row,
+                                                  const BItem *__restrict__ items, int nitems, int P, int np,
+                                                  int *__restrict__ cursor, const long long *__restrict__ boff,
+                                                  uint32_t *__restrict__ keys) {
     extern __shared__ int s_mem[];
     int *s_hist = s_mem;              // np
     int *s_row = s_mem + np;          // item rows + 1 (<= P + 1)
@@ -111,13 +134,87 @@ __global__ __launch_bounds__(kBT) void k_pb_scatter(const int *__restrict__ col,
             const int s = y / P;
             const int x = b.x0 + row_of(s_row, nr + 1, e);
             const int k = atomicAdd(&s_hist[s], 1);
-            const long long tb = (long long)b.d * np + s, ta = (long long)s * np + b.d;
-            const long long jb = boff[tb] + k, ia = aoff[ta] + k;
-            b_dst[jb] = (uint16_t)(x - b.d * P);
-            a_src[ia] = (uint16_t)(y - s * P);
-            if ((k % kG) == 0) amap[ia / kG] = (int)(boff[tb] + k);
+            keys[boff[(long long)b.d * np + s] + k] = ((uint32_t)(y - s * P) << 16) | (uint32_t)(x - b.d * P);
         }
         __syncthreads();
+    }
+}
+
+// pass 3: sort every tile's keys (bitonic network in LDS) and count its messages
+// (distinct sources); tiles above kSortMax keys are left to the host (npair = -1)
+constexpr int kSortMax = 8192;
+__global__ __launch_bounds__(kBT) void k_pb_tile_sort(long long nt, const int *__restrict__ count, const long long *__restrict__ boff,
+                                                      uint32_t *__restrict__ keys, int *__restrict__ npair) {
+    __shared__ uint32_t s_k[kSortMax];
+    __shared__ int s_tmp[kBT / MG_WAVE];
+    for (long long t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int n = count[t];
+        if (n == 0 || n > kSortMax) {
+            if (threadIdx.x == 0) npair[t] = n == 0 ? 0 : -1;
+            continue;
+        }
+        int n2 = MG_WAVE;
+        while (n2 < n) n2 <<= 1;
+        uint32_t *k = keys + boff[t];
+        for (int i = threadIdx.x; i < n2; i += kBT) s_k[i] = i < n ? k[i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (int size = 2; size <= n2; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = threadIdx.x; i < n2; i += kBT) {
+                    const int j = i ^ stride;
+                    if (j > i) {
+                        const uint32_t a = s_k[i], b = s_k[j];
+                        if ((a > b) == ((i & size) == 0)) { s_k[i] = b; s_k[j] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        int c = 0;
+        for (int i = threadIdx.x; i < n; i += kBT) {
+            c += (i == 0 || (s_k[i] >> 16) != (s_k[i - 1] >> 16)) ? 1 : 0;
+            k[i] = s_k[i];
+        }
+        c = block_sum<kBT / MG_WAVE>(c, s_tmp);
+        if (threadIdx.x == 0) npair[t] = c;
+    }
+}
+
+// pass 4: per tile, number the messages (a block scan of the new-source flags) and
+// write b_dst, gbase, a_src and amap
+__global__ __launch_bounds__(kBT) void k_pb_emit(long long nt, int np, const int *__restrict__ count,
+                                                 const long long *__restrict__ boff, const long long *__restrict__ poffB,
+                                                 const long long *__restrict__ poffA, const uint32_t *__restrict__ keys,
+                                                 uint16_t *__restrict__ b_dst, int *__restrict__ gbase,
+                                                 uint16_t *__restrict__ a_src, int *__restrict__ amap) {
+    __shared__ int s_tmp[kBT / MG_WAVE + 1];
+    for (long long t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int n = count[t];
+        if (n == 0) continue;
+        const long long d = t / np, s = t % np;
+        const long long eb = boff[t], pb0 = poffB[t], pa0 = poffA[s * np + d];
+        int carry = 0;  // messages started before this chunk of the tile
+        for (int c0 = 0; c0 < n; c0 += kBT) {
+            const int j = c0 + threadIdx.x;
+            uint32_t key = 0;
+            int flag = 0;
+            if (j < n) {
+                key = keys[eb + j];
+                flag = (j == 0 || (keys[eb + j - 1] >> 16) != (key >> 16)) ? 1 : 0;
+            }
+            int tot;
+            const int pre = block_excl_scan<kBT / MG_WAVE>(flag, s_tmp, &tot);
+            const int pi = carry + pre + flag - 1;  // message of slot j
+            if (j < n) {
+                b_dst[eb + j] = (uint16_t)((key & 0x7FFFu) | ((uint32_t)flag << 15));
+                if (flag) {
+                    a_src[pa0 + pi] = (uint16_t)(key >> 16);
+                    if ((pi % kG) == 0) amap[(pa0 + pi) / kG] = (int)(pb0 + pi);
+                }
+                if (((eb + j) % kG) == 0) gbase[(eb + j) / kG] = (int)(pb0 + pi);
+            }
+            carry += tot;
+            __syncthreads();
+        }
     }
 }
 
@@ -126,12 +223,70 @@ __global__ __launch_bounds__(kBT) void k_pb_scatter(const int *__restrict__ col,
 // partition p holds local ids [0, part_count(p)) under the round-robin numbering
 __device__ __forceinline__ int part_count(int p, int nreal, int np) { return nreal / np + (p < nreal % np ? 1 : 0); }
 
+// Messages of one source partition: c (rank / degree of its vertices) is staged in
+// LDS, the partition's source-major message slots [a0, a1) are streamed, and every
+// group of 8 is written into its destination-major run.  8 slots (one group) per
+// thread and step: one 16-byte id load, two 16-byte streaming stores; padding writes
+// 0.  The next group's loads are issued before this group is written (two register
+// sets, no copies); loads past the range repeat its last group.
+template <int NT>
+__device__ __forceinline__ void scatter_part(int a0, int a1, const float *__restrict__ s_c, const uint16_t *__restrict__ a_src,
+                                             const int *__restrict__ amap, float *__restrict__ msg) {
+    const int step = NT * kG;
+    auto ld = [&](int i, uint4 &id, int &base) {
+        const int c = min(i, a1 - kG);
+        id = ld_stream_u4(&a_src[c]);
+        base = amap[c / kG];
+    };
+    auto emit = [&](int i, const uint4 &id, int base) {
+        if (i >= a1) return;
+        const uint32_t w[4] = {id.x, id.y, id.z, id.w};
+        float m[8];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t a = w[k] & 0xFFFFu, b = w[k] >> 16;
+            m[2 * k] = a != 0xFFFFu ? s_c[a] : 0.f;
+            m[2 * k + 1] = b != 0xFFFFu ? s_c[b] : 0.f;
+        }
+        // streaming stores: the messages are read once, by the next gather
+        st_stream_f4(&msg[base], m[0], m[1], m[2], m[3]);
+        st_stream_f4(&msg[base + 4], m[4], m[5], m[6], m[7]);
+    };
+    int i = a0 + (int)threadIdx.x * kG;
+    if (i < a1) {
+        uint4 ia, ib;
+        int ba, bb;
+        ld(i, ia, ba);
+        for (;;) {
+            ld(i + step, ib, bb);
+            emit(i, ia, ba);
+            i += step;
+            if (i >= a1) break;
+            ld(i + step, ia, ba);
+            emit(i, ib, bb);
+            i += step;
+            if (i >= a1) break;
+        }
+    }
+}
+
+// message buffer an iteration reads (its number's parity) and the one it writes
+__device__ __forceinline__ const float *msg_in(const Ctl *ctl, const float *msg, long long pairs) {
+    return msg + (long long)(ctl->iter & 1) * pairs;
+}
+__device__ __forceinline__ float *msg_out(const Ctl *ctl, float *msg, long long pairs) {
+    return msg + (long long)((ctl->iter + 1) & 1) * pairs;
+}
+
+// messages of partitions whose destination side was split over several gather units
+// (their ranks are final only after k_pb_apply_split); unit = {partition, a0, a1, 0}
 __global__ __launch_bounds__(kBT) void k_pb_scatter_msg(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
                                                         const float *__restrict__ rank, const int *__restrict__ deg,
                                                         const uint16_t *__restrict__ a_src, const int *__restrict__ amap,
-                                                        float *__restrict__ msg) {
+                                                        float *__restrict__ msg, long long pairs) {
     if (ctl->mode == MODE_DONE) return;
     extern __shared__ float s_c[];
+    float *out = msg_out(ctl, msg, pairs);
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int4 un = units[u];
         const int v0 = un.x * P;
@@ -142,31 +297,7 @@ __global__ __launch_bounds__(kBT) void k_pb_scatter_msg(Ctl *ctl, const int4 *__
             s_c[k] = c;
         }
         __syncthreads();
-        // 8 slots per lane: one 16-byte id load, two 16-byte message stores into the
-        // matching B run (8 slots never straddle a 16-slot group); padding writes 0
-        for (int i0 = un.y + threadIdx.x * 8; i0 < un.z; i0 += kBT * 16) {
-            const int i1 = i0 + kBT * 8;
-            const bool has1 = i1 < un.z;
-            const uint4 ida = ld_stream_u4(&a_src[i0]);
-            const int basea = amap[i0 / kG] + (i0 % kG);
-            uint4 idb = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-            int baseb = 0;
-            if (has1) { idb = ld_stream_u4(&a_src[i1]); baseb = amap[i1 / kG] + (i1 % kG); }
-            const uint32_t w[8] = {ida.x, ida.y, ida.z, ida.w, idb.x, idb.y, idb.z, idb.w};
-            float m[16];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t a = w[k] & 0xFFFFu, b = w[k] >> 16;
-                m[2 * k] = a != 0xFFFFu ? s_c[a] : 0.f;
-                m[2 * k + 1] = b != 0xFFFFu ? s_c[b] : 0.f;
-            }
-            *(float4 *)&msg[basea] = make_float4(m[0], m[1], m[2], m[3]);
-            *(float4 *)&msg[basea + 4] = make_float4(m[4], m[5], m[6], m[7]);
-            if (has1) {
-                *(float4 *)&msg[baseb] = make_float4(m[8], m[9], m[10], m[11]);
-                *(float4 *)&msg[baseb + 4] = make_float4(m[12], m[13], m[14], m[15]);
-            }
-        }
+        scatter_part<kBT>(un.y, un.z, s_c, a_src, amap, out);
         __syncthreads();
     }
 }
@@ -182,59 +313,179 @@ __device__ __forceinline__ int pr_apply(int v, double sum, int variant, float da
     return ch;
 }
 
-__global__ __launch_bounds__(kBT) void k_pb_gather(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
+// Gather work: every wavefront walks segments of kSeg groups (512 edge slots) of its
+// unit on its own.  A segment's messages are one contiguous range of at most
+// 512 + 8 entries (pairs never outnumber edge slots inside a tile, and only the first
+// tile of a segment can start mid-tile); the wave stages them in its own LDS slice and
+// adds them into the block's FP64 accumulators.  The next segment's ids and messages
+// are loaded into registers before the current one is processed, and no block-wide
+// barrier is taken between segments.
+constexpr int kBG = 1024;                 // threads per gather workgroup
+constexpr int kSeg = MG_WAVE;             // groups per wave segment: one per lane
+constexpr int kStageM = (kSeg * kG + 16 + MG_WAVE - 1) / MG_WAVE;  // message loads per lane
+constexpr int kStageW = kStageM * MG_WAVE;                            // staged messages per wave
+constexpr int kStage = (kBG / MG_WAVE) * kStageW;           // staged messages per workgroup
+
+// A segment's loads are split in two stages whose results are only used one
+// iteration later, so the in-order wait counter never stalls on loads just issued.
+struct SegHdr {
+    uint4 id;     // this lane's 8 edge slots (lanes past the segment repeat its last group)
+    int ga, gb;   // gbase at the segment's first group and one past its last
+    int gl;       // gbase at this lane's group
+    int pad;      // this lane is past the segment: its slots count as padding
+};
+struct Seg {
+    uint4 id;
+    int base;     // message index of the lane's first slot, relative to the staged range
+    int n;        // staged messages
+    float m[kStageM];
+};
+
+// unconditional loads only (no register is written after a load is issued)
+__device__ __forceinline__ void load_hdr(SegHdr &h, int gs, int ge, const uint16_t *__restrict__ b_dst,
+                                         const int *__restrict__ gbase) {
+    const int g = min(gs + lane(), ge - 1);
+    h.pad = gs + lane() >= ge;
+    h.ga = gbase[gs];
+    h.gb = gbase[ge];
+    h.id = ld_stream_u4(&b_dst[(long long)g * kG]);
+    h.gl = gbase[g];
+}
+
+// loads every staged message (indices past the range repeat the last one: no branches)
+__device__ __forceinline__ void load_msg(Seg &sg, const SegHdr &h, const float *__restrict__ msg, long long npairs) {
+    const long long q0 = h.ga;
+    const long long q1 = min(npairs, (long long)h.gb + 1);
+    sg.n = (int)(q1 - q0);
+    const uint32_t pm = h.pad ? 0x7FFF7FFFu : 0u;
+    sg.id = make_uint4(h.id.x | pm, h.id.y | pm, h.id.z | pm, h.id.w | pm);
+    sg.base = h.pad ? 0 : h.gl - h.ga;
+#pragma unroll
+    for (int k = 0; k < kStageM; ++k) sg.m[k] = ld_stream(&msg[q0 + min(lane() + k * MG_WAVE, sg.n - 1)]);
+}
+
+constexpr int kPer = 16;  // vertices per gather thread: P <= kPer * kBG
+
+__global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
                                                    int variant, float damping, float tol,
-                                                   const uint16_t *__restrict__ b_dst, const float *__restrict__ msg,
+                                                   const uint16_t *__restrict__ b_dst, const int *__restrict__ gbase,
+                                                   float *__restrict__ msgbuf, long long npairs,
+                                                   const int *__restrict__ apart, const uint16_t *__restrict__ a_src,
+                                                   const int *__restrict__ amap,
                                                    const int *__restrict__ deg, float *__restrict__ rank,
                                                    int *__restrict__ active, double *__restrict__ psum) {
     if (ctl->mode == MODE_DONE) return;
-    extern __shared__ double s_acc[];
-    __shared__ double s_tmp[kBT / MG_WAVE];
+    const float *msg = msg_in(ctl, msgbuf, npairs);
+    float *out = msg_out(ctl, msgbuf, npairs);
+    extern __shared__ double s_acc[];  // P + 1 FP64 accumulators, then kStageW staged messages per wave
+    const int wid = threadIdx.x / MG_WAVE;
+    float *s_msg = (float *)(s_acc + P + 1) + wid * kStageW;
+    __shared__ double s_tmp[kBG / MG_WAVE];
     double res = 0.0;
     int ch = 0;
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int4 un = units[u];
         const int v0 = un.x * P;
         const int pv = part_count(un.x, nv, np);
-        for (int k = threadIdx.x; k < P; k += kBT) s_acc[k] = 0.0;
+        for (int k = threadIdx.x; k <= P; k += kBG) s_acc[k] = 0.0;
         __syncthreads();
-        // 8 slots per lane; the next vector's loads are issued before this vector's
-        // LDS atomics, so HBM requests stay in flight while the atomics run
-        int j = un.y + threadIdx.x * 8;
-        bool have = j < un.z;
-        uint4 nid = make_uint4(0u, 0u, 0u, 0u);
-        float4 nm0 = make_float4(0.f, 0.f, 0.f, 0.f), nm1 = nm0;
-        if (have) { nid = ld_stream_u4(&b_dst[j]); nm0 = ld_stream_f4(&msg[j]); nm1 = ld_stream_f4(&msg[j + 4]); }
-        while (have) {
-            const uint4 id = nid;
-            const float4 m0 = nm0, m1 = nm1;
-            const int jn = j + kBT * 8;
-            const bool have_n = jn < un.z;
-            if (have_n) { nid = ld_stream_u4(&b_dst[jn]); nm0 = ld_stream_f4(&msg[jn]); nm1 = ld_stream_f4(&msg[jn + 4]); }
-            const uint32_t w[4] = {id.x, id.y, id.z, id.w};
-            const float m[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+        const int g0 = un.y / kG, g1 = un.z / kG;
+        const int nseg = (g1 - g0 + kSeg - 1) / kSeg;
+        // three-stage pipeline over this wave's segments s, s + W, s + 2W: headers two
+        // ahead, messages one ahead, so each dependent load has a segment's time to land
+        constexpr int W = kBG / MG_WAVE;
+        auto seg_lo = [&](int si) { return g0 + si * kSeg; };
+        auto seg_hi = [&](int si) { return min(g1, g0 + (si + 1) * kSeg); };
+        // Rotation over register sets, unrolled so no in-flight value is ever copied;
+        // loads past the last segment repeat it (no exec-mask branches).
+        auto hdr = [&](SegHdr &h, int si) {
+            si = min(si, nseg - 1);
+            load_hdr(h, seg_lo(si), seg_hi(si), b_dst, gbase);
+        };
+        double hub = 0.0;
+        auto process = [&](const Seg &sg) {
+            wave_sync();  // the previous segment's staged messages are consumed
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t a = w[k] & 0xFFFFu, b = w[k] >> 16;
-                if (a != 0xFFFFu) atomicAdd(&s_acc[a], (double)m[2 * k]);
-                if (b != 0xFFFFu) atomicAdd(&s_acc[b], (double)m[2 * k + 1]);
+            for (int k = 0; k < kStageM; ++k) s_msg[lane() + k * MG_WAVE] = sg.m[k];
+            wave_sync();
+            const uint32_t w[4] = {sg.id.x, sg.id.y, sg.id.z, sg.id.w};
+            int pi = sg.base;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t v = (k & 1) ? (w[k >> 1] >> 16) : (w[k >> 1] & 0xFFFFu);
+                if (k > 0) pi += (int)(v >> 15);
+                const uint32_t dst = v & 0x7FFFu;
+                const double m = (double)s_msg[pi];
+                // local id 0 is the partition's highest-degree vertex: it takes ~10% of
+                // the edges, and lanes of one instruction adding to one LDS address
+                // serialise, so its sum stays in a register; padding (0x7FFF) adds nothing
+                if (dst == 0) hub += m;
+                else if (dst != 0x7FFFu) atomicAdd(&s_acc[dst], m);
             }
-            j = jn;
-            have = have_n;
+        };
+        int sgi = wid;
+        if (sgi < nseg) {
+            // three message buffers: while one segment is processed, the messages of
+            // the next two and the header of the third are in flight
+            SegHdr h;
+            Seg a, b, c;
+            hdr(h, sgi);
+            load_msg(a, h, msg, npairs);
+            hdr(h, sgi + W);
+            load_msg(b, h, msg, npairs);
+            hdr(h, sgi + 2 * W);
+            for (;;) {
+                load_msg(c, h, msg, npairs);
+                hdr(h, sgi + 3 * W);
+                process(a);
+                sgi += W;
+                if (sgi >= nseg) break;
+                load_msg(a, h, msg, npairs);
+                hdr(h, sgi + 3 * W);
+                process(b);
+                sgi += W;
+                if (sgi >= nseg) break;
+                load_msg(b, h, msg, npairs);
+                hdr(h, sgi + 3 * W);
+                process(c);
+                sgi += W;
+                if (sgi >= nseg) break;
+            }
         }
+        hub = wave_sum(hub);
+        if (lane() == 0 && hub != 0.0) atomicAdd(&s_acc[0], hub);
         __syncthreads();
         if (un.w) {  // partition split over several units: add partials, applied later
-            for (int k = threadIdx.x; k < pv; k += kBT) {
+            for (int k = threadIdx.x; k < pv; k += kBG) {
                 const double a = s_acc[k];
                 if (a != 0.0) atomicAdd(&psum[v0 + k], a);
             }
         } else {
-            for (int k = threadIdx.x; k < pv; k += kBT)
-                ch |= pr_apply(v0 + k, s_acc[k], variant, damping, tol, deg, rank, active, &res);
+            // apply, then write the partition's messages for the next iteration: the
+            // new c = rank / degree is staged in LDS over the consumed accumulators
+            float cl[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int k = threadIdx.x + j * kBG;
+                cl[j] = 0.f;
+                if (k < pv) {
+                    ch |= pr_apply(v0 + k, s_acc[k], variant, damping, tol, deg, rank, active, &res);
+                    PRHooks::get_partial_result(&cl[j], rank[v0 + k], deg[v0 + k]);
+                }
+            }
+            __syncthreads();
+            float *s_c = (float *)s_acc;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int k = threadIdx.x + j * kBG;
+                if (k < pv) s_c[k] = cl[j];
+            }
+            __syncthreads();
+            scatter_part<kBG>(apart[un.x], apart[un.x + 1], s_c, a_src, amap, out);
         }
         __syncthreads();
     }
-    res = block_sum<kBT / MG_WAVE>(res, s_tmp);
+    res = block_sum<kBG / MG_WAVE>(res, s_tmp);
     const int any = __syncthreads_or(ch);
     if (threadIdx.x == 0) {
         if (res != 0.0) atomicAdd(&ctl->resid, res);
@@ -305,10 +556,10 @@ int mg_pb_build(mg_engine *e, const mg_graph *g) {
     const int nv = e->nv;
     // Partition count and size.  With the round-robin numbering every partition holds
     // ceil(nv / np) vertices and about E / np edges, so one workgroup per partition is
-    // balanced and np can be any count: it is chosen as a multiple of the CU count,
-    // trading tile padding (np^2 tiles, 7.5 padding slots on average) against the
-    // rounds of workgroups each CU runs (LDS of the FP64 accumulators bounds how many
-    // are resident).  part_log2 > 0 forces P = 2^part_log2 instead.
+    // balanced and np can be any count.  Larger partitions share more messages (one per
+    // source vertex and destination partition), so np is the smallest multiple of the
+    // CU count whose partitions fit the gather's LDS (FP64 accumulators plus staged
+    // messages).  part_log2 > 0 forces P = 2^part_log2 instead.
     int cus = 256;
     {
         int dev = 0;
@@ -316,26 +567,18 @@ int mg_pb_build(mg_engine *e, const mg_graph *g) {
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
     }
+    const long long pmax = (160 * 1024 - 4 * kStage - 2048) / 8 - 1;
     int np, P;
     if (e->opt.part_log2 > 0) {
-        P = 1 << std::min(e->opt.part_log2, 14);
+        P = (int)std::min<long long>(1ll << std::min(e->opt.part_log2, 20), pmax);
         np = nv > 0 ? (nv + P - 1) / P : 1;
+    } else if (nv < cus) {
+        np = 1;
+        P = std::max(1, nv);
     } else {
-        double best = 1e300;
-        np = 1; P = std::max(1, nv);
-        for (int k = 1; k <= 64; ++k) {
-            const long long cnp = (long long)cus * k;
-            const long long cP = ((long long)nv + cnp - 1) / cnp;
-            if (cP > 16384) continue;
-            if (cP < 64 && k > 1) break;
-            const int res = (int)std::min<long long>(4, (160 * 1024) / (8 * cP + 2048));  // resident per CU
-            if (res < 1) continue;
-            const long long rounds = (k + res - 1) / res;
-            const double per_block = ((double)e->ne + 7.5 * (double)cnp * (double)cnp) / (double)cnp;
-            const double cost = (double)rounds * per_block;
-            if (cost < best) { best = cost; np = (int)cnp; P = (int)std::max<long long>(1, cP); }
-        }
-        if (nv < cus) { np = 1; P = std::max(1, nv); }
+        const long long k = ((long long)nv + (long long)cus * pmax - 1) / ((long long)cus * pmax);
+        np = (int)((long long)cus * std::max(1ll, k));
+        P = (nv + np - 1) / np;
     }
     L.plog = 0;
     L.P = P;
@@ -427,15 +670,20 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     }
     const long long nt = (long long)np * np;
     BItem *d_items = nullptr;
-    int *count = nullptr, *cursor = nullptr;
-    long long *pb = nullptr, *pa = nullptr, *boff = nullptr, *aoff = nullptr;
+    int *count = nullptr, *cursor = nullptr, *npair = nullptr;
+    long long *pb = nullptr, *pa = nullptr, *boff = nullptr, *aoff = nullptr, *poffB = nullptr;
+    uint32_t *keys = nullptr;
     auto cleanup = [&]() {
-        hipFree(d_items); hipFree(count); hipFree(cursor); hipFree(pb); hipFree(pa); hipFree(boff); hipFree(aoff);
+        hipFree(d_items); hipFree(count); hipFree(cursor); hipFree(npair); hipFree(pb); hipFree(pa); hipFree(boff);
+        hipFree(aoff); hipFree(poffB); hipFree(keys);
+        d_items = nullptr; count = cursor = npair = nullptr; pb = pa = boff = aoff = poffB = nullptr; keys = nullptr;
     };
     if (hipMalloc(&d_items, sizeof(BItem) * std::max<size_t>(1, items.size())) != hipSuccess ||
         hipMalloc(&count, sizeof(int) * nt) != hipSuccess || hipMalloc(&cursor, sizeof(int) * nt) != hipSuccess ||
+        hipMalloc(&npair, sizeof(int) * nt) != hipSuccess ||
         hipMalloc(&pb, sizeof(long long) * nt) != hipSuccess || hipMalloc(&pa, sizeof(long long) * nt) != hipSuccess ||
-        hipMalloc(&boff, sizeof(long long) * (nt + 1)) != hipSuccess || hipMalloc(&aoff, sizeof(long long) * (nt + 1)) != hipSuccess) {
+        hipMalloc(&boff, sizeof(long long) * (nt + 1)) != hipSuccess || hipMalloc(&aoff, sizeof(long long) * (nt + 1)) != hipSuccess ||
+        hipMalloc(&poffB, sizeof(long long) * (nt + 1)) != hipSuccess) {
         cleanup();
         return mg_fail(MG_ERR_NOMEM, "partitioned layout: tile tables do not fit");
     }
@@ -444,35 +692,80 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     MG_HIP(hipMemsetAsync(cursor, 0, sizeof(int) * nt, s));
     const int nitems = (int)items.size();
     const size_t lds_count = sizeof(int) * np;
-    const size_t lds_scatter = sizeof(int) * (np + P + 1);
+    const size_t lds_place = sizeof(int) * (np + P + 1);
+    const size_t lds_gather = sizeof(double) * (P + 1) + sizeof(float) * kStage;
+    if (lds_place > 160 * 1024 || lds_gather > 160 * 1024 - 1024) {
+        cleanup();
+        return mg_fail(MG_ERR_LIMIT, "partitioned layout: %d partitions of %d ids exceed LDS", np, P);
+    }
     // dynamic LDS above the 64 KiB default needs an explicit per-kernel limit
     MG_HIP(hipFuncSetAttribute((const void *)k_pb_count, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds_count, 1024)));
-    MG_HIP(hipFuncSetAttribute((const void *)k_pb_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds_scatter, 1024)));
+    MG_HIP(hipFuncSetAttribute((const void *)k_pb_place, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds_place, 1024)));
     MG_HIP(hipFuncSetAttribute((const void *)k_pb_scatter_msg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * P)));
-    MG_HIP(hipFuncSetAttribute((const void *)k_pb_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * P)));
-    if (lds_scatter > 160 * 1024) { cleanup(); return mg_fail(MG_ERR_LIMIT, "partitioned layout: %d partitions exceed LDS", np); }
+    MG_HIP(hipFuncSetAttribute((const void *)k_pb_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gather));
+    // edges per tile -> padded edge slots -> tile offsets of the destination-major stream
     hipLaunchKernelGGL(k_pb_count, dim3(grid_for(nitems, 1, 4096)), dim3(kBT), lds_count, s, icol, irow, d_items, nitems,
                        L.P, np, count);
-    hipLaunchKernelGGL(k_pb_pad, dim3(grid_for(nt, 256, 8192)), dim3(256), 0, s, count, np, pb, pa);
+    hipLaunchKernelGGL(k_pb_pad, dim3(grid_for(nt, 256, 8192)), dim3(256), 0, s, count, nt, pb);
     int rc = mg_scan64(s, pb, nullptr, nt, boff);
-    if (rc == MG_OK) rc = mg_scan64(s, pa, nullptr, nt, aoff);
     if (rc != MG_OK) { cleanup(); return rc; }
     long long slots = 0;
     MG_HIP(hipMemcpy(&slots, boff + nt, sizeof(long long), hipMemcpyDeviceToHost));
     if (slots >= (1ll << 31)) { cleanup(); return mg_fail(MG_ERR_LIMIT, "partitioned layout: %lld slots exceed int32", slots); }
     L.slots = slots;
     const long long sl = slots > 0 ? slots : kG;
-    if (hipMalloc(&L.a_src, sizeof(uint16_t) * sl) != hipSuccess || hipMalloc(&L.b_dst, sizeof(uint16_t) * sl) != hipSuccess ||
-        hipMalloc(&L.amap, sizeof(int) * (sl / kG + 1)) != hipSuccess || hipMalloc(&L.msg, sizeof(float) * sl) != hipSuccess) {
-        cleanup();
-        return mg_fail(MG_ERR_NOMEM, "partitioned layout: %lld slots do not fit", slots);
+    if (hipMalloc(&keys, sizeof(uint32_t) * sl) != hipSuccess) { cleanup(); return mg_fail(MG_ERR_NOMEM, "partitioned layout: keys"); }
+    hipLaunchKernelGGL(k_pb_place, dim3(grid_for(nitems, 1, 4096)), dim3(kBT), lds_place, s, icol, irow, d_items, nitems,
+                       L.P, np, cursor, boff, keys);
+    hipLaunchKernelGGL(k_pb_tile_sort, dim3(grid_for(nt, 1, 8192)), dim3(kBT), 0, s, nt, count, boff, keys, npair);
+    MG_HIP(hipGetLastError());
+    // tiles above the LDS sort size (only graphs with very few partitions): sort on the host
+    {
+        std::vector<int> hn(nt), hc(nt);
+        std::vector<long long> hb(nt + 1);
+        MG_HIP(hipMemcpyAsync(hn.data(), npair, sizeof(int) * nt, hipMemcpyDeviceToHost, s));
+        MG_HIP(hipMemcpyAsync(hc.data(), count, sizeof(int) * nt, hipMemcpyDeviceToHost, s));
+        MG_HIP(hipMemcpyAsync(hb.data(), boff, sizeof(long long) * (nt + 1), hipMemcpyDeviceToHost, s));
+        MG_HIP(hipStreamSynchronize(s));
+        bool fixed = false;
+        std::vector<uint32_t> tk;
+        for (long long t = 0; t < nt; ++t) {
+            if (hn[t] >= 0) continue;
+            tk.resize(hc[t]);
+            MG_HIP(hipMemcpy(tk.data(), keys + hb[t], sizeof(uint32_t) * hc[t], hipMemcpyDeviceToHost));
+            std::sort(tk.begin(), tk.end());
+            int c = 0;
+            for (int i = 0; i < hc[t]; ++i) c += (i == 0 || (tk[i] >> 16) != (tk[i - 1] >> 16)) ? 1 : 0;
+            MG_HIP(hipMemcpy(keys + hb[t], tk.data(), sizeof(uint32_t) * hc[t], hipMemcpyHostToDevice));
+            hn[t] = c;
+            fixed = true;
+        }
+        if (fixed) MG_HIP(hipMemcpy(npair, hn.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
     }
-    hipLaunchKernelGGL(k_fill_u16, dim3(grid_for(sl, 256, 8192)), dim3(256), 0, s, L.a_src, sl);
-    hipLaunchKernelGGL(k_fill_u16, dim3(grid_for(sl, 256, 8192)), dim3(256), 0, s, L.b_dst, sl);
-    MG_HIP(hipMemsetAsync(L.amap, 0, sizeof(int) * (sl / kG + 1), s));
-    MG_HIP(hipMemsetAsync(L.msg, 0, sizeof(float) * sl, s));
-    hipLaunchKernelGGL(k_pb_scatter, dim3(grid_for(nitems, 1, 4096)), dim3(kBT), lds_scatter, s, icol, irow, d_items,
-                       nitems, L.P, np, cursor, boff, aoff, L.a_src, L.b_dst, L.amap);
+    // messages per tile -> padded message slots in both orders -> offsets
+    hipLaunchKernelGGL(k_pb_pad_pairs, dim3(grid_for(nt, 256, 8192)), dim3(256), 0, s, npair, np, pb, pa);
+    rc = mg_scan64(s, pb, nullptr, nt, poffB);
+    if (rc == MG_OK) rc = mg_scan64(s, pa, nullptr, nt, aoff);
+    if (rc != MG_OK) { cleanup(); return rc; }
+    long long pairs = 0;
+    MG_HIP(hipMemcpy(&pairs, poffB + nt, sizeof(long long), hipMemcpyDeviceToHost));
+    L.pairs = pairs;
+    const long long pl = pairs > 0 ? pairs : kG;
+    const long long ngroups = sl / kG;
+    if (hipMalloc(&L.a_src, sizeof(uint16_t) * pl) != hipSuccess || hipMalloc(&L.amap, sizeof(int) * (pl / kG + 1)) != hipSuccess ||
+        hipMalloc(&L.msg, sizeof(float) * 2 * pl) != hipSuccess || hipMalloc(&L.b_dst, sizeof(uint16_t) * sl) != hipSuccess ||
+        hipMalloc(&L.gbase, sizeof(int) * (ngroups + 1)) != hipSuccess) {
+        cleanup();
+        return mg_fail(MG_ERR_NOMEM, "partitioned layout: %lld edge slots / %lld messages do not fit", slots, pairs);
+    }
+    hipLaunchKernelGGL(k_fill_u16, dim3(grid_for(pl, 256, 8192)), dim3(256), 0, s, L.a_src, pl, (uint16_t)0xFFFF);
+    hipLaunchKernelGGL(k_fill_u16, dim3(grid_for(sl, 256, 8192)), dim3(256), 0, s, L.b_dst, sl, (uint16_t)0x7FFF);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(pl / kG + 1, 256, 8192)), dim3(256), 0, s, (uint32_t *)L.amap, pl / kG + 1, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * pl, 256, 8192)), dim3(256), 0, s, (uint32_t *)L.msg, 2 * pl, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ngroups + 1, 256, 8192)), dim3(256), 0, s, (uint32_t *)L.gbase, ngroups + 1,
+                       (uint32_t)pairs);
+    hipLaunchKernelGGL(k_pb_emit, dim3(grid_for(nt, 1, 8192)), dim3(kBT), 0, s, nt, np, count, boff, poffB, aoff, keys,
+                       L.b_dst, L.gbase, L.a_src, L.amap);
     MG_HIP(hipGetLastError());
     // partition starts of both streams -> work units of about equal slot counts
     std::vector<long long> hb(nt + 1), ha(nt + 1);
@@ -482,17 +775,20 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     cleanup();
     // unit bounds stay multiples of the 16-slot group (vector loads of 8 slots)
     const long long unit = ((std::max<long long>(32768, 2 * (slots / std::max(1, np)) + 1) + kG - 1) / kG) * kG;
+    // gather units cut the destination-major edge stream; a partition cut into several
+    // units is applied, and its messages written, by separate passes (units A)
     std::vector<int4> ua, ub;
-    std::vector<int> split;
+    std::vector<int> split, apart(np + 1);
+    for (int p = 0; p <= np; ++p) apart[p] = (int)ha[(long long)p * np];
     for (int p = 0; p < np; ++p) {
-        const long long a0 = ha[(long long)p * np], a1 = ha[(long long)(p + 1) * np];
-        for (long long x = a0; x < a1; x += unit) ua.push_back(make_int4(p, (int)x, (int)std::min(a1, x + unit), 0));
         const long long b0 = hb[(long long)p * np], b1 = hb[(long long)(p + 1) * np];
         const int nb = (int)((b1 - b0 + unit - 1) / unit);
         if (nb <= 1) ub.push_back(make_int4(p, (int)b0, (int)b1, 0));
         else {
             split.push_back(p);
             for (long long x = b0; x < b1; x += unit) ub.push_back(make_int4(p, (int)x, (int)std::min(b1, x + unit), 1));
+            const long long a0 = apart[p], a1 = apart[p + 1];
+            for (long long x = a0; x < a1; x += unit) ua.push_back(make_int4(p, (int)x, (int)std::min(a1, x + unit), 0));
         }
     }
     L.nA = (int)ua.size();
@@ -501,6 +797,8 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     MG_HIP(hipMalloc(&L.unitsA, sizeof(int4) * std::max<size_t>(1, ua.size())));
     MG_HIP(hipMalloc(&L.unitsB, sizeof(int4) * std::max<size_t>(1, ub.size())));
     MG_HIP(hipMalloc(&L.split_parts, sizeof(int) * std::max<size_t>(1, split.size())));
+    MG_HIP(hipMalloc(&L.apart, sizeof(int) * (np + 1)));
+    MG_HIP(hipMemcpy(L.apart, apart.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice));
     if (!ua.empty()) MG_HIP(hipMemcpy(L.unitsA, ua.data(), sizeof(int4) * ua.size(), hipMemcpyHostToDevice));
     if (!ub.empty()) MG_HIP(hipMemcpy(L.unitsB, ub.data(), sizeof(int4) * ub.size(), hipMemcpyHostToDevice));
     if (!split.empty()) MG_HIP(hipMemcpy(L.split_parts, split.data(), sizeof(int) * split.size(), hipMemcpyHostToDevice));
@@ -509,7 +807,8 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     return MG_OK;
 }
 
-// one dense iteration over the partitioned layout (without the control kernels)
+// one dense iteration over the partitioned layout (without the control kernel): the
+// gather of every unsplit partition also writes its messages for the next iteration
 int mg_pb_iteration(mg_engine *e) {
     mg_pb_layout &L = e->pb;
     hipStream_t s = e->stream;
@@ -519,19 +818,27 @@ int mg_pb_iteration(mg_engine *e) {
     int slot = -1;
     // algorithmic bytes of one iteration (pull formulation): 8E + 16V
     const double alg = 8.0 * (double)e->ne + 16.0 * e->nv;
-    if (e->prof) mg_prof_begin(e, "k_pb_scatter_msg", 0.5 * alg, &slot);
-    hipLaunchKernelGGL(k_pb_scatter_msg, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(float) * P, s, e->ctl, L.unitsA, L.nA,
-                       L.P, e->nv, L.np, e->rank, e->deg, L.a_src, L.amap, L.msg);
+    if (e->prof) mg_prof_begin(e, "k_pb_gather", alg, &slot);
+    hipLaunchKernelGGL(k_pb_gather, dim3(grid_for(L.nB, 1, G)), dim3(kBG), sizeof(double) * (P + 1) + sizeof(float) * kStage, s,
+                       e->ctl, L.unitsB, L.nB, L.P, e->nv, L.np, variant, e->opt.pr_damping, e->opt.pr_tolerance, L.b_dst,
+                       L.gbase, L.msg, L.pairs, L.apart, L.a_src, L.amap, e->deg, e->rank, e->vfc, e->psum);
     if (e->prof) mg_prof_end(e, slot);
-    if (e->prof) mg_prof_begin(e, "k_pb_gather", 0.5 * alg, &slot);
-    hipLaunchKernelGGL(k_pb_gather, dim3(grid_for(L.nB, 1, G)), dim3(kBT), sizeof(double) * P, s, e->ctl, L.unitsB, L.nB,
-                       L.P, e->nv, L.np, variant, e->opt.pr_damping, e->opt.pr_tolerance, L.b_dst, L.msg, e->deg, e->rank,
-                       e->vfc, e->psum);
-    if (e->prof) mg_prof_end(e, slot);
-    if (L.nApply > 0)
+    if (L.nApply > 0) {
         hipLaunchKernelGGL(k_pb_apply_split, dim3(grid_for((long long)L.nApply * P, kBT, G)), dim3(kBT), 0, s, e->ctl,
                            L.split_parts, L.nApply, L.P, e->nv, L.np, variant, e->opt.pr_damping, e->opt.pr_tolerance, e->deg,
                            e->rank, e->vfc, e->psum);
+        hipLaunchKernelGGL(k_pb_scatter_msg, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(float) * P, s, e->ctl, L.unitsA,
+                           L.nA, L.P, e->nv, L.np, e->rank, e->deg, L.a_src, L.amap, L.msg, L.pairs);
+    }
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+// before the first iteration: its messages (rank 0) are all zero
+int mg_pb_reset(mg_engine *e) {
+    mg_pb_layout &L = e->pb;
+    const long long n = 2 * std::max<long long>(L.pairs, kG);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, e->stream, (uint32_t *)L.msg, n, 0u);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }

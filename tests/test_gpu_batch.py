@@ -118,3 +118,69 @@ def test_batch_sssp_needs_symmetric(mg):
     g = mg.Graph.rmat(8, 4, seed=2, flags=0)
     with pytest.raises(mg.MGError):
         mg.Batch(g, mg.BATCH_SSSP)
+
+
+@pytest.mark.parametrize("kind", ["bfs", "sssp", "bc"])
+def test_batch_union_slots(mg, orc, kind):
+    """uneven batch: disjoint member graphs share the slots of one run (4 sources each)"""
+    parts = [mg.Graph.rmat(10, 8, seed=31), mg.Graph.grid(23, 17), mg.Graph.rmat(8, 4, seed=32),
+             mg.Graph.from_coo(50, np.arange(49), np.arange(1, 50), np.full(49, 2))]
+    u = mg.Graph.union(parts)
+    rng = np.random.default_rng(8)
+    srcs, slots, offs = [], [], []
+    off = 0
+    for p in parts:
+        offs.append(off)
+        loc = rng.integers(0, p.nv, 4)
+        srcs += (loc + off).tolist()
+        slots += [0, 1, 2, 3]
+        off += p.nv
+    k = {"bfs": mg.BATCH_BFS, "sssp": mg.BATCH_SSSP, "bc": mg.BATCH_BC}[kind]
+    b = mg.Batch(u, k)
+    b.run(srcs, slots)
+    lv = b.levels()
+    assert lv.shape == (4, u.nv)
+    bc = b.bc().astype(np.float64) if kind == "bc" else None
+    for gi, p in enumerate(parts):
+        row, col, w = p.csr()
+        o = offs[gi]
+        ref_bc = np.zeros(p.nv)
+        for j in range(4):
+            s = srcs[4 * gi + j] - o
+            got = lv[j, o:o + p.nv]
+            if kind == "sssp":
+                np.testing.assert_array_equal(got, orc.sssp_bf(row, col, w, s)[0])
+            else:
+                np.testing.assert_array_equal(got, orc.bfs(row, col, s))
+            if kind == "bc":
+                _, _, dl = orc.bc_fp64(row, col, s)
+                dl = dl.copy()
+                dl[s] = 0.0
+                ref_bc += dl
+        if kind == "bc":
+            err = np.abs(bc[o:o + p.nv] - ref_bc) / np.maximum(1.0, np.abs(ref_bc))
+            assert err.max() <= 1e-5
+    b.close()
+
+
+@pytest.mark.parametrize("kind", ["bfs", "sssp"])
+@pytest.mark.parametrize("graph_levels", [1, 3, 32])
+def test_batch_graph_replay(mg, orc, kind, graph_levels):
+    """levels replayed K at a time from a captured graph (past the end as no-ops), run
+    twice on one batch object: every row matches the oracle, the level count does not
+    depend on K"""
+    g = mg.Graph.grid(40, 25)
+    row, col, w = g.csr()
+    k = mg.BATCH_BFS if kind == "bfs" else mg.BATCH_SSSP
+    b = mg.Batch(g, k, graph_levels=graph_levels)
+    ref_levels = None
+    for seed in (1, 2):
+        src = _sources(g, 64, 40 + seed)
+        b.run(src)
+        lv = b.levels()
+        for i, s in enumerate(src):
+            ref = orc.bfs(row, col, int(s)) if kind == "bfs" else orc.sssp_bf(row, col, w, int(s))[0]
+            np.testing.assert_array_equal(lv[i], ref, err_msg=f"run {seed} source {i}={s}")
+        if kind == "bfs":
+            assert b.sweeps() == max(int(orc.bfs(row, col, int(s)).max()) for s in src) + 1
+    b.close()
