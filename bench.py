@@ -70,6 +70,17 @@ def gteps(edges, ms):
     return edges / (ms * 1e-3) / 1e9
 
 
+def measured_traffic(kernel, scale):
+    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None"""
+    path = os.path.join(ROOT, "profiles", "traffic_rmat22_sssp_cc.json")
+    if scale != 22 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f).get("kernels", {}).get(kernel)
+    return float(k["bytes"]) if k else None
+
+
 def rmat(mg, scale):
     t0 = time.time()
     g = mg.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=1, weight_seed=7)
@@ -322,7 +333,9 @@ def main():
                                f"RMAT-{args.scale} (a=.57 b=.19 c=.19, edge factor 16, seed 1)",
                    "scale": args.scale, "edge_factor": 16, "nv": nv, "ne": ne, "ne_u": ne_u, "source": int(src)},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_name, args.scale),
+                     "traffic_source": "profiles/traffic_rmat22_sssp_cc.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+                     "avg_launch_ms": avg_ms,
                      "bytes_per_launch": avg_bytes, "launches": dom["launches"]},
         "cpu_baseline": cpu,
         "per_algo": per_algo,

@@ -455,29 +455,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
         const bool fits = nown <= kEPB + 1;
         for (int k = threadIdx.x; k < kEPB + 1; k += kBlock) s_best[k] = MG_INF;
         chunk_owners(row, u0, u1, e0, e1, s_own);
-        // Edges of one row are consecutive lanes: their candidates are min-reduced
-        // across the wave first (segmented by owner), so each run issues one LDS
-        // atomic instead of one per lane on the same address (a hub's row would
-        // otherwise serialise 64 lanes).  Loop bounds are uniform per block.
-        for (int eb = e0; eb < e1; eb += kBlock) {
-            const int e = eb + threadIdx.x;
-            int own = -1, cand = MG_INF;
-            if (e < e1) {
-                own = s_own[e - e0];
-                const int u = ld_stream(&col[e]);
-                if ((bm[u >> 6] >> (u & 63)) & 1ull) cand = dist[u] + ld_stream(&wgt[e]);
-            }
-#pragma unroll
-            for (int d = 1; d < MG_WAVE; d <<= 1) {
-                const int c2 = __shfl_up(cand, d);
-                const int o2 = __shfl_up(own, d);
-                if (lane() >= d && o2 == own) cand = min(cand, c2);
-            }
-            const int on = __shfl_down(own, 1);
-            const bool last = lane() == MG_WAVE - 1 || on != own;
-            if (last && own >= 0 && cand < MG_INF) {
-                if (fits) atomicMin(&s_best[own - u0], cand);
-                else if (atomicMin(&dist[own], cand) > cand) vfc[own] = iter;
+        for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
+            const int u = ld_stream(&col[e]);
+            if ((bm[u >> 6] >> (u & 63)) & 1ull) {
+                const int cand = dist[u] + ld_stream(&wgt[e]);
+                const int o = s_own[e - e0];
+                if (fits) atomicMin(&s_best[o - u0], cand);
+                else if (atomicMin(&dist[o], cand) > cand) vfc[o] = iter;
             }
         }
         __syncthreads();
