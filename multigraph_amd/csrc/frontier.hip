@@ -432,8 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_bmq_write(Ctl *ctl, int words, const
 
 // -------------------------------------------------------------------- SSSP pull
 // Dense SSSP level without global atomics: every vertex takes the minimum of
-// dist[u] + w over neighbours u that changed in the previous iteration (frontier
-// bitmap), edge-balanced over the whole CSR.  Minima are reduced per owner in LDS;
+// dist[u] + w over its reached neighbours u, edge-balanced over the whole CSR.  Minima are reduced per owner in LDS;
 // an owner whose row lies inside the chunk is written with a plain store, the (at
 // most two) owners that straddle a chunk boundary with atomicMin.  This is the
 // reference's dense all-edge relaxation (process.cu:653-934) without its message
@@ -441,8 +440,8 @@ __global__ __launch_bounds__(kBlock) void k_bmq_write(Ctl *ctl, int words, const
 // push over a large frontier pays one such atomic per improving edge.
 __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long long ne, const int *__restrict__ row,
                                                       const int *__restrict__ col, const int *__restrict__ wgt,
-                                                      const int *__restrict__ cstart, const uint64_t *__restrict__ bm,
-                                                      int *__restrict__ dist, int *__restrict__ vfc) {
+                                                      const int *__restrict__ cstart, int *__restrict__ dist,
+                                                      int *__restrict__ vfc) {
     if (ctl->mode != MODE_BU) return;
     const int iter = ctl->iter;
     __shared__ int s_own[kEPB];
@@ -455,10 +454,14 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
         const bool fits = nown <= kEPB + 1;
         for (int k = threadIdx.x; k < kEPB + 1; k += kBlock) s_best[k] = MG_INF;
         chunk_owners(row, u0, u1, e0, e1, s_own);
+        // Every reached source relaxes its edges, read live (an update made earlier in
+        // this sweep propagates at once): a frontier-bitmap probe per edge costs more
+        // random traffic than the redundant relaxations it would skip.
         for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
             const int u = ld_stream(&col[e]);
-            if ((bm[u >> 6] >> (u & 63)) & 1ull) {
-                const int cand = dist[u] + ld_stream(&wgt[e]);
+            const int du = dist[u];
+            if (du != MG_INF) {
+                const int cand = du + ld_stream(&wgt[e]);
                 const int o = s_own[e - e0];
                 if (fits) atomicMin(&s_best[o - u0], cand);
                 else if (atomicMin(&dist[o], cand) > cand) vfc[o] = iter;
@@ -777,11 +780,11 @@ static int launch_level(mg_engine *e, bool known) {
     const int words = (nv + 63) / 64;
     const int gsc = grid_for(words, kBlock / MG_WAVE, G);  // pull / scan grid = number of partials
     if (bu && e->algo == MG_SSSP) {
-        // col + weight streamed per edge, bitmap per edge, dist + stamp per vertex
+        // col + weight streamed per edge, source distance per edge, dist + stamp per vertex
         const double bytes = 8.0 * (double)e->ne + 4.0 * (nv + 1) + 8.0 * nv;
         if (e->prof) mg_prof_begin(e, "k_pull_sssp", bytes, &slot);
         hipLaunchKernelGGL(k_pull_sssp, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->row,
-                           e->col, e->w, e->pcstart, e->bm[0], e->level, e->vfc);
+                           e->col, e->w, e->pcstart, e->level, e->vfc);
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_sssp_scan, dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, e->row, e->bm[0], e->part_n,
                            e->part_mf);
