@@ -70,6 +70,10 @@ struct Params {
 };
 
 struct HugeChunk { int v, j; };
+// pull chunk of a high-degree vertex: edges [p0, p1) of v, k = v's slot in the huge list
+struct PullChunk { int v, p0, p1, k; };
+constexpr int kPullHuge = 2048;   // pull levels split vertices above this degree
+constexpr int kPullChunk = 2048;  // edges per pull chunk
 
 struct KStat {
     std::string name;
@@ -151,6 +155,10 @@ struct mg_engine {
     int *q = nullptr;          // frontier queue / level log
     long long qcap = 0;
     uint64_t *bm[2] = {nullptr, nullptr};
+    mg::PullChunk *pull_ch = nullptr;   // pull chunks of high-degree vertices (BFS, BC)
+    int *pull_hv = nullptr, *pull_found = nullptr;
+    double *pull_sum = nullptr;
+    int pull_nch = 0, pull_nh = 0;
     uint64_t *vis = nullptr;
     mg::HugeChunk *chunks = nullptr;
     long long chunk_cap = 0;

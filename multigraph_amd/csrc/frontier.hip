@@ -336,6 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
         if (vw != ~0ull && !((vw >> lane()) & 1ull)) {
             b = row[v];
             e = row[v + 1];
+            if (e - b > kPullHuge) e = b;  // left to k_pull_huge
             for (int p = b; p < e; ++p) {
                 const int u = ld_stream(&col[p]);
                 if (bm_test(fr, u)) {
@@ -358,6 +359,69 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
     const long long tn = block_sum<kBlock / MG_WAVE>((long long)cnt, s_tmp);
     const long long tm = block_sum<kBlock / MG_WAVE>(mf, s_tmp);
     if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_mf[blockIdx.x] = tm; }
+}
+
+// Unvisited vertices above kPullHuge edges, cut into static kPullChunk-edge chunks
+// (PullChunk {v, p0, p1, k}, k = the vertex's slot): a wave per chunk scans it and
+// adds what it finds into the vertex's slot (found flag, FP64 path-count sum: exact
+// in any order); k_pull_huge_fin then updates level, sigma, bitmaps and counts.
+// Without this split, one lane walks a 160 K-edge hub alone (BC scans every parent).
+template <bool kBC>
+__global__ __launch_bounds__(kBlock) void k_pull_huge(Ctl *ctl, int nch, const PullChunk *__restrict__ ch,
+                                                      const int *__restrict__ col, const double *__restrict__ sigma,
+                                                      const uint64_t *__restrict__ vis, const uint64_t *__restrict__ bm0,
+                                                      const uint64_t *__restrict__ bm1, int *__restrict__ hfound,
+                                                      double *__restrict__ hsum) {
+    if (ctl->mode != MODE_BU) return;
+    const uint64_t *fr = ctl->bm_cur ? bm1 : bm0;
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * kBlock) >> 6;
+    for (int c = wave; c < nch; c += nwaves) {
+        const PullChunk k = ch[c];
+        if (bm_test(vis, k.v)) continue;
+        bool found = false;
+        double sum = 0.0;
+        for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+            const int p = p0 + lane();
+            bool f = false;
+            if (p < k.p1) {
+                const int u = ld_stream(&col[p]);
+                f = bm_test(fr, u);
+                if (kBC && f) sum += sigma[u];
+            }
+            const bool any = ballot(f) != 0;
+            found = found || any;
+            if (!kBC && any) break;
+        }
+        if (kBC) sum = wave_sum(sum);
+        if (lane() == 0 && found) {
+            atomicOr(&hfound[k.k], 1);
+            if (kBC) atomicAdd(&hsum[k.k], sum);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pull_huge_fin(Ctl *ctl, int nh, const int *__restrict__ hv,
+                                                          const int *__restrict__ row, int *__restrict__ level,
+                                                          double *__restrict__ sigma, uint64_t *__restrict__ vis,
+                                                          uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
+                                                          int *__restrict__ hfound, double *__restrict__ hsum,
+                                                          int *__restrict__ part_n, long long *__restrict__ part_mf,
+                                                          int with_sigma) {
+    if (ctl->mode != MODE_BU) return;
+    uint64_t *nx = ctl->bm_cur ? bm0 : bm1;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < nh; k += gridDim.x * kBlock) {
+        if (!hfound[k]) continue;
+        const int v = hv[k];
+        level[v] = ctl->iter;
+        if (with_sigma) sigma[v] = hsum[k];
+        atomicOr((unsigned long long *)&nx[v >> 6], 1ull << (v & 63));
+        atomicOr((unsigned long long *)&vis[v >> 6], 1ull << (v & 63));
+        atomicAdd(&part_n[0], 1);
+        atomicAdd((unsigned long long *)&part_mf[0], (unsigned long long)(row[v + 1] - row[v]));
+        hfound[k] = 0;
+        hsum[k] = 0.0;
+    }
 }
 
 // sum the per-workgroup partials of a pull level into the control block
@@ -660,6 +724,27 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         MG_HIP(hipMalloc(&e->bm[0], sizeof(uint64_t) * words));
         MG_HIP(hipMalloc(&e->bm[1], sizeof(uint64_t) * words));
         MG_HIP(hipMalloc(&e->vis, sizeof(uint64_t) * words));
+        // pull chunks of the vertices above kPullHuge edges (static)
+        std::vector<PullChunk> pc;
+        std::vector<int> hv;
+        for (int v = 0; v < nv; ++v) {
+            const int b = g->row[v], d = g->row[v + 1] - b;
+            if (d <= kPullHuge) continue;
+            for (int p = b; p < b + d; p += kPullChunk) pc.push_back(PullChunk{v, p, std::min(b + d, p + kPullChunk), (int)hv.size()});
+            hv.push_back(v);
+        }
+        e->pull_nch = (int)pc.size();
+        e->pull_nh = (int)hv.size();
+        if (!pc.empty()) {
+            MG_HIP(hipMalloc(&e->pull_ch, sizeof(PullChunk) * pc.size()));
+            MG_HIP(hipMalloc(&e->pull_hv, sizeof(int) * hv.size()));
+            MG_HIP(hipMalloc(&e->pull_found, sizeof(int) * hv.size()));
+            MG_HIP(hipMalloc(&e->pull_sum, sizeof(double) * hv.size()));
+            MG_HIP(hipMemcpy(e->pull_ch, pc.data(), sizeof(PullChunk) * pc.size(), hipMemcpyHostToDevice));
+            MG_HIP(hipMemcpy(e->pull_hv, hv.data(), sizeof(int) * hv.size(), hipMemcpyHostToDevice));
+            MG_HIP(hipMemset(e->pull_found, 0, sizeof(int) * hv.size()));
+            MG_HIP(hipMemset(e->pull_sum, 0, sizeof(double) * hv.size()));
+        }
     }
     if (e->algo == MG_BC) {
         MG_HIP(hipMalloc(&e->sigma, sizeof(double) * padv));
@@ -798,6 +883,18 @@ static int launch_level(mg_engine *e, bool known) {
             hipLaunchKernelGGL((k_pull<true>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         else
             hipLaunchKernelGGL((k_pull<false>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+        if (e->pull_nch > 0) {
+            const int gh = grid_for(e->pull_nch, kBlock / MG_WAVE, G);
+            if (e->algo == MG_BC)
+                hipLaunchKernelGGL((k_pull_huge<true>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->col, e->sigma,
+                                   e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
+            else
+                hipLaunchKernelGGL((k_pull_huge<false>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->col, e->sigma,
+                                   e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
+            hipLaunchKernelGGL(k_pull_huge_fin, dim3(grid_for(e->pull_nh, kBlock, 64)), dim3(kBlock), 0, s, e->ctl, e->pull_nh,
+                               e->pull_hv, e->row, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum,
+                               e->part_n, e->part_mf, e->algo == MG_BC ? 1 : 0);
+        }
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
     }
