@@ -219,7 +219,7 @@ def main():
     ap.add_argument("--pr-scale", type=int, default=24, help="graph scale of the PageRank figures (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip per-algorithm, batched and PageRank figures")
-    ap.add_argument("--no-uneven", action="store_true", help="skip the 32-graph uneven batch")
+    ap.add_argument("--uneven", action="store_true", help="add the 32-graph uneven batch (RMAT-14..20 + grids)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU path",
@@ -313,7 +313,7 @@ def main():
     del g, row, col, w, srcs, d, lab
     if not args.no_extra and args.pr_scale > 0:
         pr_big = pagerank_big(mg, np, args.pr_scale)
-    if not args.no_extra and not args.no_uneven:
+    if not args.no_extra and args.uneven:
         uneven_batch = uneven(mg, np)
 
     line = {
