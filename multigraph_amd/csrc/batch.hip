@@ -21,6 +21,7 @@
 //     distance at v dropped in the last sweep; lane s relaxes dist[v][s] from the
 //     neighbours whose changed bit s is set.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 #include "engine.hpp"
 #include "mg_device.hpp"
@@ -136,8 +137,14 @@ constexpr int kCH = 1024;  // default edges per chunk (mg_options.batch_chunk)
 struct BCtl {
     int it;       // level (BFS) or sweep (SSSP) being processed
     int done;
-    int push;     // BFS: this level pushes (1) or pulls (0)
-    int pad;
+    int push;     // dense levels: push over all chunks (1) or pull (0)
+    int sparse;   // this level walks the frontier's queue instead of every chunk
+    int compact;  // build the current frontier's queue first (dense -> sparse)
+    int fill;     // zero the whole next frontier first (it is not known clean)
+    int prev_sparse;  // the previous level was sparse (its frontier buffer was cleared)
+    int need_full;    // the sparse-only graph met a level it cannot run: replay the full graph
+    int qn[2];    // queue lengths of F[0], F[1]
+    int qvalid[2];    // the queue of F[b] lists its nonzero words
     long long n, mf;  // size and degree sum of the last produced frontier
     unsigned long long all;  // the slots in use
 };
@@ -151,10 +158,10 @@ __device__ __forceinline__ void cur_next(const BCtl *ctl, uint64_t *A, uint64_t 
     *cur = ((it - 1) & 1) ? B : A;
     *next = (it & 1) ? B : A;
 }
-// want: -1 any level, 1 push levels only, 0 pull levels only
+// dense-level kernels; want: -1 any dense level, 1 push levels only, 0 pull levels only
 __device__ __forceinline__ bool level_runs(const BCtl *ctl, int want) {
     if (!ctl) return true;
-    if (ctl->done) return false;
+    if (ctl->done || ctl->sparse || ctl->need_full) return false;
     return want < 0 || ctl->push == want;
 }
 
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(kBB) void k_mb_count(const BCtl *ctl, int nv, const
 
 // zero the next frontier (a fill kernel: graphs record no memset nodes)
 __global__ void k_mb_fill_next(const BCtl *ctl, uint64_t *FA, uint64_t *FB, long long n) {
-    if (!level_runs(ctl, -1)) return;
+    if (ctl->done || !ctl->fill || ctl->need_full) return;
     uint64_t *cur, *next;
     cur_next(ctl, FA, FB, &cur, &next);
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) next[i] = 0ull;
@@ -331,32 +338,231 @@ __global__ void k_mb_ctl_init(BCtl *ctl, unsigned long long all) {
 }
 
 // between levels: total the partials of the last produced frontier; stop when it is
-// empty, else open the next level and choose its direction (BFS: push while the
-// frontier's degree sum is below E / alpha)
-__global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe, int nparts,
-                         long long ne, float alpha, int forced, int symmetric) {
-    if (ctl->done) return;
-    __shared__ long long s_tmp[256 / MG_WAVE];
-    long long a = 0, b = 0;
-    for (int i = threadIdx.x; i < nparts; i += 256) { a += pn[i]; b += pe[i]; }
-    a = block_sum<256 / MG_WAVE>(a, s_tmp);
-    b = block_sum<256 / MG_WAVE>(b, s_tmp);
-    if (threadIdx.x != 0) return;
-    BCtl c = *ctl;
+// empty, else open the next level and choose its direction (push while the frontier's
+// degree sum is below E / alpha) and whether it walks a queue (sparse: push levels of
+// fewer than nv / beta vertices, so per-level work follows the frontier, not the
+// vertex count; high-diameter members of an uneven batch run thousands of such levels)
+struct LevelRule { long long ne; int nv; float alpha, beta; int forced, symmetric; };
+
+__device__ __forceinline__ void decide(BCtl &c, long long a, long long b, const LevelRule &r) {
     c.n = a;
     c.mf = b;
-    if (a == 0) {
-        c.done = 1;
-    } else {
-        c.it += 1;
-        int push = (double)b < (double)ne / alpha;
-        if (forced == MG_MODE_SPARSE) push = 1;
-        if (forced == MG_MODE_DENSE) push = 0;
-        if (!symmetric) push = 1;
-        c.push = push;
-    }
+    c.need_full = 0;
+    if (a == 0) { c.done = 1; return; }
+    c.it += 1;
+    int push = (double)b < (double)r.ne / r.alpha;
+    if (r.forced == MG_MODE_SPARSE) push = 1;
+    if (r.forced == MG_MODE_DENSE) push = 0;
+    if (!r.symmetric) push = 1;
+    c.push = push;
+    const int cur = (c.it - 1) & 1, next = c.it & 1;
+    const int sparse = r.forced != MG_MODE_DENSE && push && (double)a * r.beta < (double)r.nv;
+    c.compact = sparse && !c.qvalid[cur];
+    c.fill = !sparse || !c.prev_sparse;
+    c.sparse = sparse;
+    c.qn[next] = 0;
+    c.qvalid[next] = sparse;
+    c.prev_sparse = sparse;
+}
+
+__device__ __forceinline__ void sum_partials(const int *pn, const long long *pe, int nparts, long long *a, long long *b) {
+    __shared__ long long s_tmp[256 / MG_WAVE];
+    long long x = 0, y = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) { x += pn[i]; y += pe[i]; }
+    *a = block_sum<256 / MG_WAVE>(x, s_tmp);
+    *b = block_sum<256 / MG_WAVE>(y, s_tmp);
+}
+
+__global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe, int nparts, LevelRule r) {
+    if (ctl->done) return;
+    long long a, b;
+    sum_partials(pn, pe, nparts, &a, &b);
+    if (threadIdx.x != 0) return;
+    BCtl c = *ctl;
+    decide(c, a, b, r);
     *ctl = c;
 }
+
+// the control kernel of the sparse-only graph: it runs the next level only when that
+// level is sparse and needs neither a compaction nor a full clear; otherwise it flags
+// need_full (state untouched) and the host replays the full graph, whose control
+// kernel decides again from the same partials
+__global__ void k_mb_ctl_sparse(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe, int nparts,
+                                LevelRule r) {
+    if (ctl->done || ctl->need_full) return;
+    long long a, b;
+    sum_partials(pn, pe, nparts, &a, &b);
+    if (threadIdx.x != 0) return;
+    BCtl c = *ctl;
+    decide(c, a, b, r);
+    if (!c.done && (!c.sparse || c.compact || c.fill)) { ctl->need_full = 1; return; }
+    *ctl = c;
+}
+
+// ---- sparse levels (queue of the frontier's vertices) -------------------------------
+// dense -> sparse: the current frontier's queue by count / scan / ordered write over
+// its nonzero words (the k_nz_* pattern), gated on ctl->compact
+__global__ __launch_bounds__(kBB) void k_mbs_qcount(const BCtl *ctl, int nv, uint64_t *FA, uint64_t *FB, int *__restrict__ part) {
+    if (ctl->done || !ctl->compact) return;
+    const uint64_t *W = ((ctl->it - 1) & 1) ? FB : FA;
+    const int per = (nv + gridDim.x - 1) / gridDim.x;
+    const int v0 = blockIdx.x * per, v1 = min(nv, v0 + per);
+    long long c = 0;
+    for (int v = v0 + threadIdx.x; v < v1; v += kBB) c += W[v] != 0;
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    c = block_sum<kBB / MG_WAVE>(c, s_tmp);
+    if (threadIdx.x == 0) part[blockIdx.x] = (int)c;
+}
+__global__ __launch_bounds__(1024) void k_mbs_qscan(BCtl *ctl, int *__restrict__ part, int nparts) {
+    if (ctl->done || !ctl->compact) return;
+    __shared__ int s_tmp[1024 / MG_WAVE + 1];
+    __shared__ int s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int b = 0; b < nparts; b += 1024) {
+        const int i = b + threadIdx.x;
+        const int v = i < nparts ? part[i] : 0;
+        int t;
+        const int pre = block_excl_scan<1024 / MG_WAVE>(v, s_tmp, &t);
+        const int carry = s_carry;
+        if (i < nparts) part[i] = carry + pre;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = carry + t;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int cur = (ctl->it - 1) & 1;
+        ctl->qn[cur] = s_carry;
+        ctl->qvalid[cur] = 1;
+    }
+}
+__global__ __launch_bounds__(kBB) void k_mbs_qwrite(const BCtl *ctl, int nv, uint64_t *FA, uint64_t *FB,
+                                                    const int *__restrict__ part, int *QA, int *QB) {
+    if (ctl->done || !ctl->compact) return;
+    const int cur = (ctl->it - 1) & 1;
+    const uint64_t *W = cur ? FB : FA;
+    int *Q = cur ? QB : QA;
+    const int per = (nv + gridDim.x - 1) / gridDim.x;
+    const int v0 = blockIdx.x * per, v1 = min(nv, v0 + per);
+    long long off = part[blockIdx.x];
+    __shared__ int s_tmp[kBB / MG_WAVE + 1];
+    for (int vb = v0; vb < v1; vb += kBB) {
+        const int v = vb + threadIdx.x;
+        const uint64_t w = v < v1 ? W[v] : 0ull;
+        int t;
+        const int pre = block_excl_scan<kBB / MG_WAVE>(w != 0 ? 1 : 0, s_tmp, &t);
+        if (w) Q[off + pre] = v;
+        off += t;
+        __syncthreads();
+    }
+}
+
+// Sparse level, BFS, in one kernel: a wave per queued vertex u, lanes over its edges.
+// New bits are claimed on the visited word itself (atomicOr returns the bits that were
+// already set), so each (vertex, source) pair gets its level exactly once, and the
+// claimed bits go into the next frontier word; a vertex whose word was empty joins the
+// next queue.  The wave clears u's consumed word, and each workgroup leaves the new
+// frontier's size and degree sum as a partial.
+__global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
+                                                       uint64_t *FA, uint64_t *FB, uint64_t *vis, int *QA, int *QB,
+                                                       int *__restrict__ lev, int *__restrict__ part_n,
+                                                       long long *__restrict__ part_e) {
+    if (ctl->done || !ctl->sparse || ctl->need_full) return;
+    const int it = ctl->it;
+    const int cur = (it - 1) & 1;
+    uint64_t *F = cur ? FB : FA;
+    uint64_t *Fn = cur ? FA : FB;
+    const int *Q = cur ? QB : QA;
+    int *Qn = cur ? QA : QB;
+    int *qn_next = &ctl->qn[cur ^ 1];
+    const int n = ctl->qn[cur];
+    const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * kBB) >> 6;
+    int cnt = 0;
+    long long mf = 0;
+    for (int i = wave; i < n; i += nw) {
+        const int u = Q[i];
+        const uint64_t fu = F[u];
+        const int p1 = row[u + 1];
+        for (int p0 = row[u]; p0 < p1; p0 += MG_WAVE) {
+            const int p = p0 + lane();
+            bool add = false;
+            int w = 0;
+            if (p < p1) {
+                w = col[p];
+                const uint64_t m = fu & ~vis[w];
+                if (m) {
+                    const uint64_t nb = m & ~(uint64_t)atomicOr((unsigned long long *)&vis[w], (unsigned long long)m);
+                    if (nb) {
+                        add = atomicOr((unsigned long long *)&Fn[w], (unsigned long long)nb) == 0ull;
+                        uint64_t bits = nb;
+                        while (bits) { const int sl = __builtin_ctzll(bits); lev[(long long)w * 64 + sl] = it; bits &= bits - 1; }
+                    }
+                }
+            }
+            const int slot = wave_append(qn_next, add);
+            if (add) {
+                Qn[slot] = w;
+                ++cnt;
+                mf += row[w + 1] - row[w];
+            }
+        }
+        if (lane() == 0) F[u] = 0ull;
+    }
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
+    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
+    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+}
+
+// Sparse level, SSSP, in one kernel: a wave per queued vertex u with changed bits,
+// lanes = sources, its edges in turn; improvements (atomicMin) are or-ed into the next
+// changed word, whose vertex joins the next queue when the word was empty.
+__global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
+                                                        const int *__restrict__ wgt, uint64_t *FA, uint64_t *FB, int *QA, int *QB,
+                                                        int *__restrict__ dist, int *__restrict__ part_n,
+                                                        long long *__restrict__ part_e) {
+    if (ctl->done || !ctl->sparse || ctl->need_full) return;
+    const int cur = (ctl->it - 1) & 1;
+    uint64_t *C = cur ? FB : FA;
+    uint64_t *Cn = cur ? FA : FB;
+    const int *Q = cur ? QB : QA;
+    int *Qn = cur ? QA : QB;
+    int *qn_next = &ctl->qn[cur ^ 1];
+    const int n = ctl->qn[cur];
+    const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * kBB) >> 6;
+    int cnt = 0;
+    long long mf = 0;
+    for (int i = wave; i < n; i += nw) {
+        const int u = Q[i];
+        const uint64_t cu = C[u];
+        const bool mine = (cu >> lane()) & 1ull;
+        const int du = mine ? dist[(long long)u * 64 + lane()] : (int)MG_INF;
+        const int p1 = row[u + 1];
+        for (int p = row[u]; p < p1; ++p) {
+            const int w = col[p];
+            const int nd = du + wgt[p];
+            bool imp = false;
+            if (mine) {
+                int *dw = &dist[(long long)w * 64 + lane()];
+                if (nd < *dw) imp = atomicMin(dw, nd) > nd;
+            }
+            const uint64_t m = ballot(imp);
+            if (lane() == 0 && m && atomicOr((unsigned long long *)&Cn[w], (unsigned long long)m) == 0ull) {
+                Qn[atomicAdd(qn_next, 1)] = w;
+                ++cnt;
+                mf += row[w + 1] - row[w];
+            }
+        }
+        if (lane() == 0) C[u] = 0ull;
+    }
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
+    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
+    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+}
+
 
 __global__ void k_sum_parts(const int *__restrict__ pn, const long long *__restrict__ pe, int n, long long *__restrict__ out) {
     __shared__ long long s_tmp[256 / MG_WAVE];
@@ -536,6 +742,7 @@ struct mg_batch {
     hipStream_t stream = nullptr;
     int *row = nullptr, *col = nullptr, *w = nullptr;
     uint64_t *vis = nullptr, *F[2] = {nullptr, nullptr}, *D = nullptr, *E = nullptr;
+    int *Q[2] = {nullptr, nullptr};   // queues of F[0], F[1] on sparse levels
     int *lev = nullptr, *dist = nullptr, *src = nullptr, *slot = nullptr;
     int seed_cap = 0, nseeds = 0;  // device capacity of src/slot, sources of the last run
     bool slotted = false;          // the last run gave an explicit slot list
@@ -552,7 +759,8 @@ struct mg_batch {
     int nsrc = 0, levels = 0;     // slots of the last run, levels it swept
     bool symmetric = true;        // pull levels need every edge stored both ways
     BCtl *ctl = nullptr, *h_ctl = nullptr;  // device level control (BFS, SSSP)
-    hipGraphExec_t gexec = nullptr;         // K captured levels
+    hipGraphExec_t gexec = nullptr;         // K captured levels (all level kinds)
+    hipGraphExec_t gsparse = nullptr;       // K captured sparse levels (control + fused kernel)
     int gexec_levels = 0;
 };
 
@@ -560,7 +768,8 @@ static void batch_free(mg_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    void *ptrs[] = {b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
+    if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
+    void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
                     b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_e, b->tot, b->ctl};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
@@ -610,6 +819,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
               hipMalloc(&b->tot, sizeof(long long) * 4) == hipSuccess &&
               hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess &&
               hipMalloc(&b->ctl, sizeof(BCtl)) == hipSuccess &&
+              hipMalloc(&b->Q[0], sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->Q[1], sizeof(int) * (nv + 1)) == hipSuccess &&
               hipHostMalloc(&b->h_ctl, sizeof(BCtl), hipHostMallocDefault) == hipSuccess;
     if (ok && kind != MG_BATCH_SSSP) ok = hipMalloc(&b->lev, sizeof(int) * slots) == hipSuccess;
     if (ok && kind == MG_BATCH_SSSP)
@@ -748,7 +958,7 @@ static int run_bfs_bc(mg_batch *b) {
 
 // BFS (without BC) and SSSP: levels run under the device control block, K at a time
 // from a captured hipGraph; the host reads the control block once per K levels
-static int capture_levels(mg_batch *b, int K, hipGraphExec_t *out) {
+static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *out) {
     hipStream_t s = b->stream;
     const int nv = b->nv;
     const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
@@ -758,9 +968,28 @@ static int capture_levels(mg_batch *b, int K, hipGraphExec_t *out) {
     const bool sssp = b->kind == MG_BATCH_SSSP;
     hipGraph_t g = nullptr;
     MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const float beta = b->opt.beta > 0 ? b->opt.beta : 18.0f;
+    const LevelRule rule{b->ne, nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0};
+    const int gq = grid_for(nv, 2048, 2048);
+    auto sparse_level = [&]() {
+        if (sssp)
+            hipLaunchKernelGGL(k_mbs_level_sssp, dim3(gv), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->w, b->F[0], b->F[1], b->Q[0],
+                               b->Q[1], b->dist, b->part_n, b->part_e);
+        else
+            hipLaunchKernelGGL(k_mbs_level_bfs, dim3(gv), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->F[0], b->F[1], b->vis, b->Q[0],
+                               b->Q[1], b->lev, b->part_n, b->part_e);
+    };
     for (int k = 0; k < K; ++k) {
-        hipLaunchKernelGGL(k_mb_ctl, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, gv, b->ne, alpha, b->opt.mode,
-                           b->symmetric ? 1 : 0);
+        if (sparse_only) {  // control + one fused kernel per level
+            hipLaunchKernelGGL(k_mb_ctl_sparse, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, gv, rule);
+            sparse_level();
+            continue;
+        }
+        hipLaunchKernelGGL(k_mb_ctl, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, gv, rule);
+        hipLaunchKernelGGL(k_mbs_qcount, dim3(gq), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->F[0], b->F[1], b->part_n);
+        hipLaunchKernelGGL(k_mbs_qscan, dim3(1), dim3(1024), 0, s, b->ctl, b->part_n, gq);
+        hipLaunchKernelGGL(k_mbs_qwrite, dim3(gq), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->F[0], b->F[1], b->part_n, b->Q[0],
+                           b->Q[1]);
         hipLaunchKernelGGL(k_mb_fill_next, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, (const BCtl *)b->ctl, b->F[0],
                            b->F[1], (long long)nv + 1);
         if (sssp) {
@@ -778,6 +1007,7 @@ static int capture_levels(mg_batch *b, int K, hipGraphExec_t *out) {
             hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->vis, b->F[0], b->F[1],
                                b->lev, 0, b->part_n, b->part_e);
         }
+        sparse_level();
     }
     hipError_t ce = hipStreamEndCapture(s, &g);
     if (ce != hipSuccess) return mg_fail(MG_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ce));
@@ -797,8 +1027,10 @@ static int run_levels(mg_batch *b) {
     const int K = b->opt.graph_levels > 0 ? b->opt.graph_levels : 8;
     if (!b->gexec || b->gexec_levels != K) {  // captured once per batch, on an idle stream
         if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        if (b->gsparse) { hipGraphExecDestroy(b->gsparse); b->gsparse = nullptr; }
         MG_HIP(hipStreamSynchronize(s));
-        MG_TRY(capture_levels(b, K, &b->gexec));
+        MG_TRY(capture_levels(b, K, false, &b->gexec));
+        MG_TRY(capture_levels(b, K, true, &b->gsparse));
         b->gexec_levels = K;
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
@@ -812,8 +1044,11 @@ static int run_levels(mg_batch *b) {
     hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->F[1], b->F[0], b->part_n,
                        b->part_e);
     MG_HIP(hipGetLastError());
+    memset(b->h_ctl, 0, sizeof(BCtl));  // the first replay is always the full graph
     for (;;) {
-        MG_HIP(hipGraphLaunch(b->gexec, s));
+        // after a sparse level the sparse-only graph runs on until a level needs more
+        const bool sparse_next = b->h_ctl->it > 0 && b->h_ctl->sparse && !b->h_ctl->need_full;
+        MG_HIP(hipGraphLaunch(sparse_next ? b->gsparse : b->gexec, s));
         MG_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s));
         MG_HIP(hipStreamSynchronize(s));
         if (b->h_ctl->done) break;
