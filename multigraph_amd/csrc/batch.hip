@@ -165,6 +165,18 @@ __device__ __forceinline__ bool level_runs(const BCtl *ctl, int want) {
     return want < 0 || ctl->push == want;
 }
 
+// the frontier a workgroup produced, as partials: its size, its degree sum and its
+// largest degree (a level may walk a queue only when no frontier vertex is split
+// into several chunks, so no wave walks more than one chunk's edges)
+__device__ __forceinline__ void put_parts(int cnt, long long mf, int mx, int *__restrict__ part_n,
+                                          long long *__restrict__ part_e, int *__restrict__ part_x) {
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
+    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
+    const long long tx = block_max<kBB / MG_WAVE>((long long)mx, s_tmp);
+    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; part_x[blockIdx.x] = (int)tx; }
+}
+
 // push: a wave per chunk whose vertex u is in the frontier.  BFS: lanes over edges,
 // new bits or-ed into Fn.  BC: lanes = sources, edges in turn, sigma[u][s] added
 // into sigma[w][s] for every new bit s.
@@ -280,12 +292,12 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
 // and leave the new frontier's size and degree sum as per-workgroup partials
 __global__ __launch_bounds__(kBB) void k_mb_apply(const BCtl *ctl, int nv, const int *__restrict__ row, uint64_t *__restrict__ vis,
                                                   uint64_t *FA, uint64_t *FB, int *__restrict__ lev, int it,
-                                                  int *__restrict__ part_n, long long *__restrict__ part_e) {
+                                                  int *__restrict__ part_n, long long *__restrict__ part_e, int *__restrict__ part_x) {
     if (!level_runs(ctl, -1)) return;
     uint64_t *F, *Fn;
     cur_next(ctl, FA, FB, &F, &Fn);
     if (ctl) it = ctl->it;
-    int cnt = 0;
+    int cnt = 0, mx = 0;
     long long mf = 0;
     for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB) {
         const uint64_t f = Fn[v];
@@ -296,31 +308,27 @@ __global__ __launch_bounds__(kBB) void k_mb_apply(const BCtl *ctl, int nv, const
         if (!nw) continue;
         vis[v] = vv | nw;
         ++cnt;
-        mf += row[v + 1] - row[v];
+        const int dg = row[v + 1] - row[v];
+        mf += dg;
+        mx = max(mx, dg);
         uint64_t m = nw;
         while (m) { const int s = __builtin_ctzll(m); lev[(long long)v * 64 + s] = it; m &= m - 1; }
     }
-    __shared__ long long s_tmp[kBB / MG_WAVE];
-    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
-    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
-    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+    put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
 
 // size and degree sum of the nonzero words of the next frontier (without a control
 // block: of WB)
 __global__ __launch_bounds__(kBB) void k_mb_count(const BCtl *ctl, int nv, const int *__restrict__ row, uint64_t *WA, uint64_t *WB,
-                                                  int *__restrict__ part_n, long long *__restrict__ part_e) {
+                                                  int *__restrict__ part_n, long long *__restrict__ part_e, int *__restrict__ part_x) {
     if (!level_runs(ctl, -1)) return;
     uint64_t *cur, *W;
     cur_next(ctl, WA, WB, &cur, &W);
-    int cnt = 0;
+    int cnt = 0, mx = 0;
     long long mf = 0;
     for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB)
-        if (W[v]) { ++cnt; mf += row[v + 1] - row[v]; }
-    __shared__ long long s_tmp[kBB / MG_WAVE];
-    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
-    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
-    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+        if (W[v]) { const int dg = row[v + 1] - row[v]; ++cnt; mf += dg; mx = max(mx, dg); }
+    put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
 
 // zero the next frontier (a fill kernel: graphs record no memset nodes)
@@ -342,9 +350,9 @@ __global__ void k_mb_ctl_init(BCtl *ctl, unsigned long long all) {
 // degree sum is below E / alpha) and whether it walks a queue (sparse: push levels of
 // fewer than nv / beta vertices, so per-level work follows the frontier, not the
 // vertex count; high-diameter members of an uneven batch run thousands of such levels)
-struct LevelRule { long long ne; int nv; float alpha, beta; int forced, symmetric; };
+struct LevelRule { long long ne; int nv; float alpha, beta; int forced, symmetric, light; };
 
-__device__ __forceinline__ void decide(BCtl &c, long long a, long long b, const LevelRule &r) {
+__device__ __forceinline__ void decide(BCtl &c, long long a, long long b, long long x, const LevelRule &r) {
     c.n = a;
     c.mf = b;
     c.need_full = 0;
@@ -356,7 +364,7 @@ __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, const 
     if (!r.symmetric) push = 1;
     c.push = push;
     const int cur = (c.it - 1) & 1, next = c.it & 1;
-    const int sparse = r.forced != MG_MODE_DENSE && push && (double)a * r.beta < (double)r.nv;
+    const int sparse = r.forced != MG_MODE_DENSE && push && (double)a * r.beta < (double)r.nv && x <= r.light;
     c.compact = sparse && !c.qvalid[cur];
     c.fill = !sparse || !c.prev_sparse;
     c.sparse = sparse;
@@ -365,21 +373,24 @@ __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, const 
     c.prev_sparse = sparse;
 }
 
-__device__ __forceinline__ void sum_partials(const int *pn, const long long *pe, int nparts, long long *a, long long *b) {
+__device__ __forceinline__ void sum_partials(const int *pn, const long long *pe, const int *px, int nparts, long long *a,
+                                             long long *b, long long *c) {
     __shared__ long long s_tmp[256 / MG_WAVE];
-    long long x = 0, y = 0;
-    for (int i = threadIdx.x; i < nparts; i += 256) { x += pn[i]; y += pe[i]; }
+    long long x = 0, y = 0, z = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) { x += pn[i]; y += pe[i]; z = max(z, (long long)px[i]); }
     *a = block_sum<256 / MG_WAVE>(x, s_tmp);
     *b = block_sum<256 / MG_WAVE>(y, s_tmp);
+    *c = block_max<256 / MG_WAVE>(z, s_tmp);
 }
 
-__global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe, int nparts, LevelRule r) {
+__global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe, const int *__restrict__ px,
+                         int nparts, LevelRule r) {
     if (ctl->done) return;
-    long long a, b;
-    sum_partials(pn, pe, nparts, &a, &b);
+    long long a, b, x;
+    sum_partials(pn, pe, px, nparts, &a, &b, &x);
     if (threadIdx.x != 0) return;
     BCtl c = *ctl;
-    decide(c, a, b, r);
+    decide(c, a, b, x, r);
     *ctl = c;
 }
 
@@ -387,14 +398,14 @@ __global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long 
 // level is sparse and needs neither a compaction nor a full clear; otherwise it flags
 // need_full (state untouched) and the host replays the full graph, whose control
 // kernel decides again from the same partials
-__global__ void k_mb_ctl_sparse(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe, int nparts,
-                                LevelRule r) {
+__global__ void k_mb_ctl_sparse(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe,
+                                const int *__restrict__ px, int nparts, LevelRule r) {
     if (ctl->done || ctl->need_full) return;
-    long long a, b;
-    sum_partials(pn, pe, nparts, &a, &b);
+    long long a, b, x;
+    sum_partials(pn, pe, px, nparts, &a, &b, &x);
     if (threadIdx.x != 0) return;
     BCtl c = *ctl;
-    decide(c, a, b, r);
+    decide(c, a, b, x, r);
     if (!c.done && (!c.sparse || c.compact || c.fill)) { ctl->need_full = 1; return; }
     *ctl = c;
 }
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_qwrite(const BCtl *ctl, int nv, uin
 __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
                                                        uint64_t *FA, uint64_t *FB, uint64_t *vis, int *QA, int *QB,
                                                        int *__restrict__ lev, int *__restrict__ part_n,
-                                                       long long *__restrict__ part_e) {
+                                                       long long *__restrict__ part_e, int *__restrict__ part_x) {
     if (ctl->done || !ctl->sparse || ctl->need_full) return;
     const int it = ctl->it;
     const int cur = (it - 1) & 1;
@@ -478,7 +489,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__r
     const int n = ctl->qn[cur];
     const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
     const int nw = (gridDim.x * kBB) >> 6;
-    int cnt = 0;
+    int cnt = 0, mx = 0;
     long long mf = 0;
     for (int i = wave; i < n; i += nw) {
         const int u = Q[i];
@@ -503,16 +514,15 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__r
             const int slot = wave_append(qn_next, add);
             if (add) {
                 Qn[slot] = w;
+                const int dg = row[w + 1] - row[w];
                 ++cnt;
-                mf += row[w + 1] - row[w];
+                mf += dg;
+                mx = max(mx, dg);
             }
         }
         if (lane() == 0) F[u] = 0ull;
     }
-    __shared__ long long s_tmp[kBB / MG_WAVE];
-    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
-    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
-    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+    put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
 
 // Sparse level, SSSP, in one kernel: a wave per queued vertex u with changed bits,
@@ -521,7 +531,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__r
 __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
                                                         const int *__restrict__ wgt, uint64_t *FA, uint64_t *FB, int *QA, int *QB,
                                                         int *__restrict__ dist, int *__restrict__ part_n,
-                                                        long long *__restrict__ part_e) {
+                                                        long long *__restrict__ part_e, int *__restrict__ part_x) {
     if (ctl->done || !ctl->sparse || ctl->need_full) return;
     const int cur = (ctl->it - 1) & 1;
     uint64_t *C = cur ? FB : FA;
@@ -532,7 +542,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__
     const int n = ctl->qn[cur];
     const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
     const int nw = (gridDim.x * kBB) >> 6;
-    int cnt = 0;
+    int cnt = 0, mx = 0;
     long long mf = 0;
     for (int i = wave; i < n; i += nw) {
         const int u = Q[i];
@@ -551,16 +561,15 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__
             const uint64_t m = ballot(imp);
             if (lane() == 0 && m && atomicOr((unsigned long long *)&Cn[w], (unsigned long long)m) == 0ull) {
                 Qn[atomicAdd(qn_next, 1)] = w;
+                const int dg = row[w + 1] - row[w];
                 ++cnt;
-                mf += row[w + 1] - row[w];
+                mf += dg;
+                mx = max(mx, dg);
             }
         }
         if (lane() == 0) C[u] = 0ull;
     }
-    __shared__ long long s_tmp[kBB / MG_WAVE];
-    const long long tn = block_sum<kBB / MG_WAVE>((long long)cnt, s_tmp);
-    const long long te = block_sum<kBB / MG_WAVE>(mf, s_tmp);
-    if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_e[blockIdx.x] = te; }
+    put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
 
 
@@ -754,7 +763,7 @@ struct mg_batch {
     int *logv = nullptr;          // (vertex, word) entries of every level (BC)
     uint64_t *logw = nullptr;
     long long log_cap = 0;
-    int *part_n = nullptr;
+    int *part_n = nullptr, *part_x = nullptr;  // per-workgroup frontier size, largest degree
     long long *part_e = nullptr, *tot = nullptr, *h_tot = nullptr;
     int nsrc = 0, levels = 0;     // slots of the last run, levels it swept
     bool symmetric = true;        // pull levels need every edge stored both ways
@@ -770,7 +779,7 @@ static void batch_free(mg_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
-                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_e, b->tot, b->ctl};
+                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -815,6 +824,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
               hipMalloc(&b->F[0], sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
               hipMalloc(&b->F[1], sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
               hipMalloc(&b->part_n, sizeof(int) * 4096) == hipSuccess &&
+              hipMalloc(&b->part_x, sizeof(int) * 4096) == hipSuccess &&
               hipMalloc(&b->part_e, sizeof(long long) * 4096) == hipSuccess &&
               hipMalloc(&b->tot, sizeof(long long) * 4) == hipSuccess &&
               hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess &&
@@ -913,7 +923,7 @@ static int run_bfs_bc(mg_batch *b) {
                        b->slotted ? b->slot : nullptr, b->vis, b->F[0], b->lev, bc ? b->sigma : nullptr, (int *)nullptr);
     std::vector<long long> off{0};  // BC level log offsets
     long long n = 0, mf = 0;
-    hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->F[1], b->F[0], b->part_n, b->part_e);
+    hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->F[1], b->F[0], b->part_n, b->part_e, b->part_x);
     MG_TRY(read_counts(b, gv, &n, &mf));
     if (bc) { long long c0 = 0; MG_TRY(compact(b, b->F[0], 0, &c0)); off.push_back(c0); }
     int cur = 0, it = 1;
@@ -931,7 +941,7 @@ static int run_bfs_bc(mg_batch *b) {
             if (bc) hipLaunchKernelGGL(k_mb_pull_bc, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, all, b->col, F, b->vis, Fn, b->sigma);
             else hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, all, b->col, F, b->vis, Fn);
         }
-        hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->vis, F, Fn, b->lev, it, b->part_n, b->part_e);
+        hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->vis, F, Fn, b->lev, it, b->part_n, b->part_e, b->part_x);
         MG_TRY(read_counts(b, gv, &n, &mf));
         if (bc && n > 0) { long long c = 0; MG_TRY(compact(b, Fn, off.back(), &c)); off.push_back(off.back() + c); }
         cur ^= 1;
@@ -969,23 +979,24 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
     hipGraph_t g = nullptr;
     MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const float beta = b->opt.beta > 0 ? b->opt.beta : 18.0f;
-    const LevelRule rule{b->ne, nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0};
+    const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;
+    const LevelRule rule{b->ne, nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0, chunk};
     const int gq = grid_for(nv, 2048, 2048);
     auto sparse_level = [&]() {
         if (sssp)
             hipLaunchKernelGGL(k_mbs_level_sssp, dim3(gv), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->w, b->F[0], b->F[1], b->Q[0],
-                               b->Q[1], b->dist, b->part_n, b->part_e);
+                               b->Q[1], b->dist, b->part_n, b->part_e, b->part_x);
         else
             hipLaunchKernelGGL(k_mbs_level_bfs, dim3(gv), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->F[0], b->F[1], b->vis, b->Q[0],
-                               b->Q[1], b->lev, b->part_n, b->part_e);
+                               b->Q[1], b->lev, b->part_n, b->part_e, b->part_x);
     };
     for (int k = 0; k < K; ++k) {
         if (sparse_only) {  // control + one fused kernel per level
-            hipLaunchKernelGGL(k_mb_ctl_sparse, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, gv, rule);
+            hipLaunchKernelGGL(k_mb_ctl_sparse, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, b->part_x, gv, rule);
             sparse_level();
             continue;
         }
-        hipLaunchKernelGGL(k_mb_ctl, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, gv, rule);
+        hipLaunchKernelGGL(k_mb_ctl, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, b->part_x, gv, rule);
         hipLaunchKernelGGL(k_mbs_qcount, dim3(gq), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->F[0], b->F[1], b->part_n);
         hipLaunchKernelGGL(k_mbs_qscan, dim3(1), dim3(1024), 0, s, b->ctl, b->part_n, gq);
         hipLaunchKernelGGL(k_mbs_qwrite, dim3(gq), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->F[0], b->F[1], b->part_n, b->Q[0],
@@ -998,14 +1009,14 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
             hipLaunchKernelGGL(k_mb_sssp, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col, b->w, b->F[0],
                                b->F[1], b->dist);
             hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->F[0], b->F[1], b->part_n,
-                               b->part_e);
+                               b->part_e, b->part_x);
         } else {
             hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col,
                                b->F[0], b->vis, b->F[1], (double *)nullptr);
             hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, 0ull, b->col,
                                b->F[0], b->vis, b->F[1]);
             hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->vis, b->F[0], b->F[1],
-                               b->lev, 0, b->part_n, b->part_e);
+                               b->lev, 0, b->part_n, b->part_e, b->part_x);
         }
         sparse_level();
     }
@@ -1042,7 +1053,7 @@ static int run_levels(mg_batch *b) {
                        sssp ? b->dist : (int *)nullptr);
     hipLaunchKernelGGL(k_mb_ctl_init, dim3(1), dim3(1), 0, s, b->ctl, (unsigned long long)b->all);
     hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->F[1], b->F[0], b->part_n,
-                       b->part_e);
+                       b->part_e, b->part_x);
     MG_HIP(hipGetLastError());
     memset(b->h_ctl, 0, sizeof(BCtl));  // the first replay is always the full graph
     for (;;) {

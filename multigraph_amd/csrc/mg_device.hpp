@@ -100,6 +100,25 @@ __device__ __forceinline__ T block_sum(T v, T *tmp) {
     return r;
 }
 
+template <int NW, class T>
+__device__ __forceinline__ T block_max(T v, T *tmp) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { const T t = __shfl_xor(v, o); v = t > v ? t : v; }
+    const int w = threadIdx.x / MG_WAVE;
+    if (lane() == 0) tmp[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T r = tmp[0];
+#pragma unroll
+        for (int i = 1; i < NW; ++i) r = tmp[i] > r ? tmp[i] : r;
+        tmp[0] = r;
+    }
+    __syncthreads();
+    const T r = tmp[0];
+    __syncthreads();
+    return r;
+}
+
 // Neumaier-compensated FP32 summation: (s, c) with s + c the running sum.
 struct KSum {
     float s, c;
