@@ -70,7 +70,9 @@ typedef struct mg_engine mg_engine;   /* device layout + run state (struct HYB +
 typedef struct mg_options {
     int    mode;            /* MG_MODE_*                                                */
     float  alpha;           /* push->pull when frontier edges > unexplored edges / alpha */
-    float  beta;            /* pull->push when frontier vertices < nv / beta            */
+    float  beta;            /* pull->push when frontier vertices < nv / beta; batched
+                               BFS / SSSP push levels walk a queue of the frontier when
+                               its vertices + edges < (nv + E) / beta                   */
     int    huge_degree;     /* vertices at or above this degree are split across blocks */
     int    grid_blocks;     /* workgroups of the grid-stride frontier kernels           */
     int    graph_levels;    /* levels / iterations captured per hipGraph launch (0: BFS,
@@ -80,6 +82,10 @@ typedef struct mg_options {
     int    pr_max_iter;     /* PageRank iteration cap (0: unbounded, as the reference)  */
     int    part_log2;       /* log2 of the vertex-partition size of the tiled layout    */
     int    batch_chunk;     /* batched runs: edges per static chunk (0: 1024)           */
+    int    persist_levels;  /* batched BFS / SSSP: run sparse levels in a persistent
+                               cooperative launch, at most this many per launch (-1:
+                               unbounded; 0, the default: every level from the captured
+                               graphs, which measures faster on MI355X)                 */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */
@@ -158,6 +164,12 @@ int mg_batch_get(mg_batch *b, int which, void *host_dst, int64_t bytes);
 int mg_batch_release(mg_batch *b);
 /* levels swept by the last mg_batch_run (BFS/BC: deepest level + 1; SSSP: sweeps) */
 int mg_batch_levels(const mg_batch *b, int *levels);
+/* How the last BFS / SSSP mg_batch_run spent its levels: out[0] levels, out[1] dense
+ * levels (every chunk), out[2] sparse levels (a queue of the frontier), out[3] of
+ * those run by persistent launches, out[4] host round trips (control-block reads).
+ * n entries are written (at most 5).  No counterpart in the reference: its level
+ * loop prints a per-iteration trace instead (code/process.cu:1300-1305). */
+int mg_batch_stats(const mg_batch *b, int64_t *out, int n);
 
 /* ---- autotuner ------------------------------------------------------------- */
 /* Pick the tunables of mg_options (push/pull thresholds alpha and beta, grid shape,

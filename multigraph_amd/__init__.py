@@ -49,6 +49,7 @@ class Options(C.Structure):
         ("pr_max_iter", C.c_int),
         ("part_log2", C.c_int),
         ("batch_chunk", C.c_int),
+        ("persist_levels", C.c_int),
     ]
 
 
@@ -89,6 +90,7 @@ _SIGS = {
     "mg_batch_get": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "mg_batch_release": (C.c_int, [_P]),
     "mg_batch_levels": (C.c_int, [_P, _I32P]),
+    "mg_batch_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.c_int]),
     "mg_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, C.c_int, C.POINTER(Options), C.POINTER(C.c_float),
                               C.POINTER(C.c_float)]),
     "mg_batch_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, _I32P, C.c_int, C.POINTER(Options),
@@ -361,6 +363,13 @@ class Batch:
         n = C.c_int32()
         _check(lib().mg_batch_levels(self._h, C.byref(n)))
         return n.value
+
+    def stats(self) -> dict:
+        """how the last BFS / SSSP run spent its levels"""
+        out = (C.c_int64 * 5)()
+        _check(lib().mg_batch_stats(self._h, out, 5))
+        return {"levels": out[0], "dense_levels": out[1], "sparse_levels": out[2], "persistent_levels": out[3],
+                "host_round_trips": out[4]}
 
     def bc(self) -> np.ndarray:
         out = np.empty(self.nv, dtype=np.float32)

@@ -145,6 +145,7 @@ struct BCtl {
     int need_full;    // the sparse-only graph met a level it cannot run: replay the full graph
     int qn[2];    // queue lengths of F[0], F[1]
     int qvalid[2];    // the queue of F[b] lists its nonzero words
+    int ndense, nsparse, npersist;  // levels run: dense, sparse, sparse by persistent launches
     long long n, mf;  // size and degree sum of the last produced frontier
     unsigned long long all;  // the slots in use
 };
@@ -347,9 +348,10 @@ __global__ void k_mb_ctl_init(BCtl *ctl, unsigned long long all) {
 
 // between levels: total the partials of the last produced frontier; stop when it is
 // empty, else open the next level and choose its direction (push while the frontier's
-// degree sum is below E / alpha) and whether it walks a queue (sparse: push levels of
-// fewer than nv / beta vertices, so per-level work follows the frontier, not the
-// vertex count; high-diameter members of an uneven batch run thousands of such levels)
+// degree sum is below E / alpha) and whether it walks a queue (sparse: push levels
+// whose vertices plus edges are below (nv + E) / beta and whose vertices all fit one
+// chunk, so per-level work follows the frontier, not the graph; high-diameter members
+// of an uneven batch run thousands of such levels)
 struct LevelRule { long long ne; int nv; float alpha, beta; int forced, symmetric, light; };
 
 __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, long long x, const LevelRule &r) {
@@ -364,7 +366,7 @@ __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, long l
     if (!r.symmetric) push = 1;
     c.push = push;
     const int cur = (c.it - 1) & 1, next = c.it & 1;
-    const int sparse = r.forced != MG_MODE_DENSE && push && (double)a * r.beta < (double)r.nv && x <= r.light;
+    const int sparse = r.forced != MG_MODE_DENSE && push && (double)(a + b) * r.beta < (double)r.nv + (double)r.ne && x <= r.light;
     c.compact = sparse && !c.qvalid[cur];
     c.fill = !sparse || !c.prev_sparse;
     c.sparse = sparse;
@@ -391,6 +393,7 @@ __global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long 
     if (threadIdx.x != 0) return;
     BCtl c = *ctl;
     decide(c, a, b, x, r);
+    if (!c.done) { if (c.sparse) ++c.nsparse; else ++c.ndense; }
     *ctl = c;
 }
 
@@ -407,6 +410,7 @@ __global__ void k_mb_ctl_sparse(BCtl *ctl, const int *__restrict__ pn, const lon
     BCtl c = *ctl;
     decide(c, a, b, x, r);
     if (!c.done && (!c.sparse || c.compact || c.fill)) { ctl->need_full = 1; return; }
+    if (!c.done) ++c.nsparse;
     *ctl = c;
 }
 
@@ -468,12 +472,149 @@ __global__ __launch_bounds__(kBB) void k_mbs_qwrite(const BCtl *ctl, int nv, uin
     }
 }
 
-// Sparse level, BFS, in one kernel: a wave per queued vertex u, lanes over its edges.
-// New bits are claimed on the visited word itself (atomicOr returns the bits that were
-// already set), so each (vertex, source) pair gets its level exactly once, and the
-// claimed bits go into the next frontier word; a vertex whose word was empty joins the
-// next queue.  The wave clears u's consumed word, and each workgroup leaves the new
-// frontier's size and degree sum as a partial.
+// Sparse levels take the queue 64 entries per wave, one per lane.  A lane whose vertex
+// has little work (kLaneWork edges, or bit-edge pairs for SSSP) walks it alone, the
+// wave stepping all such lanes together for the largest of their trip counts so that
+// queue appends stay wave-aggregated; the heavier vertices of the 64 are then walked
+// one at a time by the whole wave.  Small tails (grids, RMAT fringes: degree a few)
+// thus keep 64 independent load chains in flight per wave instead of one.
+constexpr int kLaneWork = 16;
+
+// BFS: new bits are claimed on the visited word itself (atomicOr returns the bits
+// that were already set), so each (vertex, source) pair gets its level exactly once,
+// and the claimed bits go into the next frontier word; a vertex whose word was empty
+// joins the next queue.  Every queued vertex is queued once and has at most one chunk,
+// so its consumed word is cleared by the lane that owns it.  The thread counts the new
+// frontier's vertices, degree sum and largest degree.
+__device__ __forceinline__ bool bfs_visit(int it, int w, uint64_t fu, uint64_t *Fn, uint64_t *vis, int *__restrict__ lev) {
+    const uint64_t m = fu & ~vis[w];
+    if (!m) return false;
+    const uint64_t nb = m & ~(uint64_t)atomicOr((unsigned long long *)&vis[w], (unsigned long long)m);
+    if (!nb) return false;
+    const bool add = atomicOr((unsigned long long *)&Fn[w], (unsigned long long)nb) == 0ull;
+    uint64_t bits = nb;
+    while (bits) { const int sl = __builtin_ctzll(bits); lev[(long long)w * 64 + sl] = it; bits &= bits - 1; }
+    return add;
+}
+
+__device__ __forceinline__ void queue_add(bool add, int w, const int *__restrict__ row, int *Qn, int *qn_next, int &cnt,
+                                          long long &mf, int &mx) {
+    const int slot = wave_append(qn_next, add);
+    if (add) {
+        Qn[slot] = w;
+        const int dg = row[w + 1] - row[w];
+        ++cnt;
+        mf += dg;
+        mx = max(mx, dg);
+    }
+}
+
+__device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict__ row, const int *__restrict__ col, uint64_t *F,
+                                           uint64_t *Fn, uint64_t *vis, const int *Q, int *Qn, int *qn_next,
+                                           int *__restrict__ lev, int &cnt, long long &mf, int &mx) {
+    const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * kBB) >> 6;
+    for (int base = wave * MG_WAVE; base < n; base += nw * MG_WAVE) {
+        const int i = base + lane();
+        int u = -1, p0 = 0, p1 = 0;
+        uint64_t fu = 0;
+        if (i < n) { u = Q[i]; fu = F[u]; p0 = row[u]; p1 = row[u + 1]; }
+        const bool alone = u >= 0 && p1 - p0 <= kLaneWork;
+        int steps = alone ? p1 - p0 : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
+        for (int k = 0; k < steps; ++k) {
+            const bool mine = alone && p0 + k < p1;
+            const int w = mine ? col[p0 + k] : 0;
+            const bool add = mine && bfs_visit(it, w, fu, Fn, vis, lev);
+            queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
+        }
+        uint64_t heavy = ballot(u >= 0 && !alone);
+        while (heavy) {  // the whole wave over one heavier vertex's edges
+            const int l = __builtin_ctzll(heavy);
+            heavy &= heavy - 1;
+            const int q0 = __shfl(p0, l), q1 = __shfl(p1, l);
+            const uint64_t fv = (uint64_t)__shfl((long long)fu, l);
+            for (int pb = q0; pb < q1; pb += MG_WAVE) {
+                const int pp = pb + lane();
+                const int w = pp < q1 ? col[pp] : 0;
+                const bool add = pp < q1 && bfs_visit(it, w, fv, Fn, vis, lev);
+                queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
+            }
+        }
+        if (u >= 0) F[u] = 0ull;
+    }
+}
+
+// SSSP: a queued vertex u has changed bits (the sources whose distance at u dropped);
+// along each edge (u, w) the distance of such a source at w is lowered to
+// dist[u][s] + weight (read first, atomicMin only when it improves) and the source's
+// bit is or-ed into w's next changed word; w joins the next queue when that word was
+// empty.  A lane walks its vertex's (bit, edge) pairs alone when they are few; a
+// heavier vertex is walked by the wave with lanes = sources, edges in turn.
+__device__ __forceinline__ void sparse_sssp(int n, const int *__restrict__ row, const int *__restrict__ col,
+                                            const int *__restrict__ wgt, uint64_t *C, uint64_t *Cn, const int *Q, int *Qn,
+                                            int *qn_next, int *__restrict__ dist, int &cnt, long long &mf, int &mx) {
+    const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * kBB) >> 6;
+    for (int base = wave * MG_WAVE; base < n; base += nw * MG_WAVE) {
+        const int i = base + lane();
+        int u = -1, p0 = 0, p1 = 0;
+        uint64_t cu = 0;
+        if (i < n) { u = Q[i]; cu = C[u]; p0 = row[u]; p1 = row[u + 1]; }
+        const int work = u >= 0 ? __builtin_popcountll(cu) * (p1 - p0) : 0;
+        const bool alone = u >= 0 && work <= kLaneWork;
+        int steps = alone ? work : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
+        uint64_t rem = alone ? cu : 0ull;  // bits still to walk; the lowest is the current source
+        int p = p0, du = 0;
+        if (rem) du = dist[(long long)u * 64 + __builtin_ctzll(rem)];
+        for (int k = 0; k < steps; ++k) {
+            bool add = false;
+            int w = 0;
+            if (rem) {
+                const int sl = __builtin_ctzll(rem);
+                w = col[p];
+                const int nd = du + wgt[p];
+                int *dw = &dist[(long long)w * 64 + sl];
+                if (nd < *dw && atomicMin(dw, nd) > nd)
+                    add = atomicOr((unsigned long long *)&Cn[w], 1ull << sl) == 0ull;
+                if (++p == p1) {
+                    p = p0;
+                    rem &= rem - 1;
+                    if (rem) du = dist[(long long)u * 64 + __builtin_ctzll(rem)];
+                }
+            }
+            queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
+        }
+        uint64_t heavy = ballot(u >= 0 && !alone);
+        while (heavy) {
+            const int l = __builtin_ctzll(heavy);
+            heavy &= heavy - 1;
+            const int v = __shfl(u, l), q0 = __shfl(p0, l), q1 = __shfl(p1, l);
+            const uint64_t cv = (uint64_t)__shfl((long long)cu, l);
+            const bool mine = (cv >> lane()) & 1ull;
+            const int dv = mine ? dist[(long long)v * 64 + lane()] : (int)MG_INF;
+            for (int pp = q0; pp < q1; ++pp) {
+                const int w = col[pp];
+                const int nd = dv + wgt[pp];
+                bool imp = false;
+                if (mine) {
+                    int *dw = &dist[(long long)w * 64 + lane()];
+                    if (nd < *dw) imp = atomicMin(dw, nd) > nd;
+                }
+                const uint64_t m = ballot(imp);
+                bool add = false;
+                if (lane() == 0 && m) add = atomicOr((unsigned long long *)&Cn[w], (unsigned long long)m) == 0ull;
+                queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
+            }
+        }
+        if (u >= 0) C[u] = 0ull;
+    }
+}
+
+// one sparse level from a captured graph (gated on the control block)
 __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
                                                        uint64_t *FA, uint64_t *FB, uint64_t *vis, int *QA, int *QB,
                                                        int *__restrict__ lev, int *__restrict__ part_n,
@@ -481,97 +622,172 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__r
     if (ctl->done || !ctl->sparse || ctl->need_full) return;
     const int it = ctl->it;
     const int cur = (it - 1) & 1;
-    uint64_t *F = cur ? FB : FA;
-    uint64_t *Fn = cur ? FA : FB;
-    const int *Q = cur ? QB : QA;
-    int *Qn = cur ? QA : QB;
-    int *qn_next = &ctl->qn[cur ^ 1];
-    const int n = ctl->qn[cur];
-    const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * kBB) >> 6;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    for (int i = wave; i < n; i += nw) {
-        const int u = Q[i];
-        const uint64_t fu = F[u];
-        const int p1 = row[u + 1];
-        for (int p0 = row[u]; p0 < p1; p0 += MG_WAVE) {
-            const int p = p0 + lane();
-            bool add = false;
-            int w = 0;
-            if (p < p1) {
-                w = col[p];
-                const uint64_t m = fu & ~vis[w];
-                if (m) {
-                    const uint64_t nb = m & ~(uint64_t)atomicOr((unsigned long long *)&vis[w], (unsigned long long)m);
-                    if (nb) {
-                        add = atomicOr((unsigned long long *)&Fn[w], (unsigned long long)nb) == 0ull;
-                        uint64_t bits = nb;
-                        while (bits) { const int sl = __builtin_ctzll(bits); lev[(long long)w * 64 + sl] = it; bits &= bits - 1; }
-                    }
-                }
-            }
-            const int slot = wave_append(qn_next, add);
-            if (add) {
-                Qn[slot] = w;
-                const int dg = row[w + 1] - row[w];
-                ++cnt;
-                mf += dg;
-                mx = max(mx, dg);
-            }
-        }
-        if (lane() == 0) F[u] = 0ull;
-    }
+    sparse_bfs(it, ctl->qn[cur], row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
+               lev, cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
 
-// Sparse level, SSSP, in one kernel: a wave per queued vertex u with changed bits,
-// lanes = sources, its edges in turn; improvements (atomicMin) are or-ed into the next
-// changed word, whose vertex joins the next queue when the word was empty.
 __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
                                                         const int *__restrict__ wgt, uint64_t *FA, uint64_t *FB, int *QA, int *QB,
                                                         int *__restrict__ dist, int *__restrict__ part_n,
                                                         long long *__restrict__ part_e, int *__restrict__ part_x) {
     if (ctl->done || !ctl->sparse || ctl->need_full) return;
     const int cur = (ctl->it - 1) & 1;
-    uint64_t *C = cur ? FB : FA;
-    uint64_t *Cn = cur ? FA : FB;
-    const int *Q = cur ? QB : QA;
-    int *Qn = cur ? QA : QB;
-    int *qn_next = &ctl->qn[cur ^ 1];
-    const int n = ctl->qn[cur];
-    const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * kBB) >> 6;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    for (int i = wave; i < n; i += nw) {
-        const int u = Q[i];
-        const uint64_t cu = C[u];
-        const bool mine = (cu >> lane()) & 1ull;
-        const int du = mine ? dist[(long long)u * 64 + lane()] : (int)MG_INF;
-        const int p1 = row[u + 1];
-        for (int p = row[u]; p < p1; ++p) {
-            const int w = col[p];
-            const int nd = du + wgt[p];
-            bool imp = false;
-            if (mine) {
-                int *dw = &dist[(long long)w * 64 + lane()];
-                if (nd < *dw) imp = atomicMin(dw, nd) > nd;
-            }
-            const uint64_t m = ballot(imp);
-            if (lane() == 0 && m && atomicOr((unsigned long long *)&Cn[w], (unsigned long long)m) == 0ull) {
-                Qn[atomicAdd(qn_next, 1)] = w;
-                const int dg = row[w + 1] - row[w];
-                ++cnt;
-                mf += dg;
-                mx = max(mx, dg);
-            }
-        }
-        if (lane() == 0) C[u] = 0ull;
-    }
+    sparse_sssp(ctl->qn[cur], row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1], dist,
+                cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
 
+// Persistent sparse levels: one cooperative launch (every workgroup resident) runs
+// consecutive sparse levels with a grid barrier between them, so a long tail of
+// small frontiers (the thousands of levels of a large grid in an uneven batch) costs
+// a barrier per level instead of a control kernel, a level kernel and a graph node
+// each.  It starts after a sparse level of the captured graph, decides every level
+// with the control block's rule, and stops at the first level that is not sparse
+// (need_full: the captured graph's control kernel re-decides from the partials it
+// leaves), when the frontier is empty, or after max_levels levels.  Level L appends
+// to the counters of slot L % 3; slot (L + 1) % 3 is cleared during level L, which no
+// workgroup reads after the barrier that opened level L.
+struct PCtl {
+    int qn[3]; int mx[3]; long long mf[3];
+    unsigned top;            // barrier: groups of workgroups arrived (zeroed before the launch)
+    unsigned pad[15];
+    unsigned grp[8 * 16];    // barrier: arrivals per group, one 64 B line each
+};
+
+#ifndef MG_PERSIST_PER_CU
+#define MG_PERSIST_PER_CU 1  // resident workgroups per CU of the persistent launch
+#endif
+
+// Grid barrier of a cooperative launch (every workgroup resident), in two levels so
+// that no single address takes every arrival: workgroups count in on the counter of
+// their group (blockIdx mod 8, the XCD a round-robin dispatch puts them on); the last
+// of a group counts the group in on `top`, which every group leader polls.  Counters
+// only grow, so barrier g waits for g arrivals per group member and g * groups on
+// `top`.  Each thread makes its writes visible at device scope first, and the leader's
+// fence after the wait drops stale cache lines before the workgroup goes on.
+__device__ __forceinline__ void grid_barrier(PCtl *pc, unsigned &gen) {
+    ++gen;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned groups = min(8u, gridDim.x);
+        const unsigned g = blockIdx.x % groups;
+        const unsigned members = (gridDim.x - g + groups - 1) / groups;
+        const unsigned old = __hip_atomic_fetch_add(&pc->grp[g * 16], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == gen * members) __hip_atomic_fetch_add(&pc->top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (__hip_atomic_load(&pc->top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen * groups) __builtin_amdgcn_s_sleep(2);
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+struct PersistArgs {
+    BCtl *ctl;
+    PCtl *pc;
+    LevelRule rule;
+    const int *row, *col, *wgt;
+    uint64_t *FA, *FB, *vis;
+    int *QA, *QB;
+    int *out;  // lev (BFS) or dist (SSSP)
+    int *part_n;
+    long long *part_e;
+    int *part_x;
+    int nparts, max_levels;
+};
+
+template <bool kSSSP>
+__global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    unsigned gen = 0;
+    long long n, mf, mx;
+    {  // the frontier of the level before: per-workgroup partials of the captured graph
+        long long x = 0, y = 0, z = 0;
+        for (int i = threadIdx.x; i < a.nparts; i += kBB) { x += a.part_n[i]; y += a.part_e[i]; z = max(z, (long long)a.part_x[i]); }
+        n = block_sum<kBB / MG_WAVE>(x, s_tmp);
+        mf = block_sum<kBB / MG_WAVE>(y, s_tmp);
+        mx = block_max<kBB / MG_WAVE>(z, s_tmp);
+    }
+    const BCtl c0 = *a.ctl;
+    if (blockIdx.x == 0 && threadIdx.x < 3) { a.pc->qn[threadIdx.x] = 0; a.pc->mx[threadIdx.x] = 0; a.pc->mf[threadIdx.x] = 0; }
+    grid_barrier(a.pc, gen);
+    int L = c0.it;   // the last level run
+    int ran = 0;     // levels run by this launch
+    BCtl c = c0;
+    bool handback = false;
+    for (;;) {
+        c = c0;
+        c.it = L;
+        c.qvalid[L & 1] = 1;
+        c.prev_sparse = 1;
+        decide(c, n, mf, mx, a.rule);
+        if (c.done) break;
+        if (!c.sparse || c.compact || c.fill) { handback = true; break; }
+        if (ran >= a.max_levels) break;
+        const int it = L + 1, cur = L & 1, slot = it % 3;
+        int cnt = 0, xm = 0;
+        long long sm = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {  // the counters of the next level
+            const int ns = (it + 1) % 3;
+            a.pc->qn[ns] = 0;
+            a.pc->mx[ns] = 0;
+            a.pc->mf[ns] = 0;
+        }
+        if (kSSSP)
+            sparse_sssp((int)n, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA, cur ? a.QA : a.QB,
+                        &a.pc->qn[slot], a.out, cnt, sm, xm);
+        else
+            sparse_bfs(it, (int)n, a.row, a.col, cur ? a.FB : a.FA, cur ? a.FA : a.FB, a.vis, cur ? a.QB : a.QA,
+                       cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
+        (void)cnt;
+        const long long bs = block_sum<kBB / MG_WAVE>(sm, s_tmp);
+        const long long bx = block_max<kBB / MG_WAVE>((long long)xm, s_tmp);
+        if (threadIdx.x == 0) {
+            if (bs) atomicAdd((unsigned long long *)&a.pc->mf[slot], (unsigned long long)bs);
+            if (bx) atomicMax(&a.pc->mx[slot], (int)bx);
+        }
+        grid_barrier(a.pc, gen);
+        n = __hip_atomic_load(&a.pc->qn[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mf = (long long)__hip_atomic_load((unsigned long long *)&a.pc->mf[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mx = __hip_atomic_load(&a.pc->mx[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        L = it;
+        ++ran;
+    }
+    if (ran == 0) {  // nothing run: the state is the captured graph's
+        if (handback && blockIdx.x == 0 && threadIdx.x == 0) a.ctl->need_full = 1;
+        if (c.done && blockIdx.x == 0 && threadIdx.x == 0) { a.ctl->done = 1; a.ctl->n = 0; }
+        return;
+    }
+    // leave the state a sparse level of the captured graph leaves: the last level's
+    // frontier as partials (all in part 0) and its queue length in the control block
+    for (int i = blockIdx.x * kBB + threadIdx.x; i < a.nparts; i += gridDim.x * kBB) {
+        a.part_n[i] = i == 0 ? (int)n : 0;
+        a.part_e[i] = i == 0 ? mf : 0;
+        a.part_x[i] = i == 0 ? (int)mx : 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        BCtl o = c0;
+        o.it = L;
+        o.n = n;
+        o.mf = mf;
+        o.push = 1;
+        o.sparse = 1;
+        o.compact = 0;
+        o.fill = 0;
+        o.prev_sparse = 1;
+        o.qn[L & 1] = (int)n;
+        o.qvalid[L & 1] = 1;
+        o.done = c.done;
+        o.need_full = handback ? 1 : 0;
+        o.nsparse = c0.nsparse + ran;
+        o.npersist = c0.npersist + ran;
+        *a.ctl = o;
+    }
+}
 
 __global__ void k_sum_parts(const int *__restrict__ pn, const long long *__restrict__ pe, int n, long long *__restrict__ out) {
     __shared__ long long s_tmp[256 / MG_WAVE];
@@ -771,6 +987,9 @@ struct mg_batch {
     hipGraphExec_t gexec = nullptr;         // K captured levels (all level kinds)
     hipGraphExec_t gsparse = nullptr;       // K captured sparse levels (control + fused kernel)
     int gexec_levels = 0;
+    PCtl *pc = nullptr;           // counters of the persistent sparse levels
+    int persist_grid = 0;         // workgroups of its cooperative launch (0: not available)
+    long long stats[5] = {0, 0, 0, 0, 0};  // of the last BFS / SSSP run (mg_batch_stats)
 };
 
 static void batch_free(mg_batch *b) {
@@ -779,7 +998,7 @@ static void batch_free(mg_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
-                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl};
+                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -828,7 +1047,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
               hipMalloc(&b->part_e, sizeof(long long) * 4096) == hipSuccess &&
               hipMalloc(&b->tot, sizeof(long long) * 4) == hipSuccess &&
               hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess &&
-              hipMalloc(&b->ctl, sizeof(BCtl)) == hipSuccess &&
+              hipMalloc(&b->ctl, sizeof(BCtl)) == hipSuccess && hipMalloc(&b->pc, sizeof(PCtl)) == hipSuccess &&
               hipMalloc(&b->Q[0], sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->Q[1], sizeof(int) * (nv + 1)) == hipSuccess &&
               hipHostMalloc(&b->h_ctl, sizeof(BCtl), hipHostMallocDefault) == hipSuccess;
     if (ok && kind != MG_BATCH_SSSP) ok = hipMalloc(&b->lev, sizeof(int) * slots) == hipSuccess;
@@ -843,6 +1062,13 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
              hipMalloc(&b->logw, sizeof(uint64_t) * b->log_cap) == hipSuccess;
     }
     if (!ok) return fail(mg_fail(MG_ERR_NOMEM, "batch: device allocation failed"));
+    if (kind != MG_BATCH_BC && b->opt.persist_levels != 0) {  // one resident workgroup per CU
+        int dev = 0, cus = 0, per = 0;
+        const void *fn = kind == MG_BATCH_SSSP ? (const void *)k_mbs_persist<true> : (const void *)k_mbs_persist<false>;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kBB, 0) == hipSuccess && per > 0 && cus > 0)
+            b->persist_grid = cus * std::min(per, MG_PERSIST_PER_CU);
+    }
     if (hipMemcpy(b->row, g->row.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice) != hipSuccess ||
         (g->ne > 0 && hipMemcpy(b->col, g->col.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess) ||
         (b->w && g->ne > 0 && hipMemcpy(b->w, g->w.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess) ||
@@ -966,6 +1192,14 @@ static int run_bfs_bc(mg_batch *b) {
     return MG_OK;
 }
 
+// the level rule of BFS / SSSP runs (the control kernels' and the persistent kernel's)
+static LevelRule rule_of(const mg_batch *b) {
+    const float alpha = b->opt.alpha > 0 ? b->opt.alpha : 15.0f;
+    const float beta = b->opt.beta > 0 ? b->opt.beta : 18.0f;
+    const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;
+    return LevelRule{b->ne, b->nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0, chunk};
+}
+
 // BFS (without BC) and SSSP: levels run under the device control block, K at a time
 // from a captured hipGraph; the host reads the control block once per K levels
 static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *out) {
@@ -974,13 +1208,10 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
     const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
     const int gv = grid_for(nv, kBB, G);
     const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
-    const float alpha = b->opt.alpha > 0 ? b->opt.alpha : 15.0f;
     const bool sssp = b->kind == MG_BATCH_SSSP;
     hipGraph_t g = nullptr;
     MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const float beta = b->opt.beta > 0 ? b->opt.beta : 18.0f;
-    const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;
-    const LevelRule rule{b->ne, nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0, chunk};
+    const LevelRule rule = rule_of(b);
     const int gq = grid_for(nv, 2048, 2048);
     auto sparse_level = [&]() {
         if (sssp)
@@ -1056,15 +1287,33 @@ static int run_levels(mg_batch *b) {
                        b->part_e, b->part_x);
     MG_HIP(hipGetLastError());
     memset(b->h_ctl, 0, sizeof(BCtl));  // the first replay is always the full graph
+    const int max_levels = b->opt.persist_levels > 0 ? b->opt.persist_levels : (1 << 30);  // -1: unbounded
+    long long trips = 0;
     for (;;) {
-        // after a sparse level the sparse-only graph runs on until a level needs more
+        // after a sparse level the persistent kernel (or the sparse-only graph) runs on
+        // until a level needs more
         const bool sparse_next = b->h_ctl->it > 0 && b->h_ctl->sparse && !b->h_ctl->need_full;
-        MG_HIP(hipGraphLaunch(sparse_next ? b->gsparse : b->gexec, s));
+        if (sparse_next && b->persist_grid > 0) {
+            PersistArgs pa{b->ctl, b->pc, rule_of(b), b->row, b->col, b->w, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1],
+                           sssp ? b->dist : b->lev, b->part_n, b->part_e, b->part_x, gv, max_levels};
+            void *args[] = {&pa};
+            hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(256), 0, s, (int *)&b->pc->top, (long long)(16 + 8 * 16), 0);
+            const void *fn = sssp ? (const void *)k_mbs_persist<true> : (const void *)k_mbs_persist<false>;
+            MG_HIP(hipLaunchCooperativeKernel(fn, dim3(b->persist_grid), dim3(kBB), args, 0, s));
+        } else {
+            MG_HIP(hipGraphLaunch(sparse_next ? b->gsparse : b->gexec, s));
+        }
         MG_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s));
         MG_HIP(hipStreamSynchronize(s));
+        ++trips;
         if (b->h_ctl->done) break;
     }
     b->levels = b->h_ctl->it;
+    b->stats[0] = b->levels;
+    b->stats[1] = b->h_ctl->ndense;
+    b->stats[2] = b->h_ctl->nsparse;
+    b->stats[3] = b->h_ctl->npersist;
+    b->stats[4] = trips;
     return MG_OK;
 }
 
@@ -1156,6 +1405,13 @@ extern "C" int mg_batch_get(mg_batch *b, int which, void *dst, int64_t bytes) {
         return MG_OK;
     }
     return mg_fail(MG_ERR_ARG, "unknown batch result %d", which);
+}
+
+extern "C" int mg_batch_stats(const mg_batch *b, int64_t *out, int n) {
+    if (!b || !out) return mg_fail(MG_ERR_ARG, "null argument");
+    if (n < 0 || n > 5) return mg_fail(MG_ERR_ARG, "stats: 0..5 entries (asked %d)", n);
+    for (int i = 0; i < n; ++i) out[i] = b->stats[i];
+    return MG_OK;
 }
 
 extern "C" int mg_batch_levels(const mg_batch *b, int *levels) {

@@ -184,3 +184,48 @@ def test_batch_graph_replay(mg, orc, kind, graph_levels):
         if kind == "bfs":
             assert b.sweeps() == max(int(orc.bfs(row, col, int(s)).max()) for s in src) + 1
     b.close()
+
+
+@pytest.mark.parametrize("kind", ["bfs", "sssp"])
+@pytest.mark.parametrize("persist", [0, 1, 7, -1])
+def test_batch_persistent_levels(mg, orc, kind, persist):
+    """sparse levels run by one cooperative launch (at most `persist` levels per launch,
+    -1 = unbounded) or from the captured graphs only (0): every slot matches the oracle
+    and the level count does not depend on the path.  Members: a long grid tail, a thin
+    grid, and an RMAT graph whose hub forces chunked dense levels in between."""
+    parts = [mg.Graph.grid(120, 90), mg.Graph.rmat(11, 8, seed=5), mg.Graph.grid(7, 300)]
+    u = mg.Graph.union(parts)
+    row, col, w = u.csr()
+    rng = np.random.default_rng(19)
+    srcs, slots, off = [], [], 0
+    for p in parts:
+        srcs += (rng.integers(0, p.nv, 3) + off).tolist()
+        slots += [0, 1, 2]
+        off += p.nv
+    k = mg.BATCH_BFS if kind == "bfs" else mg.BATCH_SSSP
+    b = mg.Batch(u, k, persist_levels=persist)
+    sweeps = []
+    persistent = []
+    for rep in range(2):  # twice on one batch object
+        b.run(srcs, slots)
+        lv = b.levels()
+        for j in range(3):
+            ref = None
+            for s, sl in zip(srcs, slots):
+                if sl != j:
+                    continue
+                r = orc.bfs(row, col, s) if kind == "bfs" else orc.sssp_bf(row, col, w, s)[0]
+                ref = r if ref is None else np.minimum(ref, r)
+            np.testing.assert_array_equal(lv[j], ref, err_msg=f"run {rep} slot {j}")
+        sweeps.append(b.sweeps())
+        st = b.stats()
+        assert st["levels"] == sweeps[-1] == st["dense_levels"] + st["sparse_levels"]
+        persistent.append(st["persistent_levels"])
+    b.close()
+    assert sweeps[0] == sweeps[1]
+    if persist == 0:
+        assert persistent == [0, 0]
+    else:
+        assert min(persistent) > 0  # the grid tails ran in persistent launches
+    if kind == "bfs":
+        assert sweeps[0] == max(int(orc.bfs(row, col, s)[orc.bfs(row, col, s) != 0x64646464].max()) for s in srcs) + 1
