@@ -86,6 +86,10 @@ typedef struct mg_options {
                                cooperative launch, at most this many per launch (-1:
                                unbounded; 0, the default: every level from the captured
                                graphs, which measures faster on MI355X)                 */
+    int    pull_groups;     /* SSSP dense sweeps: split every row's edges into this many
+                               source-id ranges and sweep one range per launch, so the
+                               gathered distances stay in each die's L2 (0: sized to
+                               2 MiB of distances per range; 1: one plain sweep)         */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */
