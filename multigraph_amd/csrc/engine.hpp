@@ -51,6 +51,8 @@ struct Ctl {
     int lb_chunks;   // edge chunks of the current frontier
     int need_queue;  // 1: build the new frontier's queue from its bitmap (after a pull level)
     int qsrc;        // bitmap slot holding the frontier produced by the last pull level
+    int msg_iter;    // SSSP over the partitioned layout: the level whose input messages
+                     // the last pull sweep wrote (else they are rebuilt from distances)
     long long out_mf;  // degree sum of the appended vertices
     long long mf;      // degree sum of the current frontier
     long long mu;      // degree sum of unexplored vertices
@@ -135,6 +137,11 @@ struct mg_pb_layout {
     int4 *unitsB = nullptr;      // {partition, first edge slot, end, split}
     int *split_parts = nullptr;  // destination partitions processed by more than one unit
     int *apart = nullptr;        // np + 1: first source-major message slot of every partition
+    // SSSP sweeps over the layout
+    uint16_t *b_w = nullptr;     // per edge slot: its weight (destination-major, like b_dst)
+    int *inv = nullptr;          // internal id -> compacted id (engine ids)
+    int4 *unitsS = nullptr;      // every source partition {p, a0, a1, 0}: messages rebuilt from distances
+    int nS = 0;
 };
 
 struct mg_engine {
@@ -192,6 +199,7 @@ struct mg_engine {
     double *psum = nullptr;
     mg_pb_layout pb;
     bool pr_pb = false;               // PageRank over the partitioned layout
+    bool sssp_pb = false;             // SSSP pull sweeps over the partitioned layout
     long long pr_nvi = 0;             // internal (renumbered, padded) vertex count
     std::vector<int> pr_perm_h;       // compacted id -> internal id
     // host-side progress
@@ -231,6 +239,7 @@ int mg_pr_step(mg_engine *e, int *more);
 int mg_pr_capture(mg_engine *e, int iters, hipGraphExec_t *out);
 int mg_pb_reset(mg_engine *e);
 int mg_pb_build(mg_engine *e, const mg_graph *g);
+int mg_ps_sweep(mg_engine *e);
 int mg_pb_build_alloc(mg_engine *e, const mg_graph *g);
 int mg_pb_iteration(mg_engine *e);
 void mg_free_dev(void *p);

@@ -42,7 +42,7 @@ __global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int
     q[0] = src;
     Ctl c;
     c.iter = 1; c.mode = MODE_TD; c.in_begin = 0; c.in_n = 1; c.out_begin = out_begin; c.out_n = 0;
-    c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0;
+    c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0; c.msg_iter = -1;
     c.lb = deg_src > kSmallMF ? 1 : 0; c.lb_total = 0; c.lb_chunks = 0; c.need_queue = 0; c.qsrc = 0;
     c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0;
     *ctl = c;
@@ -923,7 +923,16 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         if (e->pnch > 0)
             hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(e->pnch + 1, 256, 4096)), dim3(256), 0, e->stream, nv, ne, e->row,
                                e->pnch, e->pcstart);
-        MG_TRY(build_source_groups(e, g));
+        // dense sweeps over the partitioned layout (symmetric graphs, weights in 16 bits),
+        // else over the CSR (pull_groups: source groups of it)
+        bool w16 = g->symmetric && e->opt.pull_groups == 0;
+        for (long long p = 0; w16 && p < ne; ++p) w16 = g->w[p] >= 0 && g->w[p] <= 0xFFFF;
+        if (w16 && nv >= 2) {
+            MG_TRY(mg_pb_build(e, g));
+            e->sssp_pb = true;
+        } else {
+            MG_TRY(build_source_groups(e, g));
+        }
     } else {
         const long long words = (nv + 63) / 64 + 1;
         MG_HIP(hipMalloc(&e->bm[0], sizeof(uint64_t) * words));
@@ -1073,6 +1082,9 @@ static int launch_level(mg_engine *e, bool known) {
         // one full relaxation pass: row offsets, col + weight + the source's distance per
         // edge, the vertex's distance write: 12E + 8V (+4)
         const double bytes = 12.0 * (double)e->ne + 4.0 * (nv + 1) + 4.0 * nv;
+        if (e->sssp_pb) {
+            MG_TRY(mg_ps_sweep(e));
+        } else {
         if (e->prof) mg_prof_begin(e, "k_pull_sssp", bytes, &slot);
         if (e->sg_n > 0) {
             for (int k = 0; k < e->sg_n; ++k) {
@@ -1086,6 +1098,7 @@ static int launch_level(mg_engine *e, bool known) {
                                e->col, e->w, e->pcstart, e->level, e->vfc);
         }
         if (e->prof) mg_prof_end(e, slot);
+        }
         hipLaunchKernelGGL(k_sssp_scan, dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, e->row, e->bm[0], e->part_n,
                            e->part_mf);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);

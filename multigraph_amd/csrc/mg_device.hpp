@@ -161,10 +161,15 @@ __device__ __forceinline__ float4 ld_stream_f4(const void *p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-// 16-byte streaming store (p must be 16-byte aligned)
+// 16-byte streaming stores (p must be 16-byte aligned)
 __device__ __forceinline__ void st_stream_f4(float *p, float a, float b, float c, float d) {
     const mg_f32x4 v = {a, b, c, d};
     __builtin_nontemporal_store(v, (mg_f32x4 *)p);
+}
+typedef int mg_i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_stream_f4(int *p, int a, int b, int c, int d) {
+    const mg_i32x4 v = {a, b, c, d};
+    __builtin_nontemporal_store(v, (mg_i32x4 *)p);
 }
 
 }  // namespace mg

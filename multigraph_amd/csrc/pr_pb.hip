@@ -179,6 +179,28 @@ __global__ __launch_bounds__(kBT) void k_pb_tile_sort(long long nt, const int *_
     }
 }
 
+// SSSP: the weight of every edge slot, in the destination-major order of b_dst.  A
+// tile's sorted keys give each slot's local source and destination; the weight is
+// found in the engine's CSR (rows sorted by column) through the inverse numbering.
+__global__ __launch_bounds__(kBT) void k_pb_weights(long long nt, int np, int P, const int *__restrict__ count,
+                                                    const long long *__restrict__ boff, const uint32_t *__restrict__ keys,
+                                                    const int *__restrict__ inv, const int *__restrict__ row,
+                                                    const int *__restrict__ col, const int *__restrict__ w,
+                                                    uint16_t *__restrict__ b_w) {
+    for (long long t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int n = count[t];
+        if (n == 0) continue;
+        const long long d = t / np, s = t % np, eb = boff[t];
+        for (int j = threadIdx.x; j < n; j += kBT) {
+            const uint32_t key = keys[eb + j];
+            const int src = inv[s * P + (int)(key >> 16)], dst = inv[d * P + (int)(key & 0xFFFFu)];
+            int lo = row[dst], hi = row[dst + 1] - 1;
+            while (lo < hi) { const int mid = (lo + hi) >> 1; if (col[mid] < src) lo = mid + 1; else hi = mid; }
+            b_w[eb + j] = (uint16_t)w[lo];
+        }
+    }
+}
+
 // pass 4: per tile, number the messages (a block scan of the new-source flags) and
 // write b_dst, gbase, a_src and amap
 __global__ __launch_bounds__(kBT) void k_pb_emit(long long nt, int np, const int *__restrict__ count,
@@ -229,9 +251,9 @@ __device__ __forceinline__ int part_count(int p, int nreal, int np) { return nre
 // thread and step: one 16-byte id load, two 16-byte streaming stores; padding writes
 // 0.  The next group's loads are issued before this group is written (two register
 // sets, no copies); loads past the range repeat its last group.
-template <int NT>
-__device__ __forceinline__ void scatter_part(int a0, int a1, const float *__restrict__ s_c, const uint16_t *__restrict__ a_src,
-                                             const int *__restrict__ amap, float *__restrict__ msg) {
+template <int NT, class T = float>
+__device__ __forceinline__ void scatter_part(int a0, int a1, const T *__restrict__ s_c, const uint16_t *__restrict__ a_src,
+                                             const int *__restrict__ amap, T *__restrict__ msg) {
     const int step = NT * kG;
     auto ld = [&](int i, uint4 &id, int &base) {
         const int c = min(i, a1 - kG);
@@ -241,12 +263,12 @@ __device__ __forceinline__ void scatter_part(int a0, int a1, const float *__rest
     auto emit = [&](int i, const uint4 &id, int base) {
         if (i >= a1) return;
         const uint32_t w[4] = {id.x, id.y, id.z, id.w};
-        float m[8];
+        T m[8];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t a = w[k] & 0xFFFFu, b = w[k] >> 16;
-            m[2 * k] = a != 0xFFFFu ? s_c[a] : 0.f;
-            m[2 * k + 1] = b != 0xFFFFu ? s_c[b] : 0.f;
+            m[2 * k] = a != 0xFFFFu ? s_c[a] : T(0);
+            m[2 * k + 1] = b != 0xFFFFu ? s_c[b] : T(0);
         }
         // streaming stores: the messages are read once, by the next gather
         st_stream_f4(&msg[base], m[0], m[1], m[2], m[3]);
@@ -522,6 +544,184 @@ __global__ void k_pb_zero_f64(double *p, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
+// ---- SSSP sweeps over the layout ---------------------------------------------------
+// The min-plus form of the gather: a message is its source's distance (one per
+// (source, destination partition) pair), every edge slot adds its own weight, and a
+// destination keeps the minimum in an LDS accumulator (native LDS atomicMin).  This is
+// one dense all-edge relaxation (process.cu:819-934) with no random HBM access per
+// edge: ids, weights and messages are streamed, distances are touched once per
+// vertex.  Distances stay in engine ids (dist[inv[internal id]]); a vertex whose
+// distance drops gets the level's stamp, as in every SSSP level.
+
+// messages from the current distances: every source partition before a pull sweep
+// that no pull sweep prepared (when = 0, into the buffer the sweep reads), or the
+// partitions whose destination side was split, after the gather (when = 1, into the
+// buffer the next sweep reads); unit = {partition, a0, a1, 0}
+__global__ __launch_bounds__(kBT) void k_ps_scatter(Ctl *ctl, int when, const int4 *__restrict__ units, int nunits, int P,
+                                                    int nv, int np, const int *__restrict__ inv, const int *__restrict__ dist,
+                                                    const uint16_t *__restrict__ a_src, const int *__restrict__ amap,
+                                                    int *__restrict__ msgbuf, long long npairs) {
+    if (ctl->mode != MODE_BU) return;
+    const int iter = ctl->iter;
+    if (when == 0 && ctl->msg_iter == iter) return;
+    int *out = msgbuf + (long long)((when == 0 ? iter : iter + 1) & 1) * npairs;
+    extern __shared__ int s_dist[];
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int4 un = units[u];
+        const int v0 = un.x * P;
+        const int pv = part_count(un.x, nv, np);
+        for (int k = threadIdx.x; k < pv; k += kBT) s_dist[k] = dist[inv[v0 + k]];
+        __syncthreads();
+        scatter_part<kBT, int>(un.y, un.z, s_dist, a_src, amap, out);
+        __syncthreads();
+    }
+}
+
+// a segment of the SSSP gather: the PageRank segment plus the lane's 8 slot weights
+struct SegW {
+    uint4 id, wt;
+    int base, n;
+    int m[kStageM];
+};
+
+__device__ __forceinline__ void load_msg_w(SegW &sg, const SegHdr &h, const uint4 &wt, const int *__restrict__ msg,
+                                           long long npairs) {
+    const long long q0 = h.ga;
+    const long long q1 = min(npairs, (long long)h.gb + 1);
+    sg.n = (int)(q1 - q0);
+    const uint32_t pm = h.pad ? 0x7FFF7FFFu : 0u;
+    sg.id = make_uint4(h.id.x | pm, h.id.y | pm, h.id.z | pm, h.id.w | pm);
+    sg.wt = wt;
+    sg.base = h.pad ? 0 : h.gl - h.ga;
+#pragma unroll
+    for (int k = 0; k < kStageM; ++k) sg.m[k] = ld_stream(&msg[q0 + min(lane() + k * MG_WAVE, sg.n - 1)]);
+}
+
+__global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
+                                                   const uint16_t *__restrict__ b_dst, const uint16_t *__restrict__ b_w,
+                                                   const int *__restrict__ gbase, int *__restrict__ msgbuf, long long npairs,
+                                                   const int *__restrict__ apart, const uint16_t *__restrict__ a_src,
+                                                   const int *__restrict__ amap, const int *__restrict__ inv,
+                                                   int *__restrict__ dist, int *__restrict__ vfc) {
+    if (ctl->mode != MODE_BU) return;
+    const int iter = ctl->iter;
+    const int *msg = msgbuf + (long long)(iter & 1) * npairs;
+    int *out = msgbuf + (long long)((iter + 1) & 1) * npairs;
+    extern __shared__ int s_min[];  // P + 1 minima, then kStageW staged messages per wave
+    const int wid = threadIdx.x / MG_WAVE;
+    int *s_msg = s_min + P + 1 + wid * kStageW;
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int4 un = units[u];
+        const int v0 = un.x * P;
+        const int pv = part_count(un.x, nv, np);
+        for (int k = threadIdx.x; k <= P; k += kBG) s_min[k] = MG_INF;
+        __syncthreads();
+        const int g0 = un.y / kG, g1 = un.z / kG;
+        const int nseg = (g1 - g0 + kSeg - 1) / kSeg;
+        constexpr int W = kBG / MG_WAVE;
+        auto hdr = [&](SegHdr &h, uint4 &wt, int si) {
+            si = min(si, nseg - 1);
+            const int gs = g0 + si * kSeg, ge = min(g1, g0 + (si + 1) * kSeg);
+            load_hdr(h, gs, ge, b_dst, gbase);
+            wt = ld_stream_u4(&b_w[(long long)min(gs + lane(), ge - 1) * kG]);
+        };
+        int hub = MG_INF;
+        auto process = [&](const SegW &sg) {
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < kStageM; ++k) s_msg[lane() + k * MG_WAVE] = sg.m[k];
+            wave_sync();
+            const uint32_t w[4] = {sg.id.x, sg.id.y, sg.id.z, sg.id.w};
+            const uint32_t ww[4] = {sg.wt.x, sg.wt.y, sg.wt.z, sg.wt.w};
+            int pi = sg.base;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t v = (k & 1) ? (w[k >> 1] >> 16) : (w[k >> 1] & 0xFFFFu);
+                const int wt = (int)((k & 1) ? (ww[k >> 1] >> 16) : (ww[k >> 1] & 0xFFFFu));
+                if (k > 0) pi += (int)(v >> 15);
+                const uint32_t dst = v & 0x7FFFu;
+                const int m = s_msg[pi];
+                // local id 0 is the partition's highest-degree vertex: lanes of one
+                // instruction updating one LDS address serialise, so its minimum stays in
+                // a register; padding (0x7FFF) and unreached sources relax nothing
+                if (dst == 0x7FFFu || m == (int)MG_INF) continue;
+                if (dst == 0) hub = min(hub, m + wt);
+                else atomicMin(&s_min[dst], m + wt);
+            }
+        };
+        int sgi = wid;
+        if (sgi < nseg) {
+            SegHdr h;
+            uint4 wt;
+            SegW a, b, c;
+            hdr(h, wt, sgi);
+            load_msg_w(a, h, wt, msg, npairs);
+            hdr(h, wt, sgi + W);
+            load_msg_w(b, h, wt, msg, npairs);
+            hdr(h, wt, sgi + 2 * W);
+            for (;;) {
+                load_msg_w(c, h, wt, msg, npairs);
+                hdr(h, wt, sgi + 3 * W);
+                process(a);
+                sgi += W;
+                if (sgi >= nseg) break;
+                load_msg_w(a, h, wt, msg, npairs);
+                hdr(h, wt, sgi + 3 * W);
+                process(b);
+                sgi += W;
+                if (sgi >= nseg) break;
+                load_msg_w(b, h, wt, msg, npairs);
+                hdr(h, wt, sgi + 3 * W);
+                process(c);
+                sgi += W;
+                if (sgi >= nseg) break;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) hub = min(hub, __shfl_xor(hub, o));
+        if (lane() == 0 && hub != (int)MG_INF) atomicMin(&s_min[0], hub);
+        __syncthreads();
+        if (un.w) {  // partition split over several units: each lowers what it found
+            for (int k = threadIdx.x; k < pv; k += kBG) {
+                const int best = s_min[k];
+                if (best == (int)MG_INF) continue;
+                const int v = inv[v0 + k];
+                if (best < dist[v] && atomicMin(&dist[v], best) > best) vfc[v] = iter;
+            }
+        } else {
+            // apply, then write the partition's messages for the next sweep: its
+            // distances are staged in LDS over the consumed minima
+            int cl[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int k = threadIdx.x + j * kBG;
+                cl[j] = MG_INF;
+                if (k < pv) {
+                    const int v = inv[v0 + k];
+                    const int best = s_min[k];
+                    int d = dist[v];
+                    if (best < d) {
+                        dist[v] = best;
+                        vfc[v] = iter;
+                        d = best;
+                    }
+                    cl[j] = d;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int k = threadIdx.x + j * kBG;
+                if (k < pv) s_min[k] = cl[j];
+            }
+            __syncthreads();
+            scatter_part<kBG, int>(apart[un.x], apart[un.x + 1], s_min, a_src, amap, out);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->msg_iter = iter + 1;
+}
+
 // internal CSR under the degree round-robin numbering: degree of every internal id
 __global__ void k_pb_ideg(int nv, const int *__restrict__ row, const int *__restrict__ perm, int *__restrict__ ideg) {
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += gridDim.x * blockDim.x) ideg[perm[x]] = row[x + 1] - row[x];
@@ -620,6 +820,12 @@ int mg_pb_build(mg_engine *e, const mg_graph *g) {
     long long *irow64 = nullptr;
     MG_HIP(hipMalloc(&perm, sizeof(int) * std::max(1, nv)));
     MG_HIP(hipMemcpy(perm, e->pr_perm_h.data(), sizeof(int) * nv, hipMemcpyHostToDevice));
+    if (e->algo == MG_SSSP) {  // SSSP keeps distances in engine ids: internal id -> compacted id
+        std::vector<int> inv((size_t)nvi, 0);
+        for (int v = 0; v < nv; ++v) inv[e->pr_perm_h[v]] = v;
+        MG_HIP(hipMalloc(&L.inv, sizeof(int) * std::max<long long>(1, nvi)));
+        MG_HIP(hipMemcpy(L.inv, inv.data(), sizeof(int) * nvi, hipMemcpyHostToDevice));
+    }
     MG_HIP(hipMalloc(&e->deg, sizeof(int) * (nvi + 64)));
     MG_HIP(hipMemsetAsync(e->deg, 0, sizeof(int) * (nvi + 64), s));
     // out-degrees (the divisor of every message) and internal row lengths
@@ -742,6 +948,13 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
         }
         if (fixed) MG_HIP(hipMemcpy(npair, hn.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
     }
+    if (e->algo == MG_SSSP) {  // per edge slot weights, from the sorted keys (padding: 0)
+        if (hipMalloc(&L.b_w, sizeof(uint16_t) * sl) != hipSuccess) { cleanup(); return mg_fail(MG_ERR_NOMEM, "partitioned layout: weights"); }
+        hipLaunchKernelGGL(k_fill_u16, dim3(grid_for(sl, 256, 8192)), dim3(256), 0, s, L.b_w, sl, (uint16_t)0);
+        hipLaunchKernelGGL(k_pb_weights, dim3(grid_for(nt, 1, 8192)), dim3(kBT), 0, s, nt, np, P, count, boff, keys, L.inv, e->row,
+                           e->col, e->w, L.b_w);
+        MG_HIP(hipGetLastError());
+    }
     // messages per tile -> padded message slots in both orders -> offsets
     hipLaunchKernelGGL(k_pb_pad_pairs, dim3(grid_for(nt, 256, 8192)), dim3(256), 0, s, npair, np, pb, pa);
     rc = mg_scan64(s, pb, nullptr, nt, poffB);
@@ -794,6 +1007,17 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     L.nA = (int)ua.size();
     L.nB = (int)ub.size();
     L.nApply = (int)split.size();
+    if (e->algo == MG_SSSP) {  // every source partition, in units: messages rebuilt from distances
+        std::vector<int4> us;
+        for (int p = 0; p < np; ++p)
+            for (long long x = apart[p]; x < apart[p + 1]; x += unit) us.push_back(make_int4(p, (int)x, (int)std::min<long long>(apart[p + 1], x + unit), 0));
+        L.nS = (int)us.size();
+        MG_HIP(hipMalloc(&L.unitsS, sizeof(int4) * std::max<size_t>(1, us.size())));
+        if (!us.empty()) MG_HIP(hipMemcpy(L.unitsS, us.data(), sizeof(int4) * us.size(), hipMemcpyHostToDevice));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(int) * P)));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(int) * (P + 1) + sizeof(int) * kStage)));
+    }
     MG_HIP(hipMalloc(&L.unitsA, sizeof(int4) * std::max<size_t>(1, ua.size())));
     MG_HIP(hipMalloc(&L.unitsB, sizeof(int4) * std::max<size_t>(1, ub.size())));
     MG_HIP(hipMalloc(&L.split_parts, sizeof(int) * std::max<size_t>(1, split.size())));
@@ -830,6 +1054,34 @@ int mg_pb_iteration(mg_engine *e) {
         hipLaunchKernelGGL(k_pb_scatter_msg, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(float) * P, s, e->ctl, L.unitsA,
                            L.nA, L.P, e->nv, L.np, e->rank, e->deg, L.a_src, L.amap, L.msg, L.pairs);
     }
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+// one SSSP pull sweep over the layout, gated on the control block (a push level runs
+// none of it): messages from distances unless the last sweep wrote them, the min-plus
+// gather (unsplit partitions also write their next messages), then the messages of
+// split partitions
+int mg_ps_sweep(mg_engine *e) {
+    mg_pb_layout &L = e->pb;
+    hipStream_t s = e->stream;
+    const int P = L.P;
+    const int G = e->opt.grid_blocks > 0 ? e->opt.grid_blocks : 2048;
+    int *msg = (int *)L.msg;
+    if (L.nS > 0)
+        hipLaunchKernelGGL(k_ps_scatter, dim3(grid_for(L.nS, 1, G)), dim3(kBT), sizeof(int) * P, s, e->ctl, 0, L.unitsS, L.nS, P, e->nv,
+                           L.np, L.inv, e->level, L.a_src, L.amap, msg, L.pairs);
+    int slot = -1;
+    // one full relaxation pass: 12E + 8V (+4)
+    const double alg = 12.0 * (double)e->ne + 8.0 * e->nv + 4.0;
+    if (e->prof) mg_prof_begin(e, "k_ps_gather", alg, &slot);
+    hipLaunchKernelGGL(k_ps_gather, dim3(grid_for(L.nB, 1, G)), dim3(kBG), sizeof(int) * (P + 1) + sizeof(int) * kStage, s, e->ctl,
+                       L.unitsB, L.nB, P, e->nv, L.np, L.b_dst, L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv,
+                       e->level, e->vfc);
+    if (e->prof) mg_prof_end(e, slot);
+    if (L.nA > 0)
+        hipLaunchKernelGGL(k_ps_scatter, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(int) * P, s, e->ctl, 1, L.unitsA, L.nA, P, e->nv,
+                           L.np, L.inv, e->level, L.a_src, L.amap, msg, L.pairs);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }
