@@ -86,10 +86,9 @@ typedef struct mg_options {
                                cooperative launch, at most this many per launch (-1:
                                unbounded; 0, the default: every level from the captured
                                graphs, which measures faster on MI355X)                 */
-    int    pull_groups;     /* SSSP dense sweeps: split every row's edges into this many
-                               source-id ranges and sweep one range per launch, so the
-                               gathered distances stay in each die's L2 (0: sized to
-                               2 MiB of distances per range; 1: one plain sweep)         */
+    int    pull_layout;     /* SSSP dense sweeps: 0, the default, over the partitioned
+                               layout (distance messages, 16-bit weights; symmetric graphs
+                               with weights in [0, 65535]), 1 over the CSR               */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */

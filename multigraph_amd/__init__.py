@@ -50,7 +50,7 @@ class Options(C.Structure):
         ("part_log2", C.c_int),
         ("batch_chunk", C.c_int),
         ("persist_levels", C.c_int),
-        ("pull_groups", C.c_int),
+        ("pull_layout", C.c_int),
     ]
 
 

@@ -174,13 +174,6 @@ struct mg_engine {
     int *cstart = nullptr;     // first frontier entry of every edge chunk
     int *pcstart = nullptr;    // first row of every chunk of the whole CSR (SSSP pull)
     int pnch = 0;
-    // SSSP pull sweeps over source groups: the edges of every row split by source-id
-    // range (group g = sources [g*sg_size, (g+1)*sg_size)), concatenated group by group;
-    // rows are the destinations with edges in the group (sg_rptr offsets, sg_rvid ids)
-    int sg_n = 0, sg_size = 0;
-    int *sg_rptr = nullptr, *sg_rvid = nullptr, *sg_col = nullptr, *sg_w = nullptr;
-    int4 *sg_ch = nullptr;          // kEPB-edge chunks {e0 (8-aligned), e1, first row, last row}
-    std::vector<int> sg_choff;      // first chunk of every group, and the total
     int *part_n = nullptr;     // per-workgroup partial counts of a pull level
     long long *part_mf = nullptr;
     int *lvl_off = nullptr;    // BC level log offsets (levels + 2)

@@ -591,78 +591,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
     }
 }
 
-// The same sweep over one source group (launched once per group, in turn): its
-// edges all read the distances of one id range, which stays resident in the L2 of
-// every die while the edge stream passes, instead of one random line fetch beyond L2
-// per edge.  Rows are the destinations with edges in the group, never empty, so a
-// chunk spans at most kEPB + 1 of them; a destination is one row of the group, so
-// within a launch only the chunk owning a row writes it (straddling rows: atomicMin).
-// Chunk starts are 8-aligned for the 16-byte loads; edges below the group's first
-// edge are masked.
-__global__ __launch_bounds__(kBlock) void k_pull_sssp_g(Ctl *ctl, int c0, int c1, const int4 *__restrict__ ch,
-                                                        const int *__restrict__ rptr, const int *__restrict__ rvid,
-                                                        const int *__restrict__ col, const int *__restrict__ wgt,
-                                                        int *__restrict__ dist, int *__restrict__ vfc) {
-    if (ctl->mode != MODE_BU) return;
-    const int iter = ctl->iter;
-    __shared__ int s_row[kEPB + 2];
-    __shared__ int s_best[kEPB + 1];
-    for (int c = c0 + blockIdx.x; c < c1; c += gridDim.x) {
-        const int4 k4 = ch[c];
-        const int e0 = k4.x, e1 = k4.y, r0 = k4.z;
-        const int nown = k4.w - r0 + 1;
-        const int elo = max(e0, rptr[r0]);
-        const int eb = e0 + threadIdx.x * kPE;
-        int cu[kPE], wv[kPE], du[kPE];
-        if (eb + kPE <= e1) {
-            const uint4 a0 = ld_stream_u4(&col[eb]), a1 = ld_stream_u4(&col[eb + 4]);
-            const uint4 b0 = ld_stream_u4(&wgt[eb]), b1 = ld_stream_u4(&wgt[eb + 4]);
-            cu[0] = a0.x; cu[1] = a0.y; cu[2] = a0.z; cu[3] = a0.w; cu[4] = a1.x; cu[5] = a1.y; cu[6] = a1.z; cu[7] = a1.w;
-            wv[0] = b0.x; wv[1] = b0.y; wv[2] = b0.z; wv[3] = b0.w; wv[4] = b1.x; wv[5] = b1.y; wv[6] = b1.z; wv[7] = b1.w;
-        } else {
-#pragma unroll
-            for (int j = 0; j < kPE; ++j) {
-                cu[j] = eb + j < e1 ? col[eb + j] : 0;
-                wv[j] = eb + j < e1 ? wgt[eb + j] : 0;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kPE; ++j) du[j] = (eb + j >= elo && eb + j < e1) ? dist[cu[j]] : (int)MG_INF;
-        for (int k = threadIdx.x; k <= nown; k += kBlock) s_row[k] = rptr[r0 + k];
-        for (int k = threadIdx.x; k < nown; k += kBlock) s_best[k] = MG_INF;
-        __syncthreads();
-        const int f = max(eb, elo);
-        if (f < min(eb + kPE, e1)) {
-            int lo = 0, hi = nown - 1;  // row of edge f: the last k with s_row[k] <= f
-            while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_row[mid] <= f) lo = mid; else hi = mid - 1; }
-            int k = lo, next = s_row[lo + 1], best = MG_INF;
-#pragma unroll
-            for (int j = 0; j < kPE; ++j) {
-                const int e = eb + j;
-                if (e >= f && e < e1) {
-                    while (e >= next) { sssp_flush(s_best, k, best); best = MG_INF; ++k; next = s_row[k + 1]; }
-                    if (du[j] != (int)MG_INF) best = min(best, du[j] + wv[j]);
-                }
-            }
-            sssp_flush(s_best, k, best);
-        }
-        __syncthreads();
-        for (int k = threadIdx.x; k < nown; k += kBlock) {
-            const int best = s_best[k];
-            if (best == (int)MG_INF) continue;
-            const int o = rvid[r0 + k];
-            if (best >= dist[o]) continue;
-            if (s_row[k] < e0 || s_row[k + 1] > e1) {
-                if (atomicMin(&dist[o], best) > best) vfc[o] = iter;
-            } else {
-                dist[o] = best;
-                vfc[o] = iter;
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // Frontier of stamp `iter + offset` from the stamps: a bitmap word per wave
 // (ballot) and per-workgroup partial counts (size, degree sum) for k_reduce_parts.
 // gate 0: always; 1: only on a pull level; 2: only right after a switch to pull
@@ -807,96 +735,6 @@ int max_grid(const mg_engine *e) { return e->opt.grid_blocks > 0 ? e->opt.grid_b
 
 // ---------------------------------------------------------------------------- host side
 
-// Source groups of the SSSP pull sweeps (host build, once): every row's edges are
-// sorted by source, so the edges of one source range are a contiguous piece of it.
-// Layout: group k holds its rows (the destinations with an edge from the range) at
-// row indices [rb[k], rb[k] + nr[k]) and one sentinel row after them, so that
-// rptr[r + 1] is the end of row r for every row; its edges start at an 8-aligned
-// offset eb[k].
-static int build_source_groups(mg_engine *e, const mg_graph *g) {
-    const int nv = e->nv;
-    const long long ne = e->ne;
-    int G = e->opt.pull_groups;
-    if (G <= 0) G = (int)std::min<long long>(64, ((long long)nv * 4 + (2 << 20) - 1) / (2 << 20));  // 2 MiB of distances each
-    G = std::min(G, std::max(1, nv));
-    if (G < 2 || ne < 1 || ne + 8LL * G + kEPB >= (1LL << 31) - 1) return MG_OK;
-    const int gsz = (nv + G - 1) / G;
-    const int *row = g->row.data(), *col = g->col.data(), *wt = g->w.data();
-    // pos[v * (G + 1) + k]: first edge of row v with source >= k * gsz (k = G: row end)
-    std::vector<int> pos((size_t)nv * (G + 1));
-    std::atomic<int> unsorted{0};
-    mg_parallel_for(nv, [&](int64_t b, int64_t en) {
-        for (int64_t v = b; v < en; ++v) {
-            const int p0 = row[v], p1 = row[v + 1];
-            for (int p = p0 + 1; p < p1; ++p) if (col[p - 1] > col[p]) { unsorted = 1; break; }
-            int *pv = &pos[(size_t)v * (G + 1)];
-            for (int k = 0; k < G; ++k) pv[k] = (int)(std::lower_bound(col + p0, col + p1, (int)std::min<long long>((long long)k * gsz, nv)) - col);
-            pv[G] = p1;
-        }
-    });
-    if (unsorted) return MG_OK;  // plain sweeps
-    std::vector<long long> nr(G, 0), nedge(G, 0);
-    mg_parallel_for(G, [&](int64_t b, int64_t en) {
-        for (int64_t k = b; k < en; ++k)
-            for (int v = 0; v < nv; ++v) {
-                const int d = pos[(size_t)v * (G + 1) + k + 1] - pos[(size_t)v * (G + 1) + k];
-                nr[k] += d > 0;
-                nedge[k] += d;
-            }
-    }, 1);
-    std::vector<long long> rb(G + 1, 0), eb(G + 1, 0);
-    for (int k = 0; k < G; ++k) { rb[k + 1] = rb[k] + nr[k] + 1; eb[k + 1] = ((eb[k] + nedge[k] + 7) / 8) * 8; }
-    const long long R = rb[G], S = eb[G];
-    std::vector<int> rptr(R, 0), rvid(R, 0), gcol(S, 0), gw(S, 0);
-    std::vector<std::vector<int4>> gch(G);
-    mg_parallel_for(G, [&](int64_t b, int64_t en) {
-        for (int64_t k = b; k < en; ++k) {
-            long long r = rb[k], p = eb[k];
-            for (int v = 0; v < nv; ++v) {
-                const int a = pos[(size_t)v * (G + 1) + k], z = pos[(size_t)v * (G + 1) + k + 1];
-                if (z == a) continue;
-                rptr[r] = (int)p;
-                rvid[r] = v;
-                std::copy(col + a, col + z, gcol.begin() + p);
-                std::copy(wt + a, wt + z, gw.begin() + p);
-                ++r;
-                p += z - a;
-            }
-            rptr[r] = (int)p;  // sentinel: the group's end
-            // kEPB-edge chunks with 8-aligned starts; rows of a chunk's first and last edge
-            const long long gs = eb[k], ge = eb[k] + nedge[k], rlast = rb[k] + nr[k] - 1;
-            long long r0 = rb[k];
-            for (long long c0 = gs; c0 < ge; c0 += kEPB) {
-                const long long c1 = std::min(ge, c0 + kEPB);
-                while (r0 < rlast && rptr[r0 + 1] <= c0) ++r0;
-                long long r1 = r0;
-                while (r1 < rlast && rptr[r1 + 1] <= c1 - 1) ++r1;
-                gch[k].push_back(make_int4((int)c0, (int)c1, (int)r0, (int)r1));
-            }
-        }
-    }, 1);
-    std::vector<int4> ch;
-    e->sg_choff.assign(G + 1, 0);
-    for (int k = 0; k < G; ++k) {
-        e->sg_choff[k] = (int)ch.size();
-        ch.insert(ch.end(), gch[k].begin(), gch[k].end());
-    }
-    e->sg_choff[G] = (int)ch.size();
-    MG_HIP(hipMalloc(&e->sg_rptr, sizeof(int) * R));
-    MG_HIP(hipMalloc(&e->sg_rvid, sizeof(int) * R));
-    MG_HIP(hipMalloc(&e->sg_col, sizeof(int) * (S + 8)));
-    MG_HIP(hipMalloc(&e->sg_w, sizeof(int) * (S + 8)));
-    MG_HIP(hipMalloc(&e->sg_ch, sizeof(int4) * std::max<size_t>(1, ch.size())));
-    MG_HIP(hipMemcpy(e->sg_rptr, rptr.data(), sizeof(int) * R, hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(e->sg_rvid, rvid.data(), sizeof(int) * R, hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(e->sg_col, gcol.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(e->sg_w, gw.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-    if (!ch.empty()) MG_HIP(hipMemcpy(e->sg_ch, ch.data(), sizeof(int4) * ch.size(), hipMemcpyHostToDevice));
-    e->sg_n = G;
-    e->sg_size = gsz;
-    return MG_OK;
-}
-
 int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     const int nv = e->nv;
     const long long ne = e->ne;
@@ -924,14 +762,12 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
             hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(e->pnch + 1, 256, 4096)), dim3(256), 0, e->stream, nv, ne, e->row,
                                e->pnch, e->pcstart);
         // dense sweeps over the partitioned layout (symmetric graphs, weights in 16 bits),
-        // else over the CSR (pull_groups: source groups of it)
-        bool w16 = g->symmetric && e->opt.pull_groups == 0;
+        // else over the CSR
+        bool w16 = g->symmetric && e->opt.pull_layout == 0 && nv >= 2;
         for (long long p = 0; w16 && p < ne; ++p) w16 = g->w[p] >= 0 && g->w[p] <= 0xFFFF;
-        if (w16 && nv >= 2) {
+        if (w16) {
             MG_TRY(mg_pb_build(e, g));
             e->sssp_pb = true;
-        } else {
-            MG_TRY(build_source_groups(e, g));
         }
     } else {
         const long long words = (nv + 63) / 64 + 1;
@@ -1086,17 +922,8 @@ static int launch_level(mg_engine *e, bool known) {
             MG_TRY(mg_ps_sweep(e));
         } else {
         if (e->prof) mg_prof_begin(e, "k_pull_sssp", bytes, &slot);
-        if (e->sg_n > 0) {
-            for (int k = 0; k < e->sg_n; ++k) {
-                const int c0 = e->sg_choff[k], c1 = e->sg_choff[k + 1];
-                if (c1 > c0)
-                    hipLaunchKernelGGL(k_pull_sssp_g, dim3(grid_for(c1 - c0, 1, G)), dim3(kBlock), 0, s, e->ctl, c0, c1, e->sg_ch,
-                                       e->sg_rptr, e->sg_rvid, e->sg_col, e->sg_w, e->level, e->vfc);
-            }
-        } else {
-            hipLaunchKernelGGL(k_pull_sssp, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->row,
-                               e->col, e->w, e->pcstart, e->level, e->vfc);
-        }
+        hipLaunchKernelGGL(k_pull_sssp, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->row,
+                           e->col, e->w, e->pcstart, e->level, e->vfc);
         if (e->prof) mg_prof_end(e, slot);
         }
         hipLaunchKernelGGL(k_sssp_scan, dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, e->row, e->bm[0], e->part_n,

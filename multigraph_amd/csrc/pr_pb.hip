@@ -554,9 +554,8 @@ __global__ void k_pb_zero_f64(double *p, long long n) {
 // distance drops gets the level's stamp, as in every SSSP level.
 
 // messages from the current distances: every source partition before a pull sweep
-// that no pull sweep prepared (when = 0, into the buffer the sweep reads), or the
-// partitions whose destination side was split, after the gather (when = 1, into the
-// buffer the next sweep reads); unit = {partition, a0, a1, 0}
+// that no pull sweep prepared (when = 0), or the partitions whose destination side was
+// split, after the gather (when = 1); unit = {partition, a0, a1, 0}
 __global__ __launch_bounds__(kBT) void k_ps_scatter(Ctl *ctl, int when, const int4 *__restrict__ units, int nunits, int P,
                                                     int nv, int np, const int *__restrict__ inv, const int *__restrict__ dist,
                                                     const uint16_t *__restrict__ a_src, const int *__restrict__ amap,
@@ -564,7 +563,7 @@ __global__ __launch_bounds__(kBT) void k_ps_scatter(Ctl *ctl, int when, const in
     if (ctl->mode != MODE_BU) return;
     const int iter = ctl->iter;
     if (when == 0 && ctl->msg_iter == iter) return;
-    int *out = msgbuf + (long long)((when == 0 ? iter : iter + 1) & 1) * npairs;
+    int *out = msgbuf;
     extern __shared__ int s_dist[];
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int4 un = units[u];
@@ -605,8 +604,12 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
                                                    int *__restrict__ dist, int *__restrict__ vfc) {
     if (ctl->mode != MODE_BU) return;
     const int iter = ctl->iter;
-    const int *msg = msgbuf + (long long)(iter & 1) * npairs;
-    int *out = msgbuf + (long long)((iter + 1) & 1) * npairs;
+    // One message buffer: a partition's new distances are written over the messages
+    // this sweep reads, so partitions gathered later already relax from them.  Any mix
+    // of old and new messages is an upper bound of the true distances, so the result
+    // is exact whatever the order; only the number of sweeps depends on it.
+    const int *msg = msgbuf;
+    int *out = msgbuf;
     extern __shared__ int s_min[];  // P + 1 minima, then kStageW staged messages per wave
     const int wid = threadIdx.x / MG_WAVE;
     int *s_msg = s_min + P + 1 + wid * kStageW;

@@ -6,9 +6,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 MODES = [dict(), dict(mode=1), dict(mode=2), dict(graph_levels=3), dict(mode=2, graph_levels=2), dict(alpha=1000.0),
-         dict(pull_groups=1), dict(pull_groups=3), dict(mode=2, pull_groups=7), dict(mode=2, pull_groups=64, graph_levels=4)]
-IDS = ["hybrid", "push", "pull", "graph", "pull-graph", "eager-pull-switch", "one-group", "groups3", "pull-groups7",
-       "pull-groups64-graph"]
+         dict(pull_layout=1), dict(mode=2, pull_layout=1), dict(mode=2, pull_layout=1, graph_levels=4), dict(part_log2=6),
+         dict(mode=2, part_log2=5)]
+IDS = ["hybrid", "push", "pull", "graph", "pull-graph", "eager-pull-switch", "csr", "csr-pull", "csr-pull-graph",
+       "small-parts", "pull-tiny-parts"]
 
 
 @pytest.mark.parametrize("opts", MODES, ids=IDS)
@@ -27,17 +28,38 @@ def test_sssp_modes(mg, orc, opts, gname):
     e.close()
 
 
-@pytest.mark.parametrize("groups", [2, 5, 13])
-def test_sssp_source_groups_union(mg, orc, groups):
-    """pull sweeps over source groups on a disjoint union (some groups hold rows of
-    one member only, group edge ranges start 8-aligned, a chunk can hold one row's
-    edges from several chunks' worth of a hub): distances bit-exact"""
+@pytest.mark.parametrize("opts", [dict(mode=2), dict(mode=2, part_log2=7), dict(mode=2, pull_layout=1), dict()],
+                         ids=["layout", "layout-p128", "csr", "hybrid"])
+def test_sssp_pull_union(mg, orc, opts):
+    """dense sweeps on a disjoint union (hub rows split over several gather units at
+    small partitions, members with and without edges in a partition): distances
+    bit-exact, through the partitioned layout and the CSR"""
     parts = [mg.Graph.rmat(12, 16, seed=41), mg.Graph.grid(33, 29), mg.Graph.rmat(10, 4, seed=42)]
     u = mg.Graph.union(parts)
     row, col, w = u.csr()
-    e = mg.Engine(u, mg.SSSP, mode=2, pull_groups=groups)
+    e = mg.Engine(u, mg.SSSP, **opts)
     for s in (0, parts[0].nv + 5, u.nv - 1):
         e.run(s)
         ref, _ = orc.sssp_bf(row, col, w, s)
-        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref, err_msg=f"groups {groups} source {s}")
+        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref, err_msg=f"{opts} source {s}")
+    e.close()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_sssp_layout_split_partition(mg, orc, mode):
+    """a hub with 50 K neighbours: its destination partition's edge stream is cut over
+    several gather units (partial minima lowered with atomics, messages rebuilt by a
+    separate pass); distances bit-exact from the hub, a leaf and a far vertex"""
+    rng = np.random.default_rng(77)
+    n = 60000
+    src = [0] * 50000 + rng.integers(1, n, 40000).tolist()
+    dst = list(range(1, 50001)) + rng.integers(1, n, 40000).tolist()
+    val = rng.integers(1, 65, len(src)).tolist()
+    g = mg.Graph.from_coo(n, np.array(src), np.array(dst), np.array(val))
+    row, col, w = g.csr()
+    e = mg.Engine(g, mg.SSSP, mode=mode)
+    for s in (0, 17, g.nv - 1):
+        e.run(s)
+        ref, _ = orc.sssp_bf(row, col, w, s)
+        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref, err_msg=f"mode {mode} source {s}")
     e.close()
