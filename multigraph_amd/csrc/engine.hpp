@@ -75,6 +75,7 @@ struct HugeChunk { int v, j; };
 // pull chunk of a high-degree vertex: edges [p0, p1) of v, k = v's slot in the huge list
 struct PullChunk { int v, p0, p1, k; };
 constexpr int kPullHuge = 2048;   // pull levels split vertices above this degree
+constexpr int kPullLane = 32;     // BC pull: rows up to this many edges are walked by one lane
 constexpr int kPullChunk = 2048;  // edges per pull chunk
 
 struct KStat {
@@ -159,6 +160,8 @@ struct mg_engine {
     double *sigma = nullptr;   // BC path counts
     float *delta = nullptr;    // BC dependencies
     double *dacc = nullptr;    // BC split-vertex partial sums
+    uint8_t *lev8 = nullptr;   // BC backward: levels as bytes
+    double *bc_cw = nullptr;   // BC backward: (1 + delta[w]) / sigma[w] of finished vertices
     int *q = nullptr;          // frontier queue / level log
     long long qcap = 0;
     uint64_t *bm[2] = {nullptr, nullptr};

@@ -337,13 +337,40 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
             b = row[v];
             e = row[v + 1];
             if (e - b > kPullHuge) e = b;  // left to k_pull_huge
-            for (int p = b; p < e; ++p) {
-                const int u = ld_stream(&col[p]);
-                if (bm_test(fr, u)) {
-                    found = true;
-                    if (!kBC) break;
-                    s += sigma[u];
+            if (!kBC) {
+                for (int p = b; p < e; ++p)
+                    if (bm_test(fr, ld_stream(&col[p]))) { found = true; break; }
+            }
+        }
+        if (kBC) {
+            // BC scans every parent: lanes walk rows of at most kPullLane edges alone,
+            // stepping together for the longest of them; longer rows are walked by the
+            // whole wave, one at a time (lanes over edges, a wave sum).  Path counts
+            // are integers, exact in FP64 whatever the order of the additions.
+            const bool alone = e - b <= kPullLane;
+            int steps = alone ? e - b : 0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
+            for (int k = 0; k < steps; ++k) {
+                if (b + k < e && alone) {
+                    const int u = ld_stream(&col[b + k]);
+                    if (bm_test(fr, u)) { found = true; s += sigma[u]; }
                 }
+            }
+            uint64_t heavy = ballot(!alone);
+            while (heavy) {
+                const int l = __builtin_ctzll(heavy);
+                heavy &= heavy - 1;
+                const int q0 = __shfl(b, l), q1 = __shfl(e, l);
+                double hs = 0.0;
+                bool hf = false;
+                for (int p = q0 + lane(); p < q1; p += MG_WAVE) {
+                    const int u = ld_stream(&col[p]);
+                    if (bm_test(fr, u)) { hf = true; hs += sigma[u]; }
+                }
+                hs = wave_sum(hs);
+                const bool any = ballot(hf) != 0ull;
+                if (lane() == l) { found = any; s = hs; }
             }
         }
         const uint64_t m = ballot(found);
@@ -619,13 +646,29 @@ __global__ __launch_bounds__(kBlock) void k_sssp_scan(Ctl *ctl, int nv, int offs
 }
 
 // -------------------------------------------------------------------- BC backward
+// Levels as bytes for the backward pass (255 = level 255 or deeper, or unreached):
+// every edge of the pass probes its neighbour's level, and the byte array (nv bytes)
+// stays in each die's L2 where the int array would not.
+__global__ void k_lev8(int nv, const int *__restrict__ level, uint8_t *__restrict__ lev8) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+        const int l = level[v];
+        lev8[v] = (uint8_t)(l >= 0 && l < 255 ? l : 255);
+    }
+}
+// is w one level deeper than the level being processed (l + 1)?
+__device__ __forceinline__ bool bc_next(int l, int w, const uint8_t *__restrict__ lev8, const int *__restrict__ level) {
+    return l + 1 < 255 ? lev8[w] == (uint8_t)(l + 1) : level[w] == l + 1;
+}
+
 // delta[v] = sum over neighbours w one level deeper of sigma[v]/sigma[w] * (1 + delta[w])
 // (process2.cu:5-123, validate.cu:182-189), for the vertices of one level.  Degree
 // tiers: a lane per vertex below 32 edges, a wave per vertex below 2048, and the
 // whole grid (4096-edge chunks, FP64 partials) above.  Sums are Neumaier-compensated
 // FP32 over terms split into two floats, so each term keeps ~48 bits.
-__device__ __forceinline__ void bc_term(KSum &acc, double sv, double sw, float dw) {
-    double t = (sv / sw) * (1.0 + (double)dw);
+// term sigma[v] * c[w] with c[w] = (1 + delta[w]) / sigma[w], stored in FP64 once per
+// vertex when its delta is final (one gather per successor edge, no division)
+__device__ __forceinline__ void bc_term(KSum &acc, double sv, double cw) {
+    const double t = sv * cw;
     float hi = (float)t;
     float lo = (float)(t - (double)hi);
     acc.add(hi);
@@ -634,9 +677,10 @@ __device__ __forceinline__ void bc_term(KSum &acc, double sv, double sw, float d
 
 __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
                                                     const int *__restrict__ row, const int *__restrict__ col,
-                                                    const int *__restrict__ level, const double *__restrict__ sigma,
-                                                    float *__restrict__ delta, HugeChunk *__restrict__ chunks,
-                                                    int *__restrict__ nchunks) {
+                                                    const int *__restrict__ level, const uint8_t *__restrict__ lev8,
+                                                    const double *__restrict__ sigma,
+                                                    float *__restrict__ delta, double *__restrict__ cw,
+                                                    HugeChunk *__restrict__ chunks, int *__restrict__ nchunks) {
     const int b = lvl_off[l], n = lvl_off[l + 1] - b;
     __shared__ int s_wv[kBlock];
     __shared__ int s_wn;
@@ -652,9 +696,11 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
                 KSum acc;
                 for (int p = rb; p < re; ++p) {
                     const int wv = col[p];
-                    if (level[wv] == l + 1) bc_term(acc, sv, sigma[wv], delta[wv]);
+                    if (bc_next(l, wv, lev8, level)) bc_term(acc, sv, cw[wv]);
                 }
-                delta[v] = acc.value();
+                const float dv = acc.value();
+                delta[v] = dv;
+                cw[v] = (1.0 + (double)dv) / sv;
             } else if (deg < 2048) {
                 s_wv[atomicAdd(&s_wn, 1)] = v;
             } else {
@@ -672,18 +718,23 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
             KSum acc;
             for (int p = rb + lane(); p < re; p += MG_WAVE) {
                 const int wv = ld_stream(&col[p]);
-                if (level[wv] == l + 1) bc_term(acc, sv, sigma[wv], delta[wv]);
+                if (bc_next(l, wv, lev8, level)) bc_term(acc, sv, cw[wv]);
             }
             acc = wave_ksum(acc);
-            if (lane() == 0) delta[v] = acc.value();
+            if (lane() == 0) {
+                const float dv = acc.value();
+                delta[v] = dv;
+                cw[v] = (1.0 + (double)dv) / sv;
+            }
         }
         __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__restrict__ row, const int *__restrict__ col,
-                                                         const int *__restrict__ level, const double *__restrict__ sigma,
-                                                         const float *__restrict__ delta, const HugeChunk *__restrict__ chunks,
+                                                         const int *__restrict__ level, const uint8_t *__restrict__ lev8,
+                                                         const double *__restrict__ sigma,
+                                                         const double *__restrict__ cw, const HugeChunk *__restrict__ chunks,
                                                          const int *__restrict__ nchunks, double *__restrict__ dacc) {
     const int nch = *nchunks;
     __shared__ float s_s[kBlock / MG_WAVE], s_c[kBlock / MG_WAVE];
@@ -695,7 +746,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
         KSum acc;
         for (int p = b + threadIdx.x; p < e; p += kBlock) {
             const int wv = ld_stream(&col[p]);
-            if (level[wv] == l + 1) bc_term(acc, sv, sigma[wv], delta[wv]);
+            if (bc_next(l, wv, lev8, level)) bc_term(acc, sv, cw[wv]);
         }
         acc = wave_ksum(acc);
         if (lane() == 0) { s_s[threadIdx.x >> 6] = acc.s; s_c[threadIdx.x >> 6] = acc.c; }
@@ -710,12 +761,14 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
 }
 
 __global__ void k_bc_back_fin(const HugeChunk *__restrict__ chunks, int *__restrict__ nchunks, double *__restrict__ dacc,
-                              float *__restrict__ delta) {
+                              const double *__restrict__ sigma, float *__restrict__ delta, double *__restrict__ cw) {
     const int nch = *nchunks;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nch; c += gridDim.x * blockDim.x) {
         if (chunks[c].j != 0) continue;  // the first chunk of each vertex finalizes it
         const int v = chunks[c].v;
-        delta[v] = (float)dacc[v];
+        const float dv = (float)dacc[v];
+        delta[v] = dv;
+        cw[v] = (1.0 + (double)dv) / sigma[v];
         dacc[v] = 0.0;
     }
 }
@@ -800,6 +853,8 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         MG_HIP(hipMalloc(&e->sigma, sizeof(double) * padv));
         MG_HIP(hipMalloc(&e->delta, sizeof(float) * padv));
         MG_HIP(hipMalloc(&e->dacc, sizeof(double) * padv));
+        MG_HIP(hipMalloc(&e->lev8, padv));
+        MG_HIP(hipMalloc(&e->bc_cw, sizeof(double) * padv));
         MG_HIP(hipMalloc(&e->lvl_off, sizeof(int) * ((long long)nv + 4)));
         hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(padv, 256, 4096)), dim3(256), 0, e->stream, e->dacc, padv, 0.0);
     }
@@ -1018,6 +1073,7 @@ int mg_bc_backward_impl(mg_engine *e) {
     off[1] = 1;
     MG_HIP(hipMemcpyAsync(e->lvl_off, off.data(), sizeof(int) * 2, hipMemcpyHostToDevice, s));
     int *nch = e->aux;  // split-vertex chunk counter
+    hipLaunchKernelGGL(k_lev8, dim3(grid_for(e->nv, 256, 4096)), dim3(256), 0, s, e->nv, e->level, e->lev8);
     for (int l = L - 1; l >= 0; --l) {
         const int n = off[l + 1] - off[l];
         if (n <= 0) continue;
@@ -1025,11 +1081,11 @@ int mg_bc_backward_impl(mg_engine *e) {
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, nch);
         if (e->prof) mg_prof_begin(e, "k_bc_back", 0.0, &slot);
         hipLaunchKernelGGL(k_bc_back, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
-                           e->level, e->sigma, e->delta, e->chunks, nch);
+                           e->level, e->lev8, e->sigma, e->delta, e->bc_cw, e->chunks, nch);
         if (e->prof) mg_prof_end(e, slot);
-        hipLaunchKernelGGL(k_bc_back_huge, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->sigma, e->delta,
+        hipLaunchKernelGGL(k_bc_back_huge, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->lev8, e->sigma, e->bc_cw,
                            e->chunks, nch, e->dacc);
-        hipLaunchKernelGGL(k_bc_back_fin, dim3(64), dim3(256), 0, s, e->chunks, nch, e->dacc, e->delta);
+        hipLaunchKernelGGL(k_bc_back_fin, dim3(64), dim3(256), 0, s, e->chunks, nch, e->dacc, e->sigma, e->delta, e->bc_cw);
     }
     MG_HIP(hipGetLastError());
     return MG_OK;
