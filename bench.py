@@ -168,7 +168,9 @@ def uneven(mg, np):
         t_def = float(np.median([b.run(src, slots) for _ in range(3)]))
         lv_def = b.levels()
         levels = b.sweeps()
+        st_def = b.stats()
         b.close()
+        log(f"uneven {name}: untuned {t_def:.2f} ms ({st_def}), tuning")
         t1 = time.time()
         opt, _, _ = mg.batch_autotune(u, kind, src, slots, reps=2)
         tune_s = time.time() - t1
@@ -180,8 +182,10 @@ def uneven(mg, np):
         out[name] = {"untuned_ms": t_def, "tuned_ms": t_tun, "untuned_gteps": gteps(4 * ne, t_def),
                      "tuned_gteps": gteps(4 * ne, t_tun), "levels": levels, "tuning_s": tune_s,
                      "results_equal": same,
-                     "tuned_options": {"alpha": opt.alpha, "batch_chunk": opt.batch_chunk,
-                                       "grid_blocks": opt.grid_blocks, "graph_levels": opt.graph_levels}}
+                     "tuned_options": {"alpha": opt.alpha, "beta": opt.beta, "batch_chunk": opt.batch_chunk,
+                                       "grid_blocks": opt.grid_blocks, "graph_levels": opt.graph_levels,
+                                       "persist_levels": opt.persist_levels},
+                     "untuned_stats": st_def}
         log(f"uneven {name}: {t_def:.2f} ms untuned, {t_tun:.2f} ms tuned ({tune_s:.1f}s of tuning)")
     return out
 

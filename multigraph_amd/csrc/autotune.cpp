@@ -2,7 +2,8 @@
 // graph itself.  The reference re-decides push vs pull every level from wall-clock
 // samples of both (process.cu:1161-1255, pseudo_* kernels); here the decision rule is
 // deterministic and its thresholds, with the grid shape, the levels per captured
-// graph, the PageRank partition size and the batched chunk size, are searched once.
+// graph, the PageRank partition size, the batched chunk size and whether batched
+// sparse levels run in a persistent launch, are searched once.
 //
 // Search: coordinate descent from the starting options, one pass over the
 // coordinates in the order listed; each candidate is built, run once to warm up, then
@@ -113,8 +114,13 @@ extern "C" int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32
     };
     std::vector<Coord> coords;
     coords.push_back({[](mg_options &o, double v) { o.alpha = (float)v; }, {4, 8, 15, 30, 60, 120}});
+    if (kind != MG_BATCH_BC)  // the sparse-level threshold (queue levels below (nv + E) / beta)
+        coords.push_back({[](mg_options &o, double v) { o.beta = (float)v; }, {4, 9, 18, 36, 72}});
     coords.push_back({[](mg_options &o, double v) { o.batch_chunk = (int)v; }, {256, 512, 1024, 4096}});
     coords.push_back({[](mg_options &o, double v) { o.grid_blocks = (int)v; }, {1024, 2048, 4096}});
-    if (kind != MG_BATCH_BC) coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {2, 8, 32}});
+    if (kind != MG_BATCH_BC) {
+        coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {2, 8, 32}});
+        coords.push_back({[](mg_options &o, double v) { o.persist_levels = (int)v; }, {0, -1}});
+    }
     return descend(coords, cost, best, ms_default, ms_tuned);
 }

@@ -130,6 +130,10 @@ __global__ __launch_bounds__(kBB) void k_nz_write(int nv, const uint64_t *__rest
 // with atomics (or, and, add, min) into state that starts the pass at its identity.
 struct Chunk { int v, p0, p1, multi; };
 constexpr int kCH = 1024;  // default edges per chunk (mg_options.batch_chunk)
+#ifndef MG_HIT_STEP
+#define MG_HIT_STEP 8
+#endif
+constexpr int kHitStep = MG_HIT_STEP;  // matched edges whose row gathers a wave issues together
 
 // Device-side level control (BFS and SSSP runs): the level kernels read the current
 // level from here, so K levels can be captured in one hipGraph and replayed past the
@@ -201,13 +205,33 @@ __global__ __launch_bounds__(kBB) void k_mb_push(const BCtl *ctl, long long nch,
                 if (m) atomicOr((unsigned long long *)&Fn[w], (unsigned long long)m);
             }
         } else {
+            // lanes load 64 edges and their visited words at once and mark the next
+            // frontier; then the wave walks the edges with new bits, four at a time,
+            // lanes = sources adding sigma[u][s] into sigma[w][s]
+            __shared__ int s_w[kBB];
+            __shared__ uint64_t s_m[kBB];
+            const int wl = threadIdx.x & ~(MG_WAVE - 1);
             const double su = ((fu >> lane()) & 1ull) ? sigma[(long long)k.v * 64 + lane()] : 0.0;
-            for (int p = k.p0; p < k.p1; ++p) {
-                const int w = col[p];
-                const uint64_t m = fu & ~vis[w];
-                if (!m) continue;
-                if ((m >> lane()) & 1ull) atomicAdd(&sigma[(long long)w * 64 + lane()], su);
-                if (lane() == 0) atomicOr((unsigned long long *)&Fn[w], (unsigned long long)m);
+            for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+                const int p = p0 + lane();
+                int w = 0;
+                uint64_t m = 0;
+                if (p < k.p1) { w = col[p]; m = fu & ~vis[w]; }
+                if (m) atomicOr((unsigned long long *)&Fn[w], (unsigned long long)m);
+                uint64_t hits = ballot(m != 0ull);
+                if (!hits) continue;
+                wave_sync();
+                s_w[threadIdx.x] = w;
+                s_m[threadIdx.x] = m;
+                wave_sync();
+                while (hits) {
+                    int j[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (j[q] >= 0 && ((s_m[wl + j[q]] >> lane()) & 1ull)) atomicAdd(&sigma[(long long)s_w[wl + j[q]] * 64 + lane()], su);
+                }
             }
         }
     }
@@ -271,9 +295,17 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
             s_u[threadIdx.x] = u;
             s_f[threadIdx.x] = f;
             wave_sync();
-            const int nn = min(MG_WAVE, k.p1 - p0);
-            for (int j = 0; j < nn; ++j) {
-                if ((s_f[wl + j] >> lane()) & 1ull) sg += sigma[(long long)s_u[wl + j] * 64 + lane()];
+            uint64_t hits = ballot(f != 0ull);  // only the parents in some source's frontier
+            while (hits) {  // kHitStep parents per step: their row gathers overlap
+                int j[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
+                double t[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q)
+                    t[q] = (j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull)) ? sigma[(long long)s_u[wl + j[q]] * 64 + lane()] : 0.0;
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q) sg += t[q];
             }
         }
         if (!acc) continue;
@@ -805,17 +837,38 @@ __global__ void k_ms_scatter_words(long long n, const int *__restrict__ lv, cons
         D[lv[i]] = clear ? 0ull : lw[i];
 }
 
-__device__ __forceinline__ void ms_term(KSum &acc, double sv, double sw, float dw) {
-    const double t = (sv / sw) * (1.0 + (double)dw);
+// term sigma[v][s] * c[w][s] with c = (1 + delta) / sigma of the successor, computed in
+// FP64 and stored in FP32 (like delta) once per (vertex, source) when its level is
+// finished (k_ms_cw): a matched successor costs one 256-byte row, not two rows and a
+// division per source
+__device__ __forceinline__ void ms_term(KSum &acc, double sv, float cw) {
+    const double t = sv * (double)cw;
     const float hi = (float)t;
     acc.add(hi);
     acc.add((float)(t - (double)hi));
 }
 
-// E = words of level d, D = words of level d+1; a wave per chunk of a vertex in E
+// c[v][s] = (1 + delta[v][s]) / sigma[v][s] for the (vertex, word) entries of a finished
+// level whose vertex is split into several chunks (k_mb_back writes the others' c)
+__global__ void k_ms_cw(long long n, const int *__restrict__ lv, const uint64_t *__restrict__ lw, const int *__restrict__ row,
+                        int chunk, const double *__restrict__ sigma, const float *__restrict__ delta, float *__restrict__ cw) {
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long i = wave; i < n; i += nw) {
+        const int v = lv[i];
+        if (row[v + 1] - row[v] <= chunk || !((lw[i] >> lane()) & 1ull)) continue;
+        const long long kv = (long long)v * 64 + lane();
+        cw[kv] = (float)((1.0 + (double)delta[kv]) / sigma[kv]);
+    }
+}
+
+// E = words of level d, D = words of level d+1; a wave per chunk of a vertex in E.
+// Lanes load 64 edges' successor words at a time; then only the edges with a common
+// bit are walked (lanes = sources), each one gather of the successor's c row.
 __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
                                                  const uint64_t *__restrict__ E, const uint64_t *__restrict__ D,
-                                                 const double *__restrict__ sigma, float *__restrict__ delta) {
+                                                 const double *__restrict__ sigma, float *__restrict__ cw,
+                                                 float *__restrict__ delta) {
     __shared__ int s_u[kBB];
     __shared__ uint64_t s_f[kBB];
     const int wl = threadIdx.x & ~(MG_WAVE - 1);
@@ -833,23 +886,34 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
             int w = 0;
             uint64_t f = 0;
             if (p < k.p1) { w = ld_stream(&col[p]); f = D[w] & word; }
-            if (!wave_or(f)) continue;
+            uint64_t hits = ballot(f != 0ull);
+            if (!hits) continue;
             wave_sync();
             s_u[threadIdx.x] = w;
             s_f[threadIdx.x] = f;
             wave_sync();
-            const int nn = min(MG_WAVE, k.p1 - p0);
-            for (int j = 0; j < nn; ++j) {
-                if ((s_f[wl + j] >> lane()) & 1ull) {
-                    const long long kk = (long long)s_u[wl + j] * 64 + lane();
-                    ms_term(acc, sv, sigma[kk], delta[kk]);
-                }
+            while (hits) {  // kHitStep matched edges per step: their row gathers overlap
+                int j[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
+                float c[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q)
+                    c[q] = (j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull)) ? cw[(long long)s_u[wl + j[q]] * 64 + lane()] : 0.f;
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q)
+                    if (c[q] != 0.f) ms_term(acc, sv, c[q]);
             }
         }
         if (mine) {
             const long long kv = (long long)k.v * 64 + lane();
-            if (k.multi) atomicAdd(&delta[kv], acc.value());
-            else delta[kv] = acc.value();
+            if (k.multi) {
+                atomicAdd(&delta[kv], acc.value());
+            } else {  // final: also its c for the level above
+                const float dv = acc.value();
+                delta[kv] = dv;
+                cw[kv] = (float)((1.0 + (double)dv) / sv);
+            }
         }
     }
 }
@@ -972,8 +1036,9 @@ struct mg_batch {
     int seed_cap = 0, nseeds = 0;  // device capacity of src/slot, sources of the last run
     bool slotted = false;          // the last run gave an explicit slot list
     uint64_t all = 0;              // the slots in use
-    double *sigma = nullptr;
+    double *sigma = nullptr;      // BC path counts
     float *delta = nullptr, *bc = nullptr;
+    float *cw = nullptr;          // BC: (1 + delta) / sigma of the finished levels
     Chunk *chunks = nullptr;      // static edge chunks
     long long nch = 0;
     int *logv = nullptr;          // (vertex, word) entries of every level (BC)
@@ -998,7 +1063,7 @@ static void batch_free(mg_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
-                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc};
+                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->cw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1056,6 +1121,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     if (ok && kind == MG_BATCH_BC) {
         b->log_cap = nv * 8 + 1024;  // grown on demand
         ok = hipMalloc(&b->sigma, sizeof(double) * slots) == hipSuccess && hipMalloc(&b->delta, sizeof(float) * slots) == hipSuccess &&
+             hipMalloc(&b->cw, sizeof(float) * slots) == hipSuccess &&
              hipMalloc(&b->bc, sizeof(float) * (nv + 1)) == hipSuccess && hipMalloc(&b->D, sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
              hipMalloc(&b->E, sizeof(uint64_t) * (nv + 1)) == hipSuccess &&
              hipMalloc(&b->logv, sizeof(int) * b->log_cap) == hipSuccess &&
@@ -1128,6 +1194,7 @@ static int read_counts(mg_batch *b, int nparts, long long *n, long long *mf) {
 
 static int run_bfs_bc(mg_batch *b) {
     hipStream_t s = b->stream;
+    const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;  // edges per static chunk
     const int nv = b->nv;
     const bool bc = b->kind == MG_BATCH_BC;
     const uint64_t all = b->all;
@@ -1183,7 +1250,9 @@ static int run_bfs_bc(mg_batch *b) {
         if (n0 <= 0) continue;
         hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->E, 0);
         if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], b->D, 0);
-        hipLaunchKernelGGL(k_mb_back, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, b->E, b->D, b->sigma, b->delta);
+        hipLaunchKernelGGL(k_mb_back, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, b->E, b->D, b->sigma, b->cw, b->delta);
+        hipLaunchKernelGGL(k_ms_cw, dim3(grid_for(n0, 4, 8192)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->row,
+                           chunk, b->sigma, b->delta, b->cw);
         hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->E, 1);
         if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], b->D, 1);
     }
