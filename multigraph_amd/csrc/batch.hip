@@ -962,9 +962,18 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp(const BCtl *ctl, long long nch,
             s_w[threadIdx.x] = wu;
             s_f[threadIdx.x] = f;
             wave_sync();
-            const int nn = min(MG_WAVE, k.p1 - p0);
-            for (int j = 0; j < nn; ++j) {
-                if ((s_f[wl + j] >> lane()) & 1ull) best = min(best, dist[(long long)s_u[wl + j] * 64 + lane()] + s_w[wl + j]);
+            uint64_t hits = ballot(f != 0ull);  // only neighbours that changed for some source
+            while (hits) {  // kHitStep neighbours per step: their row gathers overlap
+                int j[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
+                int cand[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q)
+                    cand[q] = (j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull))
+                                  ? dist[(long long)s_u[wl + j[q]] * 64 + lane()] + s_w[wl + j[q]] : (int)MG_INF;
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q) best = min(best, cand[q]);
             }
         }
         bool imp;
@@ -996,16 +1005,24 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp_push(const BCtl *ctl, long long
         if (!cu) continue;
         const bool mine = (cu >> lane()) & 1ull;
         const int du = mine ? dist[(long long)k.v * 64 + lane()] : (int)MG_INF;
-        for (int p = k.p0; p < k.p1; ++p) {
-            const int w = col[p];
-            const int nd = du + wgt[p];
-            bool imp = false;
-            if (mine) {
-                int *dw = &dist[(long long)w * 64 + lane()];
-                if (nd < *dw) imp = atomicMin(dw, nd) > nd;
+        // edges kHitStep at a time: their distance-row reads are in flight together
+        for (int p0 = k.p0; p0 < k.p1; p0 += kHitStep) {
+            int w[kHitStep], nd[kHitStep], dw[kHitStep];
+#pragma unroll
+            for (int q = 0; q < kHitStep; ++q) {
+                const int p = min(p0 + q, k.p1 - 1);
+                w[q] = col[p];
+                nd[q] = p0 + q < k.p1 ? du + wgt[p] : (int)MG_INF;
             }
-            const uint64_t m = ballot(imp);
-            if (lane() == 0 && m) atomicOr((unsigned long long *)&Cn[w], (unsigned long long)m);
+#pragma unroll
+            for (int q = 0; q < kHitStep; ++q) dw[q] = mine ? dist[(long long)w[q] * 64 + lane()] : (int)MG_INF;
+#pragma unroll
+            for (int q = 0; q < kHitStep; ++q) {
+                bool imp = false;
+                if (mine && nd[q] < dw[q]) imp = atomicMin(&dist[(long long)w[q] * 64 + lane()], nd[q]) > nd[q];
+                const uint64_t m = ballot(imp);
+                if (lane() == 0 && m) atomicOr((unsigned long long *)&Cn[w[q]], (unsigned long long)m);
+            }
         }
     }
 }
