@@ -138,6 +138,10 @@ int mg_reset(mg_engine *e, int source);
  * must be called again, 0 when the run is finished, < 0 on error.  *iter is the
  * level / iteration number and is advanced by the callee, as in the reference. */
 int mg_step(mg_engine *e, int *iter);
+/* Mode of the level the last mg_step ran: 0 dense (pull / all-edge pass), 1 sparse
+ * (push over the frontier queue), -1 none; the first field of the reference's TRACE
+ * lines ("0 %d %f" / "1 %d %f", process.cu:1300-1305,1349-1353). */
+int mg_step_mode(const mg_engine *e, int *mode);
 /* BC dependency pass over the logged levels (MultiGraph_V_E_V_pull). */
 int mg_bc_backward(mg_engine *e, int iter);
 /* reset + all steps (+ backward for BC), timed with HIP events on the engine's

@@ -92,6 +92,7 @@ _SIGS = {
     "mg_batch_release": (C.c_int, [_P]),
     "mg_batch_levels": (C.c_int, [_P, _I32P]),
     "mg_batch_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.c_int]),
+    "mg_step_mode": (C.c_int, [_P, _I32P]),
     "mg_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, C.c_int, C.POINTER(Options), C.POINTER(C.c_float),
                               C.POINTER(C.c_float)]),
     "mg_batch_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, _I32P, C.c_int, C.POINTER(Options),
@@ -273,6 +274,12 @@ class Engine:
         it = C.c_int32(0)
         more = _check(lib().mg_step(self._h, C.byref(it)))
         return more, it.value
+
+    def step_mode(self) -> int:
+        """mode of the level the last step ran: 0 dense (pull), 1 sparse (push), -1 none"""
+        m = C.c_int32(-1)
+        _check(lib().mg_step_mode(self._h, C.byref(m)))
+        return m.value
 
     def bc_backward(self) -> None:
         _check(lib().mg_bc_backward(self._h, 0))

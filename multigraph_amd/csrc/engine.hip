@@ -131,6 +131,7 @@ extern "C" int mg_step(mg_engine *e, int *iter) {
     int more = 0;
     switch (e->algo) {
     case MG_CC: {
+        e->last_mode = 0;
         if (e->done) { more = 0; break; }
         int rc = mg_cc_step(e, &more);
         if (rc != MG_OK) return rc;
@@ -138,11 +139,13 @@ extern "C" int mg_step(mg_engine *e, int *iter) {
     }
     case MG_PR_T:
     case MG_PR_D: {
+        e->last_mode = 0;
         int rc = mg_pr_step(e, &more);
         if (rc != MG_OK) return rc;
         break;
     }
     default: {
+        e->last_mode = e->done ? -1 : (e->h_ctl->mode == mg::MODE_BU ? 0 : 1);
         int rc = mg_frontier_level(e);
         if (rc != MG_OK) return rc;
         more = e->done ? 0 : 1;
@@ -151,6 +154,12 @@ extern "C" int mg_step(mg_engine *e, int *iter) {
     }
     if (iter) *iter = e->iter;
     return more;
+}
+
+extern "C" int mg_step_mode(const mg_engine *e, int *mode) {
+    if (!e || !mode) return mg_fail(MG_ERR_ARG, "null argument");
+    *mode = e->last_mode;
+    return MG_OK;
 }
 
 extern "C" int mg_bc_backward(mg_engine *e, int iter) {

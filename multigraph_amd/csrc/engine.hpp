@@ -199,6 +199,7 @@ struct mg_engine {
     long long pr_nvi = 0;             // internal (renumbered, padded) vertex count
     std::vector<int> pr_perm_h;       // compacted id -> internal id
     // host-side progress
+    int last_mode = -1;        // mode of the level the last mg_step ran: 0 dense (pull), 1 sparse (push)
     int iter = 1;
     bool ready = false;
     bool done = false;

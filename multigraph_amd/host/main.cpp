@@ -26,10 +26,6 @@
 #include <cstring>
 #include "multigraph_compat.h"
 
-static double rtclock() {
-    using namespace std::chrono;
-    return duration<double>(steady_clock::now().time_since_epoch()).count();
-}
 
 // ------------------------------------------------------------------ certificates
 static int verify_levels(const csc_package *inp, const std::vector<int> &L, int src, const int *w) {

@@ -11,6 +11,7 @@
 // message to stderr and exits with status 1 (status 0 for the out-of-range id, as
 // the reference does).
 #pragma once
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -134,10 +135,32 @@ inline void mg_fill_output(struct HYB *m, struct OUTPUT *out) {
 #endif
 }
 
-// process.cu:1079 / processi.cu:682 / process.cu:1051,1012: nonzero = call again
+// function.cu:4-12: wall clock in seconds
+inline double rtclock() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// process.cu:1079 / processi.cu:682 / process.cu:1051,1012: nonzero = call again.
+// With MG_TRACE set (the reference's -DTRACE), every level prints "<mode> <level> <ms>"
+// as process.cu:1300-1305,1349-1353 do: mode 0 dense, 1 sparse, the level counted from
+// 0, milliseconds between device synchronisations around it.
 inline int mg_compat_step(struct HYB *m, int *iter) {
+    static const bool trace = getenv("MG_TRACE") != nullptr;
+    const int level = iter ? *iter : 1;
+    double t0 = 0.0;
+    if (trace) {
+        mg_check(mg_sync(m->e), "sync");
+        t0 = rtclock();
+    }
     int rc = mg_step(m->e, iter);
     mg_check(rc, "MultiGraph step");
+    if (trace) {
+        mg_check(mg_sync(m->e), "sync");
+        int mode = -1;
+        mg_check(mg_step_mode(m->e, &mode), "step mode");
+        if (mode >= 0) printf("%d %d %f\n", mode, level - 1, (rtclock() - t0) * 1000.0);
+    }
     return rc;
 }
 inline int MultiGraph_V_E_V(struct HYB *m, struct csc_package *, struct vector_data *, struct OUTPUT *, int *iter) {

@@ -1,0 +1,60 @@
+"""The reference-style drivers (./BFS file.mtx source iterations, code/main.cu:18-159)
+on an MTX file: exit status, the `processing : X ms,Y MTEPS,` and `validation : PASS`
+lines, the per-level trace lines with MG_TRACE (process.cu:1300-1305: "<mode> <level>
+<ms>", mode 0 dense, 1 sparse), and the MG_PRINT_OUTPUT file (validate.cu:321-329:
+"idx<TAB>value" in compacted ids) against the oracle on the same graph."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "multigraph_amd", "bin")
+DRIVERS = ["BFS", "SSSP", "BC", "CC", "DATA_DRIVEN_PR", "TOPOLOGY_DRIVEN_PR"]
+SOURCE = 5  # the source argument; the reader remaps it into compacted ids (read_input.cu:130)
+
+
+@pytest.fixture(scope="module")
+def mtx(mg, tmp_path_factory):
+    g = mg.Graph.rmat(12, 8, seed=4)
+    path = tmp_path_factory.mktemp("mtx") / "rmat12.mtx"
+    g.write_mtx(str(path))
+    return str(path)
+
+
+@pytest.mark.parametrize("name", DRIVERS)
+def test_driver_cli(mg, orc, mtx, name, tmp_path):
+    out = tmp_path / "out.txt"
+    env = dict(os.environ, MG_PRINT_OUTPUT=str(out), MG_TRACE="1")
+    print(f"driver {name} {mtx} {SOURCE} 2", flush=True)
+    r = subprocess.run([os.path.join(BIN, name), mtx, str(SOURCE), "2"], capture_output=True, text=True, timeout=120,
+                       env=env)
+    assert r.returncode == 0, r.stderr
+    assert re.search(r"processing : [0-9.]+ ms,[0-9.]+ MTEPS,", r.stdout), r.stdout
+    assert "validation : PASS" in r.stdout, r.stdout
+    trace = [ln.split() for ln in r.stdout.splitlines() if re.fullmatch(r"[01] \d+ [0-9.]+", ln)]
+    assert trace, r.stdout
+    levels = [int(t[1]) for t in trace]
+    assert levels[0] == 0 and levels.count(0) == 2  # two runs, each from level 0
+    flags = mg.READ_SYM | (mg.READ_CC if name == "CC" else 0)
+    g = mg.Graph.load_mtx(mtx, source=SOURCE, flags=flags)
+    row, col, w = g.csr()
+    s = g.start_p
+    vals = np.loadtxt(out, delimiter="\t")
+    assert vals.shape == (g.nv, 2)
+    np.testing.assert_array_equal(vals[:, 0], np.arange(g.nv))
+    got = vals[:, 1]
+    if name == "BFS":
+        np.testing.assert_array_equal(got.astype(np.int64), orc.bfs(row, col, s))
+    elif name == "SSSP":
+        np.testing.assert_array_equal(got.astype(np.int64), orc.sssp_bf(row, col, w, s)[0])
+    elif name == "BC":
+        np.testing.assert_array_equal(got.astype(np.int64), orc.bc_fp64(row, col, s)[0])
+    elif name == "CC":
+        np.testing.assert_array_equal(got.astype(np.int64), orc.cc_unionfind(row, col)[0])
+    else:  # ranks: r = 0.15 + 0.85 * sum >= 0.15 (code/common.h:58-61)
+        assert np.all(got >= 0.15 - 1e-6) and np.isfinite(got).all()
