@@ -139,7 +139,8 @@ struct mg_pb_layout {
     int *split_parts = nullptr;  // destination partitions processed by more than one unit
     int *apart = nullptr;        // np + 1: first source-major message slot of every partition
     // SSSP sweeps over the layout
-    uint16_t *b_w = nullptr;     // per edge slot: its weight (destination-major, like b_dst)
+    void *b_w = nullptr;         // per edge slot: its weight (destination-major, like b_dst), uint8 or uint16
+    bool w8 = false;             // b_w holds bytes (every weight <= 255)
     int *inv = nullptr;          // internal id -> compacted id (engine ids)
     int4 *unitsS = nullptr;      // every source partition {p, a0, a1, 0}: messages rebuilt from distances
     int nS = 0;

@@ -817,8 +817,13 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         // dense sweeps over the partitioned layout (symmetric graphs, weights in 16 bits),
         // else over the CSR
         bool w16 = g->symmetric && e->opt.pull_layout == 0 && nv >= 2;
-        for (long long p = 0; w16 && p < ne; ++p) w16 = g->w[p] >= 0 && g->w[p] <= 0xFFFF;
+        int wmax = 0;
+        for (long long p = 0; w16 && p < ne; ++p) {
+            w16 = g->w[p] >= 0 && g->w[p] <= 0xFFFF;
+            wmax = std::max(wmax, (int)g->w[p]);
+        }
         if (w16) {
+            e->pb.w8 = wmax <= 0xFF;
             MG_TRY(mg_pb_build(e, g));
             e->sssp_pb = true;
         }

@@ -151,6 +151,13 @@ __device__ __forceinline__ T ld_stream(const T *p) { return __builtin_nontempora
 typedef unsigned int mg_u32x4 __attribute__((ext_vector_type(4)));
 typedef float mg_f32x4 __attribute__((ext_vector_type(4)));
 
+typedef unsigned int mg_u32x2 __attribute__((ext_vector_type(2)));
+// 8-byte streaming load (p must be 8-byte aligned)
+__device__ __forceinline__ uint2 ld_stream_u2(const void *p) {
+    const mg_u32x2 v = __builtin_nontemporal_load((const mg_u32x2 *)p);
+    return make_uint2(v.x, v.y);
+}
+
 // 16-byte streaming loads (p must be 16-byte aligned)
 __device__ __forceinline__ uint4 ld_stream_u4(const void *p) {
     const mg_u32x4 v = __builtin_nontemporal_load((const mg_u32x4 *)p);

@@ -182,11 +182,12 @@ __global__ __launch_bounds__(kBT) void k_pb_tile_sort(long long nt, const int *_
 // SSSP: the weight of every edge slot, in the destination-major order of b_dst.  A
 // tile's sorted keys give each slot's local source and destination; the weight is
 // found in the engine's CSR (rows sorted by column) through the inverse numbering.
+template <class W>
 __global__ __launch_bounds__(kBT) void k_pb_weights(long long nt, int np, int P, const int *__restrict__ count,
                                                     const long long *__restrict__ boff, const uint32_t *__restrict__ keys,
                                                     const int *__restrict__ inv, const int *__restrict__ row,
                                                     const int *__restrict__ col, const int *__restrict__ w,
-                                                    uint16_t *__restrict__ b_w) {
+                                                    W *__restrict__ b_w) {
     for (long long t = blockIdx.x; t < nt; t += gridDim.x) {
         const int n = count[t];
         if (n == 0) continue;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(kBT) void k_pb_weights(long long nt, int np, int P,
             const int src = inv[s * P + (int)(key >> 16)], dst = inv[d * P + (int)(key & 0xFFFFu)];
             int lo = row[dst], hi = row[dst + 1] - 1;
             while (lo < hi) { const int mid = (lo + hi) >> 1; if (col[mid] < src) lo = mid + 1; else hi = mid; }
-            b_w[eb + j] = (uint16_t)w[lo];
+            b_w[eb + j] = (W)w[lo];
         }
     }
 }
@@ -596,8 +597,11 @@ __device__ __forceinline__ void load_msg_w(SegW &sg, const SegHdr &h, const uint
     for (int k = 0; k < kStageM; ++k) sg.m[k] = ld_stream(&msg[q0 + min(lane() + k * MG_WAVE, sg.n - 1)]);
 }
 
+// WT: uint8_t weights when every weight fits a byte (a slot's 8 weights are one 8-byte
+// load), else uint16_t (one 16-byte load)
+template <class WT>
 __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
-                                                   const uint16_t *__restrict__ b_dst, const uint16_t *__restrict__ b_w,
+                                                   const uint16_t *__restrict__ b_dst, const WT *__restrict__ b_w,
                                                    const int *__restrict__ gbase, int *__restrict__ msgbuf, long long npairs,
                                                    const int *__restrict__ apart, const uint16_t *__restrict__ a_src,
                                                    const int *__restrict__ amap, const int *__restrict__ inv,
@@ -626,7 +630,13 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
             si = min(si, nseg - 1);
             const int gs = g0 + si * kSeg, ge = min(g1, g0 + (si + 1) * kSeg);
             load_hdr(h, gs, ge, b_dst, gbase);
-            wt = ld_stream_u4(&b_w[(long long)min(gs + lane(), ge - 1) * kG]);
+            const long long wg = (long long)min(gs + lane(), ge - 1) * kG;
+            if (sizeof(WT) == 1) {
+                const uint2 x = ld_stream_u2(&b_w[wg]);
+                wt = make_uint4(x.x, x.y, 0u, 0u);
+            } else {
+                wt = ld_stream_u4(&b_w[wg]);
+            }
         };
         int hub = MG_INF;
         auto process = [&](const SegW &sg) {
@@ -640,7 +650,8 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const uint32_t v = (k & 1) ? (w[k >> 1] >> 16) : (w[k >> 1] & 0xFFFFu);
-                const int wt = (int)((k & 1) ? (ww[k >> 1] >> 16) : (ww[k >> 1] & 0xFFFFu));
+                const int wt = sizeof(WT) == 1 ? (int)((ww[k >> 2] >> (8 * (k & 3))) & 0xFFu)
+                                              : (int)((k & 1) ? (ww[k >> 1] >> 16) : (ww[k >> 1] & 0xFFFFu));
                 if (k > 0) pi += (int)(v >> 15);
                 const uint32_t dst = v & 0x7FFFu;
                 const int m = s_msg[pi];
@@ -952,10 +963,15 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
         if (fixed) MG_HIP(hipMemcpy(npair, hn.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
     }
     if (e->algo == MG_SSSP) {  // per edge slot weights, from the sorted keys (padding: 0)
-        if (hipMalloc(&L.b_w, sizeof(uint16_t) * sl) != hipSuccess) { cleanup(); return mg_fail(MG_ERR_NOMEM, "partitioned layout: weights"); }
-        hipLaunchKernelGGL(k_fill_u16, dim3(grid_for(sl, 256, 8192)), dim3(256), 0, s, L.b_w, sl, (uint16_t)0);
-        hipLaunchKernelGGL(k_pb_weights, dim3(grid_for(nt, 1, 8192)), dim3(kBT), 0, s, nt, np, P, count, boff, keys, L.inv, e->row,
-                           e->col, e->w, L.b_w);
+        const size_t wb = L.w8 ? 1 : 2;
+        if (hipMalloc(&L.b_w, wb * sl + 16) != hipSuccess) { cleanup(); return mg_fail(MG_ERR_NOMEM, "partitioned layout: weights"); }
+        MG_HIP(hipMemsetAsync(L.b_w, 0, wb * sl + 16, s));
+        if (L.w8)
+            hipLaunchKernelGGL(k_pb_weights<uint8_t>, dim3(grid_for(nt, 1, 8192)), dim3(kBT), 0, s, nt, np, P, count, boff, keys, L.inv,
+                               e->row, e->col, e->w, (uint8_t *)L.b_w);
+        else
+            hipLaunchKernelGGL(k_pb_weights<uint16_t>, dim3(grid_for(nt, 1, 8192)), dim3(kBT), 0, s, nt, np, P, count, boff, keys,
+                               L.inv, e->row, e->col, e->w, (uint16_t *)L.b_w);
         MG_HIP(hipGetLastError());
     }
     // messages per tile -> padded message slots in both orders -> offsets
@@ -1018,8 +1034,9 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
         MG_HIP(hipMalloc(&L.unitsS, sizeof(int4) * std::max<size_t>(1, us.size())));
         if (!us.empty()) MG_HIP(hipMemcpy(L.unitsS, us.data(), sizeof(int4) * us.size(), hipMemcpyHostToDevice));
         MG_HIP(hipFuncSetAttribute((const void *)k_ps_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(int) * P)));
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(sizeof(int) * (P + 1) + sizeof(int) * kStage)));
+        const int lds_ps = (int)(sizeof(int) * (P + 1) + sizeof(int) * kStage);
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
     }
     MG_HIP(hipMalloc(&L.unitsA, sizeof(int4) * std::max<size_t>(1, ua.size())));
     MG_HIP(hipMalloc(&L.unitsB, sizeof(int4) * std::max<size_t>(1, ub.size())));
@@ -1078,9 +1095,13 @@ int mg_ps_sweep(mg_engine *e) {
     // one full relaxation pass: 12E + 8V (+4)
     const double alg = 12.0 * (double)e->ne + 8.0 * e->nv + 4.0;
     if (e->prof) mg_prof_begin(e, "k_ps_gather", alg, &slot);
-    hipLaunchKernelGGL(k_ps_gather, dim3(grid_for(L.nB, 1, G)), dim3(kBG), sizeof(int) * (P + 1) + sizeof(int) * kStage, s, e->ctl,
-                       L.unitsB, L.nB, P, e->nv, L.np, L.b_dst, L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv,
-                       e->level, e->vfc);
+    const size_t lds = sizeof(int) * (P + 1) + sizeof(int) * kStage;
+    if (L.w8)
+        hipLaunchKernelGGL(k_ps_gather<uint8_t>, dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P, e->nv, L.np,
+                           L.b_dst, (const uint8_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv, e->level, e->vfc);
+    else
+        hipLaunchKernelGGL(k_ps_gather<uint16_t>, dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P, e->nv, L.np,
+                           L.b_dst, (const uint16_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv, e->level, e->vfc);
     if (e->prof) mg_prof_end(e, slot);
     if (L.nA > 0)
         hipLaunchKernelGGL(k_ps_scatter, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(int) * P, s, e->ctl, 1, L.unitsA, L.nA, P, e->nv,

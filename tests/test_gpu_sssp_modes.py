@@ -63,3 +63,22 @@ def test_sssp_layout_split_partition(mg, orc, mode):
         ref, _ = orc.sssp_bf(row, col, w, s)
         np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref, err_msg=f"mode {mode} source {s}")
     e.close()
+
+
+@pytest.mark.parametrize("wmax", [255, 60000, 100000], ids=["w8", "w16", "csr-fallback"])
+def test_sssp_weight_widths(mg, orc, wmax):
+    """the layout stores weights as bytes up to 255, as 16 bits up to 65535, and larger
+    weights fall back to the CSR sweeps: distances bit-exact in every case"""
+    rng = np.random.default_rng(wmax)
+    n = 3000
+    src = rng.integers(0, n, 24000)
+    dst = rng.integers(0, n, 24000)
+    val = rng.integers(1, wmax + 1, 24000)
+    g = mg.Graph.from_coo(n, src, dst, val)
+    row, col, w = g.csr()
+    assert w.max() <= wmax
+    e = mg.Engine(g, mg.SSSP, mode=2)
+    for s in (0, g.nv // 2):
+        e.run(s)
+        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.sssp_bf(row, col, w, s)[0], err_msg=f"wmax {wmax} src {s}")
+    e.close()
