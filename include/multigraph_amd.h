@@ -89,6 +89,10 @@ typedef struct mg_options {
     int    pull_layout;     /* SSSP dense sweeps: 0, the default, over the partitioned
                                layout (distance messages, 16-bit weights; symmetric graphs
                                with weights in [0, 65535]), 1 over the CSR               */
+    int    sssp_delta;      /* batched SSSP: sparse sweeps relax only the (vertex, source)
+                               pairs below a distance threshold that rises by this much
+                               per level, the others wait (0: a quarter of the mean edge
+                               weight; -1: every pair, plain Bellman-Ford)              */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */

@@ -51,6 +51,7 @@ class Options(C.Structure):
         ("batch_chunk", C.c_int),
         ("persist_levels", C.c_int),
         ("pull_layout", C.c_int),
+        ("sssp_delta", C.c_int),
     ]
 
 

@@ -150,6 +150,7 @@ struct BCtl {
     int qn[2];    // queue lengths of F[0], F[1]
     int qvalid[2];    // the queue of F[b] lists its nonzero words
     int ndense, nsparse, npersist;  // levels run: dense, sparse, sparse by persistent launches
+    int thr;      // SSSP: sparse sweeps relax the (vertex, source) pairs below this distance
     long long n, mf;  // size and degree sum of the last produced frontier
     unsigned long long all;  // the slots in use
 };
@@ -384,7 +385,7 @@ __global__ void k_mb_ctl_init(BCtl *ctl, unsigned long long all) {
 // whose vertices plus edges are below (nv + E) / beta and whose vertices all fit one
 // chunk, so per-level work follows the frontier, not the graph; high-diameter members
 // of an uneven batch run thousands of such levels)
-struct LevelRule { long long ne; int nv; float alpha, beta; int forced, symmetric, light; };
+struct LevelRule { long long ne; int nv; float alpha, beta; int forced, symmetric, light, delta; };
 
 __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, long long x, const LevelRule &r) {
     c.n = a;
@@ -392,6 +393,7 @@ __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, long l
     c.need_full = 0;
     if (a == 0) { c.done = 1; return; }
     c.it += 1;
+    c.thr = r.delta <= 0 ? 0x7FFFFFFF : (c.thr > 0x7FFFFFFF - r.delta ? 0x7FFFFFFF : c.thr + r.delta);
     int push = (double)b < (double)r.ne / r.alpha;
     if (r.forced == MG_MODE_SPARSE) push = 1;
     if (r.forced == MG_MODE_DENSE) push = 0;
@@ -584,7 +586,11 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
 // bit is or-ed into w's next changed word; w joins the next queue when that word was
 // empty.  A lane walks its vertex's (bit, edge) pairs alone when they are few; a
 // heavier vertex is walked by the wave with lanes = sources, edges in turn.
-__device__ __forceinline__ void sparse_sssp(int n, const int *__restrict__ row, const int *__restrict__ col,
+// Pairs whose distance is not below `thr` are not relaxed in this sweep: their bits go
+// back into the vertex's next changed word (near-far order: the threshold rises by a
+// fixed band per level, so distances are settled roughly in increasing order instead of
+// being lowered again and again as in plain Bellman-Ford; the result is the same).
+__device__ __forceinline__ void sparse_sssp(int n, int thr, const int *__restrict__ row, const int *__restrict__ col,
                                             const int *__restrict__ wgt, uint64_t *C, uint64_t *Cn, const int *Q, int *Qn,
                                             int *qn_next, int *__restrict__ dist, int &cnt, long long &mf, int &mx) {
     const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
@@ -594,12 +600,19 @@ __device__ __forceinline__ void sparse_sssp(int n, const int *__restrict__ row, 
         int u = -1, p0 = 0, p1 = 0;
         uint64_t cu = 0;
         if (i < n) { u = Q[i]; cu = C[u]; p0 = row[u]; p1 = row[u + 1]; }
-        const int work = u >= 0 ? __builtin_popcountll(cu) * (p1 - p0) : 0;
-        const bool alone = u >= 0 && work <= kLaneWork;
-        int steps = alone ? work : 0;
+        const bool alone = u >= 0 && __builtin_popcountll(cu) * (p1 - p0) <= kLaneWork;
+        uint64_t rem = alone ? cu : 0ull;  // bits still to walk; the lowest is the current source
+        uint64_t far = 0ull;               // bits left for a later sweep
+        if (thr != 0x7FFFFFFF) {
+            for (uint64_t t = rem; t; t &= t - 1) {
+                const int sl = __builtin_ctzll(t);
+                if (dist[(long long)u * 64 + sl] >= thr) far |= 1ull << sl;
+            }
+            rem &= ~far;
+        }
+        int steps = __builtin_popcountll(rem) * (p1 - p0);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
-        uint64_t rem = alone ? cu : 0ull;  // bits still to walk; the lowest is the current source
         int p = p0, du = 0;
         if (rem) du = dist[(long long)u * 64 + __builtin_ctzll(rem)];
         for (int k = 0; k < steps; ++k) {
@@ -620,14 +633,25 @@ __device__ __forceinline__ void sparse_sssp(int n, const int *__restrict__ row, 
             }
             queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
         }
+        {  // far pairs wait in the next changed word
+            const bool add = far && atomicOr((unsigned long long *)&Cn[u], (unsigned long long)far) == 0ull;
+            queue_add(add, u, row, Qn, qn_next, cnt, mf, mx);
+        }
         uint64_t heavy = ballot(u >= 0 && !alone);
         while (heavy) {
             const int l = __builtin_ctzll(heavy);
             heavy &= heavy - 1;
             const int v = __shfl(u, l), q0 = __shfl(p0, l), q1 = __shfl(p1, l);
             const uint64_t cv = (uint64_t)__shfl((long long)cu, l);
-            const bool mine = (cv >> lane()) & 1ull;
-            const int dv = mine ? dist[(long long)v * 64 + lane()] : (int)MG_INF;
+            const bool bit = (cv >> lane()) & 1ull;
+            const int dv0 = bit ? dist[(long long)v * 64 + lane()] : (int)MG_INF;
+            const bool mine = bit && dv0 < thr;
+            const int dv = mine ? dv0 : (int)MG_INF;
+            {
+                const uint64_t fv = ballot(bit && !mine);
+                const bool add = lane() == 0 && fv && atomicOr((unsigned long long *)&Cn[v], (unsigned long long)fv) == 0ull;
+                queue_add(add, v, row, Qn, qn_next, cnt, mf, mx);
+            }
             for (int pp = q0; pp < q1; ++pp) {
                 const int w = col[pp];
                 const int nd = dv + wgt[pp];
@@ -669,8 +693,8 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__
     const int cur = (ctl->it - 1) & 1;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    sparse_sssp(ctl->qn[cur], row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1], dist,
-                cnt, mf, mx);
+    sparse_sssp(ctl->qn[cur], ctl->thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
+                dist, cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
 
@@ -751,9 +775,11 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
     int ran = 0;     // levels run by this launch
     BCtl c = c0;
     bool handback = false;
+    int thr = c0.thr;  // the threshold of the last level run
     for (;;) {
         c = c0;
         c.it = L;
+        c.thr = thr;
         c.qvalid[L & 1] = 1;
         c.prev_sparse = 1;
         decide(c, n, mf, mx, a.rule);
@@ -770,8 +796,8 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
             a.pc->mf[ns] = 0;
         }
         if (kSSSP)
-            sparse_sssp((int)n, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA, cur ? a.QA : a.QB,
-                        &a.pc->qn[slot], a.out, cnt, sm, xm);
+            sparse_sssp((int)n, c.thr, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA,
+                        cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
         else
             sparse_bfs(it, (int)n, a.row, a.col, cur ? a.FB : a.FA, cur ? a.FA : a.FB, a.vis, cur ? a.QB : a.QA,
                        cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
@@ -787,6 +813,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
         mf = (long long)__hip_atomic_load((unsigned long long *)&a.pc->mf[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         mx = __hip_atomic_load(&a.pc->mx[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         L = it;
+        thr = c.thr;
         ++ran;
     }
     if (ran == 0) {  // nothing run: the state is the captured graph's
@@ -817,6 +844,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
         o.need_full = handback ? 1 : 0;
         o.nsparse = c0.nsparse + ran;
         o.npersist = c0.npersist + ran;
+        o.thr = thr;
         *a.ctl = o;
     }
 }
@@ -1072,6 +1100,7 @@ struct mg_batch {
     PCtl *pc = nullptr;           // counters of the persistent sparse levels
     int persist_grid = 0;         // workgroups of its cooperative launch (0: not available)
     long long stats[5] = {0, 0, 0, 0, 0};  // of the last BFS / SSSP run (mg_batch_stats)
+    int auto_delta = 0;           // SSSP near-far band by default: a quarter of the mean weight
 };
 
 static void batch_free(mg_batch *b) {
@@ -1133,8 +1162,12 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
               hipMalloc(&b->Q[0], sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->Q[1], sizeof(int) * (nv + 1)) == hipSuccess &&
               hipHostMalloc(&b->h_ctl, sizeof(BCtl), hipHostMallocDefault) == hipSuccess;
     if (ok && kind != MG_BATCH_SSSP) ok = hipMalloc(&b->lev, sizeof(int) * slots) == hipSuccess;
-    if (ok && kind == MG_BATCH_SSSP)
+    if (ok && kind == MG_BATCH_SSSP) {
         ok = hipMalloc(&b->w, sizeof(int) * ne) == hipSuccess && hipMalloc(&b->dist, sizeof(int) * slots) == hipSuccess;
+        double sw = 0.0;
+        for (long long p = 0; p < g->ne; ++p) sw += (double)g->w[p];
+        b->auto_delta = g->ne > 0 ? std::max(1, (int)(sw / (double)g->ne / 4.0)) : 1;
+    }
     if (ok && kind == MG_BATCH_BC) {
         b->log_cap = nv * 8 + 1024;  // grown on demand
         ok = hipMalloc(&b->sigma, sizeof(double) * slots) == hipSuccess && hipMalloc(&b->delta, sizeof(float) * slots) == hipSuccess &&
@@ -1283,7 +1316,8 @@ static LevelRule rule_of(const mg_batch *b) {
     const float alpha = b->opt.alpha > 0 ? b->opt.alpha : 15.0f;
     const float beta = b->opt.beta > 0 ? b->opt.beta : 18.0f;
     const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;
-    return LevelRule{b->ne, b->nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0, chunk};
+    const int delta = b->kind != MG_BATCH_SSSP ? 0 : b->opt.sssp_delta > 0 ? b->opt.sssp_delta : b->opt.sssp_delta < 0 ? 0 : b->auto_delta;
+    return LevelRule{b->ne, b->nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0, chunk, delta};
 }
 
 // BFS (without BC) and SSSP: levels run under the device control block, K at a time

@@ -229,3 +229,25 @@ def test_batch_persistent_levels(mg, orc, kind, persist):
         assert min(persistent) > 0  # the grid tails ran in persistent launches
     if kind == "bfs":
         assert sweeps[0] == max(int(orc.bfs(row, col, s)[orc.bfs(row, col, s) != 0x64646464].max()) for s in srcs) + 1
+
+
+@pytest.mark.parametrize("delta", [-1, 0, 1, 16, 100000])
+@pytest.mark.parametrize("gname", ["grid", "rmat13", "union"])
+@pytest.mark.parametrize("persist", [0, -1], ids=["graph", "persistent"])
+def test_batch_sssp_near_far(mg, orc, delta, gname, persist):
+    """sparse sweeps relax only pairs below a rising distance threshold (the rest wait in
+    the next changed word): distances bit-exact for any band, graph-replayed or
+    persistent levels"""
+    if gname == "union":
+        parts = [mg.Graph.grid(60, 45), mg.Graph.rmat(10, 8, seed=31), mg.Graph.grid(5, 200)]
+        g = mg.Graph.union(parts)
+    else:
+        g = _graphs(mg)[gname]
+    row, col, w = g.csr()
+    src = _sources(g, 64, 23)
+    b = mg.Batch(g, mg.BATCH_SSSP, sssp_delta=delta, persist_levels=persist)
+    b.run(src)
+    d = b.levels()
+    for i, s in enumerate(src):
+        np.testing.assert_array_equal(d[i], orc.sssp_bf(row, col, w, int(s))[0], err_msg=f"delta {delta} source {i}={s}")
+    b.close()
