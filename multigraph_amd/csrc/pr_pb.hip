@@ -5,26 +5,30 @@
 // column y carries rank mass from y (source) to x (destination); it belongs to tile
 // (s, d) = (y / P, x / P).  Inside a tile the edges are sorted by source, and every
 // run of edges with one source shares one message: a message slot stands for a
-// (source vertex, destination partition) pair.  Streams, each tile padded to G = 16:
+// (source vertex, destination partition) pair.  Streams, each tile padded to groups
+// of 8 slots:
 //   a_src (message slots, source-major: s, then d): local source id (16 bit);
-//   amap[g]: for the g-th group of 16 a_src slots, the position of their messages
-//            in the destination-major message array msg;
+//   amap[g]: for the g-th group of a_src slots, the position of their messages in the
+//            destination-major message array msg;
 //   b_dst (edge slots, destination-major: d, then s): local destination id, with bit
 //            15 set where a new message (source) starts;
-//   gbase[g]: message index of the first edge slot of the g-th group of 16.
+//   gbase[g]: message index of the first edge slot of the g-th group;
+//   b_w (SSSP only): per edge slot, its weight (8 or 16 bit).
 //
 // One iteration = two streaming passes, no random HBM access and no global atomics:
 //   scatter: per source partition, stage c = rank / degree of its P vertices in LDS,
 //            stream a_src and write msg[amap position] = c[local source];
-//   gather:  per destination partition, stage the messages of 8192 edge slots at a
-//            time in LDS, stream b_dst and add each edge's message into P FP64 LDS
-//            accumulators (native LDS FP64 atomic add), then apply the PR_T / PR_D
-//            rule to the partition's vertices and write the rank.
+//   gather:  per destination partition, every wave walks 512-slot segments of its
+//            edge stream, stages the segment's messages (at most 520) in its own LDS
+//            slice and adds each edge's message into P FP64 LDS accumulators (native
+//            LDS FP64 atomic add), then the workgroup applies the PR_T / PR_D rule to
+//            the partition's vertices, writes the ranks and (fused scatter) the
+//            partition's messages for the next iteration.
 // Work is cut into units of about equal slot counts; a destination partition whose
 // stream is cut into several units adds its partial sums into an FP64 vector instead
 // and is applied by a separate pass.  Traffic per iteration, with M message slots and
-// S edge slots: 6.25 B * M in the scatter, 4 B * M + 2.25 B * S in the gather, plus
-// 20 B per vertex.  On RMAT graphs M is about half of S.
+// S edge slots: 6.5 B * M in the scatter, 4 B * M + 2.5 B * S in the gather, plus
+// 20 B per vertex.  On RMAT graphs M is a half to three quarters of S.
 #include <algorithm>
 #include <vector>
 #include "engine.hpp"
