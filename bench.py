@@ -78,7 +78,9 @@ def measured_traffic(kernel, scale):
     if scale != 22 or not os.path.exists(path):
         return None
     with open(path) as f:
-        k = json.load(f).get("kernels", {}).get(kernel)
+        ks = json.load(f).get("kernels", {})
+    # the trace names template instantiations ("k_ps_gather<unsigned char>")
+    k = ks.get(kernel) or next((v for n, v in ks.items() if n.split("<")[0] == kernel), None)
     return float(k["bytes"]) if k else None
 
 
