@@ -48,12 +48,13 @@ __global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int
     *ctl = c;
 }
 
-__global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off) {
+__global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf) {
     Ctl c = *ctl;
     if (c.mode == MODE_DONE) return;
     if (lvl_off) {
         lvl_off[c.iter] = c.out_begin;
         lvl_off[c.iter + 1] = c.out_begin + c.out_n;
+        lvl_mf[c.iter] = c.out_mf;  // degree sum of the level (the BC backward pass picks its direction by it)
     }
     c.edges += c.mf;
     advance(c, p);
@@ -773,6 +774,88 @@ __global__ void k_bc_back_fin(const HugeChunk *__restrict__ chunks, int *__restr
     }
 }
 
+// Backward level l by push: the vertices w of level l + 1 add c[w] into an FP64
+// accumulator of each neighbour v one level up (l); k_bc_push_fin then sets
+// delta[v] = sigma[v] * sum.  Chosen over the pull when level l + 1 has far fewer edges
+// than level l (from a hub, the widest level sits above a thin one).  Tiers as in the
+// pull: a lane per vertex below 32 edges, a wave below 2048, kHugeChunk-edge chunks
+// above (k_bc_push_huge).
+__global__ __launch_bounds__(kBlock) void k_bc_push(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
+                                                    const int *__restrict__ row, const int *__restrict__ col,
+                                                    const int *__restrict__ level, const uint8_t *__restrict__ lev8,
+                                                    const double *__restrict__ cw, double *__restrict__ dacc,
+                                                    HugeChunk *__restrict__ chunks, int *__restrict__ nchunks) {
+    const int b = lvl_off[l + 1], n = lvl_off[l + 2] - b;
+    __shared__ int s_wv[kBlock];
+    __shared__ int s_wn;
+    const int up = l;  // the level the contributions go to
+    auto is_up = [&](int v) { return up < 255 ? lev8[v] == (uint8_t)up : level[v] == up; };
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        if (threadIdx.x == 0) s_wn = 0;
+        __syncthreads();
+        const int i = base + threadIdx.x;
+        if (i < n) {
+            const int w = q[b + i];
+            const int rb = row[w], re = row[w + 1], deg = re - rb;
+            if (deg < 32) {
+                const double c = cw[w];
+                for (int p = rb; p < re; ++p) {
+                    const int v = col[p];
+                    if (is_up(v)) atomicAdd(&dacc[v], c);
+                }
+            } else if (deg < 2048) {
+                s_wv[atomicAdd(&s_wn, 1)] = w;
+            } else {
+                const int nch = (deg + kHugeChunk - 1) / kHugeChunk;
+                const int c0 = atomicAdd(nchunks, nch);
+                for (int j = 0; j < nch; ++j) chunks[c0 + j] = HugeChunk{w, j};
+            }
+        }
+        __syncthreads();
+        const int wn = s_wn;
+        for (int k = threadIdx.x >> 6; k < wn; k += kBlock / MG_WAVE) {
+            const int w = s_wv[k];
+            const double c = cw[w];
+            for (int p = row[w] + lane(); p < row[w + 1]; p += MG_WAVE) {
+                const int v = ld_stream(&col[p]);
+                if (is_up(v)) atomicAdd(&dacc[v], c);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bc_push_huge(int l, const int *__restrict__ row, const int *__restrict__ col,
+                                                         const int *__restrict__ level, const uint8_t *__restrict__ lev8,
+                                                         const double *__restrict__ cw, const HugeChunk *__restrict__ chunks,
+                                                         const int *__restrict__ nchunks, double *__restrict__ dacc) {
+    const int nch = *nchunks;
+    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+        const HugeChunk hc = chunks[c];
+        const int w = hc.v;
+        const int b = row[w] + hc.j * kHugeChunk, e = min(row[w + 1], b + kHugeChunk);
+        const double cv = cw[w];
+        for (int p = b + threadIdx.x; p < e; p += kBlock) {
+            const int v = ld_stream(&col[p]);
+            if (l < 255 ? lev8[v] == (uint8_t)l : level[v] == l) atomicAdd(&dacc[v], cv);
+        }
+    }
+}
+
+// level l after a push: delta = sigma * sum, its c for the level above, sums cleared
+__global__ void k_bc_push_fin(int l, const int *__restrict__ lvl_off, const int *__restrict__ q, const double *__restrict__ sigma,
+                              double *__restrict__ dacc, float *__restrict__ delta, double *__restrict__ cw) {
+    const int b = lvl_off[l], n = lvl_off[l + 1] - b;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = q[b + i];
+        const double sv = sigma[v];
+        const float dv = (float)(sv * dacc[v]);
+        delta[v] = dv;
+        cw[v] = (1.0 + (double)dv) / sv;
+        dacc[v] = 0.0;
+    }
+}
+
 __global__ void k_zero_i32(int *p) { *p = 0; }
 
 int grid_for(long long items, int per_block, int cap) {
@@ -861,6 +944,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         MG_HIP(hipMalloc(&e->lev8, padv));
         MG_HIP(hipMalloc(&e->bc_cw, sizeof(double) * padv));
         MG_HIP(hipMalloc(&e->lvl_off, sizeof(int) * ((long long)nv + 4)));
+        MG_HIP(hipMalloc(&e->lvl_mf, sizeof(long long) * ((long long)nv + 4)));
         hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(padv, 256, 4096)), dim3(256), 0, e->stream, e->dacc, padv, 0.0);
     }
     Params &P = e->P;
@@ -870,6 +954,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     // pull levels read a vertex's CSR row as its in-edges, which only a symmetric CSR
     // holds; a directed graph runs push levels only
     P.forced = g->symmetric ? e->opt.mode : MG_MODE_SPARSE;
+    e->symmetric = g->symmetric;
     P.sssp = e->algo == MG_SSSP ? 1 : 0;
     P.need_log = e->algo == MG_BC ? 1 : 0;
     P.huge = e->opt.huge_degree;
@@ -1013,7 +1098,7 @@ static int launch_level(mg_engine *e, bool known) {
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
     }
-    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off);
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf);
     if (!known) MG_TRY(launch_after_ctl(e));
     MG_HIP(hipGetLastError());
     return MG_OK;
@@ -1071,12 +1156,19 @@ int mg_bc_backward_impl(mg_engine *e) {
     hipStream_t s = e->stream;
     const int G = max_grid(e);
     std::vector<int> off(L + 2, 0);
-    // level offsets live on the device (written by the control kernel); level 0 is {source}
+    std::vector<long long> mf(L + 1, 0);
+    // level offsets and degree sums live on the device (written by the control kernel);
+    // level 0 is {source}
     MG_HIP(hipMemcpyAsync(off.data(), e->lvl_off, sizeof(int) * (L + 1), hipMemcpyDeviceToHost, s));
+    if (L > 1) MG_HIP(hipMemcpyAsync(mf.data() + 1, e->lvl_mf + 1, sizeof(long long) * (L - 1), hipMemcpyDeviceToHost, s));
+    int rs[2] = {0, 0};
+    MG_HIP(hipMemcpyAsync(rs, e->row + e->source, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));
+    mf[0] = rs[1] - rs[0];
     off[0] = 0;
     off[1] = 1;
-    MG_HIP(hipMemcpyAsync(e->lvl_off, off.data(), sizeof(int) * 2, hipMemcpyHostToDevice, s));
+    off[L + 1] = off[L];  // the (empty) level below the deepest
+    MG_HIP(hipMemcpyAsync(e->lvl_off, off.data(), sizeof(int) * (L + 2), hipMemcpyHostToDevice, s));
     int *nch = e->aux;  // split-vertex chunk counter
     hipLaunchKernelGGL(k_lev8, dim3(grid_for(e->nv, 256, 4096)), dim3(256), 0, s, e->nv, e->level, e->lev8);
     for (int l = L - 1; l >= 0; --l) {
@@ -1084,6 +1176,25 @@ int mg_bc_backward_impl(mg_engine *e) {
         if (n <= 0) continue;
         int slot = -1;
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, nch);
+        // push from level l + 1 when its edges are fewer than half of level l's (an
+        // atomic per contribution costs about twice a probe); the deepest level has no
+        // successors and is finished by the push's final pass alone.  A push walks the
+        // rows of level l + 1 as in-edges, which needs a symmetric CSR.
+        const long long mf_next = l + 1 < L ? mf[l + 1] : 0;
+        if (e->symmetric && 2 * mf_next < mf[l]) {
+            const int n1 = off[l + 2] - off[l + 1];
+            if (e->prof) mg_prof_begin(e, "k_bc_push", 0.0, &slot);
+            if (n1 > 0) {
+                hipLaunchKernelGGL(k_bc_push, dim3(grid_for(n1, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
+                                   e->level, e->lev8, e->bc_cw, e->dacc, e->chunks, nch);
+                hipLaunchKernelGGL(k_bc_push_huge, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->lev8, e->bc_cw, e->chunks,
+                                   nch, e->dacc);
+            }
+            hipLaunchKernelGGL(k_bc_push_fin, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, l, e->lvl_off, e->q, e->sigma, e->dacc,
+                               e->delta, e->bc_cw);
+            if (e->prof) mg_prof_end(e, slot);
+            continue;
+        }
         if (e->prof) mg_prof_begin(e, "k_bc_back", 0.0, &slot);
         hipLaunchKernelGGL(k_bc_back, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
                            e->level, e->lev8, e->sigma, e->delta, e->bc_cw, e->chunks, nch);
