@@ -109,6 +109,13 @@ int mg_graph_load_mtx(const char *path, int source, int flags, uint64_t weight_s
 /* Same semantics for an in-memory list of 0-based entries (one per MTX line). */
 int mg_graph_from_coo(int nv, int64_t nnz, const int32_t *src, const int32_t *dst,
                       const int32_t *val, int source, int flags, mg_graph **out);
+/* A CSR built by the caller (row[nv + 1] offsets, col, optional weights, 1 when NULL):
+ * ids are kept as given (no compaction), every row is sorted by column with a repeated
+ * column keeping its first entry, and the graph is symmetric when every stored edge
+ * has its reverse.  The reference has no such entry: its graphs come from
+ * generate_CSC only (read_input.cu:11-254); this is the CSR it builds (:151-171). */
+int mg_graph_from_csr(int nv, const int64_t *row, const int32_t *col, const int32_t *weight,
+                      mg_graph **out);
 /* RMAT (a,b,c, d = 1-a-b-c), 2^scale vertices, edge_factor * 2^scale draws, self-loops
  * dropped, weights 1 + hash(weight_seed, min(u,v), max(u,v)) % 64; then read as above. */
 int mg_graph_gen_rmat(int scale, int edge_factor, double a, double b, double c,

@@ -63,6 +63,7 @@ _SIGS = {
     "mg_last_error": (C.c_char_p, []),
     "mg_version": (C.c_char_p, []),
     "mg_graph_load_mtx": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
+    "mg_graph_from_csr": (C.c_int, [C.c_int, C.POINTER(C.c_int64), _I32P, _I32P, C.POINTER(_P)]),
     "mg_graph_from_coo": (C.c_int, [C.c_int, C.c_int64, _I32P, _I32P, _I32P, C.c_int, C.c_int, C.POINTER(_P)]),
     "mg_graph_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
                                     C.c_int, C.POINTER(_P)]),
@@ -175,6 +176,17 @@ class Graph:
         h = C.c_void_p()
         _check(lib().mg_graph_from_coo(nv, len(s), _ptr(s), _ptr(d), _ptr(v) if v is not None else None, source, flags,
                                        C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, row, col, weight=None) -> "Graph":
+        """a CSR built by the caller: ids kept, rows sorted by column, repeats dropped"""
+        r = np.ascontiguousarray(row, dtype=np.int64)
+        c = _i32(col)
+        w = _i32(weight) if weight is not None else None
+        h = C.c_void_p()
+        _check(lib().mg_graph_from_csr(len(r) - 1, r.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(c),
+                                       _ptr(w) if w is not None else None, C.byref(h)))
         return cls(h)
 
     @classmethod

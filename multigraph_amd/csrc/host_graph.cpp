@@ -400,6 +400,61 @@ extern "C" int mg_graph_gen_grid(int width, int height, uint64_t weight_seed, in
     return MG_OK;
 }
 
+// A CSR built by the caller, taken as it is: vertex ids are kept (no compaction), each
+// row is sorted by column (weights carried along) and a repeated column keeps its
+// first entry, as the reader's stable sort does; the graph counts as symmetric when
+// every stored edge has its reverse.  Weights default to 1.
+extern "C" int mg_graph_from_csr(int nv, const int64_t *row, const int32_t *col, const int32_t *w, mg_graph **out) {
+    if (!out || !row || (nv > 0 && row[nv] > 0 && !col)) return mg_fail(MG_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (nv < 0) return mg_fail(MG_ERR_ARG, "negative vertex count");
+    if (row[0] != 0) return mg_fail(MG_ERR_ARG, "row[0] must be 0");
+    for (int v = 0; v < nv; ++v)
+        if (row[v + 1] < row[v]) return mg_fail(MG_ERR_ARG, "row offsets decrease at vertex %d", v);
+    const int64_t nnz = row[nv];
+    if (nnz > INT32_MAX) return mg_fail(MG_ERR_LIMIT, "%lld edges exceed int32 offsets", (long long)nnz);
+    for (int64_t p = 0; p < nnz; ++p)
+        if (col[p] < 0 || col[p] >= nv) return mg_fail(MG_ERR_RANGE, "column %d of edge %lld outside [0,%d)", col[p], (long long)p, nv);
+    mg_graph *g = new mg_graph();
+    g->nv = nv;
+    g->orig_nv = nv;
+    g->row.assign((size_t)nv + 1, 0);
+    g->col.reserve((size_t)nnz);
+    g->w.reserve((size_t)nnz);
+    std::vector<std::pair<int32_t, int32_t>> r;
+    for (int v = 0; v < nv; ++v) {
+        r.clear();
+        for (int64_t p = row[v]; p < row[v + 1]; ++p) r.emplace_back(col[p], w ? w[p] : 1);
+        std::stable_sort(r.begin(), r.end(), [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
+            return a.first < b.first;
+        });
+        for (size_t k = 0; k < r.size(); ++k) {
+            if (k > 0 && r[k].first == r[k - 1].first) continue;
+            g->col.push_back(r[k].first);
+            g->w.push_back(r[k].second);
+        }
+        g->row[v + 1] = (int32_t)g->col.size();
+    }
+    g->ne = (int64_t)g->col.size();
+    g->entries = g->ne;
+    g->orig.resize(nv);
+    g->valid.resize((size_t)nv + 1);
+    for (int v = 0; v < nv; ++v) { g->orig[v] = v; g->valid[v] = v; }
+    g->valid[nv] = nv;
+    g->start_p = 0;
+    g->flags = 0;
+    bool sym = true;
+    for (int v = 0; v < nv && sym; ++v)
+        for (int32_t p = g->row[v]; p < g->row[v + 1] && sym; ++p) {
+            const int32_t u = g->col[p];
+            sym = std::binary_search(g->col.begin() + g->row[u], g->col.begin() + g->row[u + 1], v);
+        }
+    g->symmetric = sym;
+    if (sym) g->flags |= MG_READ_SYM;
+    *out = g;
+    return MG_OK;
+}
+
 extern "C" int mg_graph_union(int count, const mg_graph *const *graphs, mg_graph **out) {
     if (!out || count < 1 || !graphs) return mg_fail(MG_ERR_ARG, "bad union arguments");
     int64_t nv = 0, ne = 0;

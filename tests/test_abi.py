@@ -79,3 +79,39 @@ def test_graph_info_and_csr_views(mg):
     row, col, w = g.csr()
     assert row.dtype == np.int32 and len(row) == nv + 1 and len(col) == ne == len(w)
     np.testing.assert_array_equal(g.orig_ids(), np.arange(12))
+
+
+def test_graph_from_csr(mg):
+    """a caller-built CSR: ids kept, rows sorted by column (weights carried), a repeated
+    column keeps its first entry; bad offsets and columns are refused"""
+    row = np.array([0, 3, 5, 7, 8])
+    col = np.array([2, 1, 2, 0, 3, 1, 0, 1])
+    w = np.array([7, 5, 9, 5, 4, 4, 7, 6])
+    g = mg.Graph.from_csr(row, col, w)
+    r, c, ww = g.csr()
+    assert g.nv == 4
+    np.testing.assert_array_equal(r, [0, 2, 4, 6, 7])
+    np.testing.assert_array_equal(c, [1, 2, 0, 3, 0, 1, 1])
+    np.testing.assert_array_equal(ww, [5, 7, 5, 4, 7, 4, 6])   # row 0: column 2 keeps its first weight, 7
+    g2 = mg.Graph.from_csr([0, 1, 1], [1])                      # unweighted: weight 1
+    np.testing.assert_array_equal(g2.csr()[2], [1])
+    with pytest.raises(mg.MGError) as e:
+        mg.Graph.from_csr([0, 2, 1], [1, 0])
+    assert e.value.code == mg.ERR_ARG
+    with pytest.raises(mg.MGError) as e:
+        mg.Graph.from_csr([0, 1, 2], [1, 5])
+    assert e.value.code == mg.ERR_RANGE
+
+
+def test_graph_from_csr_symmetry(mg):
+    """the symmetric property is detected: batched SSSP refuses a directed graph on the
+    host (before any device call), a symmetric one gets as far as the device"""
+    sym = mg.Graph.from_csr([0, 1, 3, 4], [1, 0, 2, 1])
+    dirg = mg.Graph.from_csr([0, 1, 2, 2], [1, 2])
+    with pytest.raises(mg.MGError) as e:
+        mg.Batch(dirg, mg.BATCH_SSSP)
+    assert e.value.code == mg.ERR_ARG
+    if mg.device_count() == 0:
+        with pytest.raises(mg.MGError) as e:
+            mg.Batch(sym, mg.BATCH_SSSP)
+        assert e.value.code == mg.ERR_DEVICE
