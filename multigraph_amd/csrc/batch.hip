@@ -670,11 +670,24 @@ __device__ __forceinline__ void sparse_sssp(int n, int thr, const int *__restric
     }
 }
 
-// one sparse level from a captured graph (gated on the control block)
+// partials of workgroups [gridDim.x, nparts) are zero: a sparse level runs on a smaller
+// grid than the dense kernels whose partial count the control kernel sums
+__device__ __forceinline__ void zero_rest(int nparts, int *__restrict__ part_n, long long *__restrict__ part_e,
+                                          int *__restrict__ part_x) {
+    for (int i = gridDim.x + blockIdx.x * kBB + threadIdx.x; i < nparts; i += gridDim.x * kBB) {
+        part_n[i] = 0;
+        part_e[i] = 0;
+        part_x[i] = 0;
+    }
+}
+
+// one sparse level from a captured graph (gated on the control block), on a grid of
+// about one workgroup per CU (queues are short: a 2048-workgroup launch would mostly
+// dispatch idle workgroups)
 __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
                                                        uint64_t *FA, uint64_t *FB, uint64_t *vis, int *QA, int *QB,
                                                        int *__restrict__ lev, int *__restrict__ part_n,
-                                                       long long *__restrict__ part_e, int *__restrict__ part_x) {
+                                                       long long *__restrict__ part_e, int *__restrict__ part_x, int nparts) {
     if (ctl->done || !ctl->sparse || ctl->need_full) return;
     const int it = ctl->it;
     const int cur = (it - 1) & 1;
@@ -683,12 +696,13 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__r
     sparse_bfs(it, ctl->qn[cur], row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
                lev, cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
+    zero_rest(nparts, part_n, part_e, part_x);
 }
 
 __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__restrict__ row, const int *__restrict__ col,
                                                         const int *__restrict__ wgt, uint64_t *FA, uint64_t *FB, int *QA, int *QB,
                                                         int *__restrict__ dist, int *__restrict__ part_n,
-                                                        long long *__restrict__ part_e, int *__restrict__ part_x) {
+                                                        long long *__restrict__ part_e, int *__restrict__ part_x, int nparts) {
     if (ctl->done || !ctl->sparse || ctl->need_full) return;
     const int cur = (ctl->it - 1) & 1;
     int cnt = 0, mx = 0;
@@ -696,6 +710,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__
     sparse_sssp(ctl->qn[cur], ctl->thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
                 dist, cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
+    zero_rest(nparts, part_n, part_e, part_x);
 }
 
 // Persistent sparse levels: one cooperative launch (every workgroup resident) runs
@@ -715,6 +730,9 @@ struct PCtl {
     unsigned grp[8 * 16];    // barrier: arrivals per group, one 64 B line each
 };
 
+#ifndef MG_SPARSE_PER_CU
+#define MG_SPARSE_PER_CU 1  // workgroups per CU of a sparse level
+#endif
 #ifndef MG_PERSIST_PER_CU
 #define MG_PERSIST_PER_CU 1  // resident workgroups per CU of the persistent launch
 #endif
@@ -1101,6 +1119,7 @@ struct mg_batch {
     int persist_grid = 0;         // workgroups of its cooperative launch (0: not available)
     long long stats[5] = {0, 0, 0, 0, 0};  // of the last BFS / SSSP run (mg_batch_stats)
     int auto_delta = 0;           // SSSP near-far band by default: a quarter of the mean weight
+    int cus = 256;                // compute units of the device
 };
 
 static void batch_free(mg_batch *b) {
@@ -1178,6 +1197,12 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
              hipMalloc(&b->logw, sizeof(uint64_t) * b->log_cap) == hipSuccess;
     }
     if (!ok) return fail(mg_fail(MG_ERR_NOMEM, "batch: device allocation failed"));
+    {
+        int dev = 0, cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            cu > 0)
+            b->cus = cu;
+    }
     if (kind != MG_BATCH_BC && b->opt.persist_levels != 0) {  // one resident workgroup per CU
         int dev = 0, cus = 0, per = 0;
         const void *fn = kind == MG_BATCH_SSSP ? (const void *)k_mbs_persist<true> : (const void *)k_mbs_persist<false>;
@@ -1333,13 +1358,14 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
     MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const LevelRule rule = rule_of(b);
     const int gq = grid_for(nv, 2048, 2048);
+    const int gsp = std::min(gv, std::max(1, b->cus * MG_SPARSE_PER_CU));
     auto sparse_level = [&]() {
         if (sssp)
-            hipLaunchKernelGGL(k_mbs_level_sssp, dim3(gv), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->w, b->F[0], b->F[1], b->Q[0],
-                               b->Q[1], b->dist, b->part_n, b->part_e, b->part_x);
+            hipLaunchKernelGGL(k_mbs_level_sssp, dim3(gsp), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->w, b->F[0], b->F[1], b->Q[0],
+                               b->Q[1], b->dist, b->part_n, b->part_e, b->part_x, gv);
         else
-            hipLaunchKernelGGL(k_mbs_level_bfs, dim3(gv), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->F[0], b->F[1], b->vis, b->Q[0],
-                               b->Q[1], b->lev, b->part_n, b->part_e, b->part_x);
+            hipLaunchKernelGGL(k_mbs_level_bfs, dim3(gsp), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->F[0], b->F[1], b->vis, b->Q[0],
+                               b->Q[1], b->lev, b->part_n, b->part_e, b->part_x, gv);
     };
     for (int k = 0; k < K; ++k) {
         if (sparse_only) {  // control + one fused kernel per level
