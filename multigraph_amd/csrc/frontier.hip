@@ -1153,6 +1153,7 @@ int mg_frontier_capture(mg_engine *e, int levels, hipGraphExec_t *out) {
 
 int mg_bc_backward_impl(mg_engine *e) {
     const int L = e->levels;  // levels 0 .. L-1 hold vertices
+    if (e->nv == 0 || L < 1) return MG_OK;  // nothing was reached
     hipStream_t s = e->stream;
     const int G = max_grid(e);
     std::vector<int> off(L + 2, 0);
