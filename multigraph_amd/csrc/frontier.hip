@@ -1177,12 +1177,14 @@ int mg_bc_backward_impl(mg_engine *e) {
         if (n <= 0) continue;
         int slot = -1;
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, nch);
-        // push from level l + 1 when its edges are fewer than half of level l's (an
-        // atomic per contribution costs about twice a probe); the deepest level has no
-        // successors and is finished by the push's final pass alone.  A push walks the
-        // rows of level l + 1 as in-edges, which needs a symmetric CSR.
+        // push from level l + 1 when its edges are fewer than an eighth of level l's: a
+        // global FP64 atomic per contribution costs several probes of the pull (on
+        // RMAT-22 from sampled sources, a factor 2 here chose the push on levels where
+        // the pull took half the time; factors 8 and 16 measure alike).  The deepest
+        // level has no successors and is finished by the push's final pass alone.  A
+        // push walks the rows of level l + 1 as in-edges, which needs a symmetric CSR.
         const long long mf_next = l + 1 < L ? mf[l + 1] : 0;
-        if (e->symmetric && 2 * mf_next < mf[l]) {
+        if (e->symmetric && (mf_next == 0 || 8 * mf_next < mf[l])) {
             const int n1 = off[l + 2] - off[l + 1];
             if (e->prof) mg_prof_begin(e, "k_bc_push", 0.0, &slot);
             if (n1 > 0) {
