@@ -51,7 +51,7 @@ static void release_all(mg_engine *e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) hipGraphExecDestroy(e->gexec);
-    void *ptrs[] = {e->pull_ch, e->pull_hv, e->pull_found, e->pull_sum, e->part_n, e->part_mf, e->eoff, e->tsum, e->cstart, e->pcstart, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->lev8, e->bc_cw, e->q, e->bm[0], e->bm[1],
+    void *ptrs[] = {e->pull_ch, e->pull_hv, e->pull_found, e->pull_sum, e->part_n, e->part_mf, e->eoff, e->tsum, e->cstart, e->pcstart, e->row, e->col, e->w, e->trow, e->tcol, e->tw, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->lev8, e->bc_cw, e->q, e->bm[0], e->bm[1],
                     e->vis, e->chunks, e->lvl_off, e->lvl_mf, e->aux, e->ctl, e->urow, e->ucol, e->rank, e->cval, e->deg, e->psum,
                     e->pb.a_src, e->pb.b_dst, e->pb.amap, e->pb.msg, e->pb.gbase, e->pb.unitsA,
                     e->pb.unitsB, e->pb.split_parts, e->pb.apart, e->pb.b_w, e->pb.inv, e->pb.unitsS};

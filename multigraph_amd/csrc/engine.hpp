@@ -155,6 +155,12 @@ struct mg_engine {
     hipStream_t stream = nullptr;
     // CSR on the device (the renumbered CSR of HYB: _ncsc_v/_ncsc_e/_ncsc_ev)
     int *row = nullptr, *col = nullptr, *w = nullptr;
+    // in-edges of a directed graph (the transposed CSR, rows sorted by source, weights
+    // carried along) for pull levels; null on a symmetric graph, whose rows are both
+    int *trow = nullptr, *tcol = nullptr, *tw = nullptr;
+    const int *in_row() const { return trow ? trow : row; }
+    const int *in_col() const { return trow ? tcol : col; }
+    const int *in_w() const { return trow ? tw : w; }
     // run state (struct vector_data)
     int *level = nullptr;      // BFS/BC level, SSSP distance, CC label (vv)
     int *vfc = nullptr;        // SSSP enqueue stamp / PR active flag

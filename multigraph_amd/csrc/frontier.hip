@@ -310,12 +310,14 @@ __global__ __launch_bounds__(kBlock) void k_level2bm(Ctl *ctl, int nv, const int
 __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[u >> 6] >> (u & 63)) & 1ull; }
 
 // Pull level for BFS (early exit at the first parent) and for the BC forward pass
-// (all parents, summing their path counts).
+// (all parents, summing their path counts).  row / col are the in-edges; orow, the
+// out-edge rows of a directed graph (null when symmetric), gives the degree sum.
 // The new frontier is the bitmap itself; its size and degree sum leave as one
 // partial per workgroup (no global atomics), summed by k_reduce_parts.
 template <bool kBC>
 __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__restrict__ row,
-                                                 const int *__restrict__ col, int *__restrict__ level,
+                                                 const int *__restrict__ col, const int *__restrict__ orow,
+                                                 int *__restrict__ level,
                                                  double *__restrict__ sigma, uint64_t *__restrict__ vis,
                                                  uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
                                                  int *__restrict__ part_n, long long *__restrict__ part_mf) {
@@ -333,10 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
         const int v = (w << 6) + lane();
         bool found = false;
         double s = 0.0;
-        int b = 0, e = 0;
+        int b = 0, e = 0, deg = 0;
         if (vw != ~0ull && !((vw >> lane()) & 1ull)) {
             b = row[v];
             e = row[v + 1];
+            deg = e - b;
             if (e - b > kPullHuge) e = b;  // left to k_pull_huge
             if (!kBC) {
                 for (int p = b; p < e; ++p)
@@ -380,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
             level[v] = iter;
             if (kBC) sigma[v] = s;
             cnt += 1;
-            mf += e - b;
+            mf += orow ? orow[v + 1] - orow[v] : deg;  // out-degree: the next level's push edges
         }
     }
     __shared__ long long s_tmp[kBlock / MG_WAVE];
@@ -888,6 +891,33 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
     MG_HIP(hipMalloc(&e->aux, sizeof(int) * 16));
     MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    // a directed graph's pull levels read in-edges: transpose the CSR once (counting
+    // sort by destination; scanning sources in order leaves every row sorted by source)
+    std::vector<int32_t> hrow_t, hcol_t;
+    const int32_t *irow = g->row.data();
+    if (!g->symmetric && nv > 0) {
+        hrow_t.assign((size_t)nv + 1, 0);
+        hcol_t.resize((size_t)ne);
+        std::vector<int32_t> hw_t(e->algo == MG_SSSP ? (size_t)ne : 0);
+        for (long long p = 0; p < ne; ++p) ++hrow_t[(size_t)g->col[p] + 1];
+        for (int v = 0; v < nv; ++v) hrow_t[v + 1] += hrow_t[v];
+        std::vector<int32_t> cur(hrow_t.begin(), hrow_t.end() - 1);
+        for (int u = 0; u < nv; ++u)
+            for (int p = g->row[u]; p < g->row[u + 1]; ++p) {
+                const int32_t q = cur[g->col[p]]++;
+                hcol_t[q] = u;
+                if (!hw_t.empty()) hw_t[q] = g->w[p];
+            }
+        MG_HIP(hipMalloc(&e->trow, sizeof(int) * ((long long)nv + 1)));
+        MG_HIP(hipMalloc(&e->tcol, sizeof(int) * (ne > 0 ? ne : 1)));
+        MG_HIP(hipMemcpy(e->trow, hrow_t.data(), sizeof(int) * ((long long)nv + 1), hipMemcpyHostToDevice));
+        if (ne > 0) MG_HIP(hipMemcpy(e->tcol, hcol_t.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+        if (!hw_t.empty()) {
+            MG_HIP(hipMalloc(&e->tw, sizeof(int) * ne));
+            MG_HIP(hipMemcpy(e->tw, hw_t.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+        }
+        irow = hrow_t.data();
+    }
     if (e->algo == MG_SSSP) {
         MG_HIP(hipMalloc(&e->vfc, sizeof(int) * padv));
         const long long words = (nv + 63) / 64 + 1;
@@ -895,7 +925,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         e->pnch = (int)((ne + kEPB - 1) / kEPB);
         MG_HIP(hipMalloc(&e->pcstart, sizeof(int) * (e->pnch + 2)));
         if (e->pnch > 0)
-            hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(e->pnch + 1, 256, 4096)), dim3(256), 0, e->stream, nv, ne, e->row,
+            hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(e->pnch + 1, 256, 4096)), dim3(256), 0, e->stream, nv, ne, e->in_row(),
                                e->pnch, e->pcstart);
         // dense sweeps over the partitioned layout (symmetric graphs, weights in 16 bits),
         // else over the CSR
@@ -919,7 +949,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         std::vector<PullChunk> pc;
         std::vector<int> hv;
         for (int v = 0; v < nv; ++v) {
-            const int b = g->row[v], d = g->row[v + 1] - b;
+            const int b = irow[v], d = irow[v + 1] - b;
             if (d <= kPullHuge) continue;
             for (int p = b; p < b + d; p += kPullChunk) pc.push_back(PullChunk{v, p, std::min(b + d, p + kPullChunk), (int)hv.size()});
             hv.push_back(v);
@@ -951,9 +981,8 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     P.nv = nv;
     P.ne = ne;
     P.log_queue = (e->algo == MG_SSSP) ? 0 : 1;
-    // pull levels read a vertex's CSR row as its in-edges, which only a symmetric CSR
-    // holds; a directed graph runs push levels only
-    P.forced = g->symmetric ? e->opt.mode : MG_MODE_SPARSE;
+    // pull levels read in-edges: the CSR itself when symmetric, else its transpose
+    P.forced = e->opt.mode;
     e->symmetric = g->symmetric;
     P.sssp = e->algo == MG_SSSP ? 1 : 0;
     P.need_log = e->algo == MG_BC ? 1 : 0;
@@ -1067,8 +1096,8 @@ static int launch_level(mg_engine *e, bool known) {
             MG_TRY(mg_ps_sweep(e));
         } else {
         if (e->prof) mg_prof_begin(e, "k_pull_sssp", bytes, &slot);
-        hipLaunchKernelGGL(k_pull_sssp, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->row,
-                           e->col, e->w, e->pcstart, e->level, e->vfc);
+        hipLaunchKernelGGL(k_pull_sssp, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->in_row(),
+                           e->in_col(), e->in_w(), e->pcstart, e->level, e->vfc);
         if (e->prof) mg_prof_end(e, slot);
         }
         hipLaunchKernelGGL(k_sssp_scan, dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, e->row, e->bm[0], e->part_n,
@@ -1079,17 +1108,18 @@ static int launch_level(mg_engine *e, bool known) {
     if (bu) {
         const double bytes = known ? (8.0 * nv / 64.0 * 3 + 4.0 * (double)c.mu + 8.0 * nv) : 0.0;
         if (e->prof) mg_prof_begin(e, "k_pull", bytes, &slot);
+        const int *orow = e->trow ? e->row : nullptr;
         if (e->algo == MG_BC)
-            hipLaunchKernelGGL((k_pull<true>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            hipLaunchKernelGGL((k_pull<true>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         else
-            hipLaunchKernelGGL((k_pull<false>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->row, e->col, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            hipLaunchKernelGGL((k_pull<false>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         if (e->pull_nch > 0) {
             const int gh = grid_for(e->pull_nch, kBlock / MG_WAVE, G);
             if (e->algo == MG_BC)
-                hipLaunchKernelGGL((k_pull_huge<true>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->col, e->sigma,
+                hipLaunchKernelGGL((k_pull_huge<true>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
                                    e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
             else
-                hipLaunchKernelGGL((k_pull_huge<false>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->col, e->sigma,
+                hipLaunchKernelGGL((k_pull_huge<false>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
                                    e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
             hipLaunchKernelGGL(k_pull_huge_fin, dim3(grid_for(e->pull_nh, kBlock, 64)), dim3(kBlock), 0, s, e->ctl, e->pull_nh,
                                e->pull_hv, e->row, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum,
@@ -1182,16 +1212,17 @@ int mg_bc_backward_impl(mg_engine *e) {
         // RMAT-22 from sampled sources, a factor 2 here chose the push on levels where
         // the pull took half the time; factors 8 and 16 measure alike).  The deepest
         // level has no successors and is finished by the push's final pass alone.  A
-        // push walks the rows of level l + 1 as in-edges, which needs a symmetric CSR.
+        // push walks the in-edges of level l + 1 (the transposed CSR of a directed
+        // graph, whose degree sums here are then out-degrees: an estimate).
         const long long mf_next = l + 1 < L ? mf[l + 1] : 0;
-        if (e->symmetric && (mf_next == 0 || 8 * mf_next < mf[l])) {
+        if (mf_next == 0 || 8 * mf_next < mf[l]) {
             const int n1 = off[l + 2] - off[l + 1];
             if (e->prof) mg_prof_begin(e, "k_bc_push", 0.0, &slot);
             if (n1 > 0) {
-                hipLaunchKernelGGL(k_bc_push, dim3(grid_for(n1, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
-                                   e->level, e->lev8, e->bc_cw, e->dacc, e->chunks, nch);
-                hipLaunchKernelGGL(k_bc_push_huge, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->lev8, e->bc_cw, e->chunks,
-                                   nch, e->dacc);
+                hipLaunchKernelGGL(k_bc_push, dim3(grid_for(n1, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->in_row(),
+                                   e->in_col(), e->level, e->lev8, e->bc_cw, e->dacc, e->chunks, nch);
+                hipLaunchKernelGGL(k_bc_push_huge, dim3(G), dim3(kBlock), 0, s, l, e->in_row(), e->in_col(), e->level, e->lev8, e->bc_cw,
+                                   e->chunks, nch, e->dacc);
             }
             hipLaunchKernelGGL(k_bc_push_fin, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, l, e->lvl_off, e->q, e->sigma, e->dacc,
                                e->delta, e->bc_cw);

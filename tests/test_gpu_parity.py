@@ -177,16 +177,54 @@ def test_directed_bfs_sssp(mg, orc, directed, mode):
     es.close()
 
 
-def test_directed_bc_cc_pr(mg, orc, directed):
+@pytest.mark.parametrize("mode", MODES, ids=[m[0] for m in MODES])
+def test_directed_bc(mg, orc, directed, mode):
+    """pull levels of a directed graph read the transposed CSR (forward) and push the
+    backward pass along in-edges"""
     g, row, col, w = directed
-    e = mg.Engine(g, mg.BC)
+    e = mg.Engine(g, mg.BC, **mode[1])
     for s in sources(g):
+        print(f"directed bc mode={mode[0]} src={s}", flush=True)
         e.run(s)
         lv, sg, dl = orc.bc_fp64(row, col, s)
         np.testing.assert_array_equal(e.result(mg.RES_LEVEL), lv)
         np.testing.assert_array_equal(e.result(mg.RES_SIGMA), sg)
         np.testing.assert_allclose(e.result(mg.RES_DELTA).astype(np.float64), dl, rtol=1e-6, atol=1e-6)
     e.close()
+
+
+def fan_in(mg, n=5000):
+    """0 -> 1..n -> n+1 -> n+2 (directed): the hub n+1 has n in-edges, above the pull
+    split threshold, and one out-edge; weights vary per edge"""
+    src = np.concatenate([np.zeros(n, np.int32), np.arange(1, n + 1, dtype=np.int32), [n + 1]]).astype(np.int32)
+    dst = np.concatenate([np.arange(1, n + 1, dtype=np.int32), np.full(n, n + 1, np.int32), [n + 2]]).astype(np.int32)
+    val = (1 + (np.arange(len(src)) * 7919) % 64).astype(np.int32)
+    return mg.Graph.from_coo(n + 3, src, dst, val, flags=0)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=[m[0] for m in MODES])
+def test_directed_fan_in(mg, orc, mode):
+    g = fan_in(mg)
+    row, col, w = g.csr()
+    for algo in (mg.BFS, mg.SSSP, mg.BC):
+        e = mg.Engine(g, algo, **mode[1])
+        for s in (0, 1, g.nv - 2):
+            print(f"fan-in algo={algo} mode={mode[0]} src={s}", flush=True)
+            e.run(s)
+            if algo == mg.BFS:
+                np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.bfs(row, col, s))
+            elif algo == mg.SSSP:
+                np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.sssp_bf(row, col, w, s)[0])
+            else:
+                lv, sg, dl = orc.bc_fp64(row, col, s)
+                np.testing.assert_array_equal(e.result(mg.RES_LEVEL), lv)
+                np.testing.assert_array_equal(e.result(mg.RES_SIGMA), sg)
+                np.testing.assert_allclose(e.result(mg.RES_DELTA).astype(np.float64), dl, rtol=1e-6, atol=1e-6)
+        e.close()
+
+
+def test_directed_cc_pr(mg, orc, directed):
+    g, row, col, w = directed
     e = mg.Engine(g, mg.CC)
     e.run(0)
     np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.cc_unionfind(row, col)[0])
