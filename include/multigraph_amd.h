@@ -112,7 +112,7 @@ int mg_graph_from_coo(int nv, int64_t nnz, const int32_t *src, const int32_t *ds
 /* A CSR built by the caller (row[nv + 1] offsets, col, optional weights, 1 when NULL):
  * ids are kept as given (no compaction), every row is sorted by column with a repeated
  * column keeping its first entry, and the graph is symmetric when every stored edge
- * has its reverse.  The reference has no such entry: its graphs come from
+ * has its reverse with the same weight.  The reference has no such entry: its graphs come from
  * generate_CSC only (read_input.cu:11-254); this is the CSR it builds (:151-171). */
 int mg_graph_from_csr(int nv, const int64_t *row, const int32_t *col, const int32_t *weight,
                       mg_graph **out);
@@ -127,6 +127,11 @@ int mg_graph_gen_grid(int width, int height, uint64_t weight_seed, int flags, mg
 int mg_graph_union(int count, const mg_graph *const *graphs, mg_graph **out);
 int mg_graph_write_mtx(const mg_graph *g, const char *path);
 int mg_graph_info(const mg_graph *g, int32_t *nv, int64_t *ne, int32_t *start_p);
+/* 1 when every stored edge has its reverse with the same weight (the reader's -DSYM
+ * graphs): rows double as in-edges; else the engines also keep the transposed CSR for
+ * pull levels.  No reference counterpart (its -DNSYM path is unfinished,
+ * read_input.cu:42-44). */
+int mg_graph_symmetric(const mg_graph *g, int *symmetric);
 int mg_graph_csr(const mg_graph *g, const int32_t **row, const int32_t **col,
                  const int32_t **weight);
 /* Original (pre-compaction) id of every compacted vertex, length nv. */

@@ -71,6 +71,7 @@ _SIGS = {
     "mg_graph_union": (C.c_int, [C.c_int, C.POINTER(_P), C.POINTER(_P)]),
     "mg_graph_write_mtx": (C.c_int, [_P, C.c_char_p]),
     "mg_graph_info": (C.c_int, [_P, _I32P, C.POINTER(C.c_int64), _I32P]),
+    "mg_graph_symmetric": (C.c_int, [_P, _I32P]),
     "mg_graph_csr": (C.c_int, [_P, C.POINTER(_I32P), C.POINTER(_I32P), C.POINTER(_I32P)]),
     "mg_graph_orig_ids": (C.c_int, [_P, C.POINTER(_I32P)]),
     "mg_graph_remap": (C.c_int, [_P, C.c_int, _I32P]),
@@ -214,6 +215,13 @@ class Graph:
         nv, ne, sp = C.c_int32(), C.c_int64(), C.c_int32()
         _check(lib().mg_graph_info(self._h, C.byref(nv), C.byref(ne), C.byref(sp)))
         return nv.value, ne.value, sp.value
+
+    @property
+    def symmetric(self) -> bool:
+        """every stored edge has its reverse with the same weight"""
+        v = C.c_int32()
+        _check(lib().mg_graph_symmetric(self._h, C.byref(v)))
+        return bool(v.value)
 
     @property
     def nv(self) -> int:

@@ -385,7 +385,7 @@ __global__ void k_mb_ctl_init(BCtl *ctl, unsigned long long all) {
 // whose vertices plus edges are below (nv + E) / beta and whose vertices all fit one
 // chunk, so per-level work follows the frontier, not the graph; high-diameter members
 // of an uneven batch run thousands of such levels)
-struct LevelRule { long long ne; int nv; float alpha, beta; int forced, symmetric, light, delta; };
+struct LevelRule { long long ne; int nv; float alpha, beta; int forced, light, delta; };
 
 __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, long long x, const LevelRule &r) {
     c.n = a;
@@ -397,7 +397,6 @@ __device__ __forceinline__ void decide(BCtl &c, long long a, long long b, long l
     int push = (double)b < (double)r.ne / r.alpha;
     if (r.forced == MG_MODE_SPARSE) push = 1;
     if (r.forced == MG_MODE_DENSE) push = 0;
-    if (!r.symmetric) push = 1;
     c.push = push;
     const int cur = (c.it - 1) & 1, next = c.it & 1;
     const int sparse = r.forced != MG_MODE_DENSE && push && (double)(a + b) * r.beta < (double)r.nv + (double)r.ne && x <= r.light;
@@ -601,7 +600,10 @@ __device__ __forceinline__ void sparse_sssp(int n, int thr, const int *__restric
         uint64_t cu = 0;
         if (i < n) { u = Q[i]; cu = C[u]; p0 = row[u]; p1 = row[u + 1]; }
         const bool alone = u >= 0 && __builtin_popcountll(cu) * (p1 - p0) <= kLaneWork;
-        uint64_t rem = alone ? cu : 0ull;  // bits still to walk; the lowest is the current source
+        // bits still to walk, the lowest being the current source; a vertex without
+        // out-edges (a sink of a directed graph, an isolated id) has nothing to relax,
+        // and walking it would step into the next vertex's row
+        uint64_t rem = alone && p1 > p0 ? cu : 0ull;
         uint64_t far = 0ull;               // bits left for a later sweep
         if (thr != 0x7FFFFFFF) {
             for (uint64_t t = rem; t; t &= t - 1) {
@@ -1104,13 +1106,22 @@ struct mg_batch {
     float *cw = nullptr;          // BC: (1 + delta) / sigma of the finished levels
     Chunk *chunks = nullptr;      // static edge chunks
     long long nch = 0;
+    // a directed graph's in-edges for pull levels (transposed CSR, its own chunks);
+    // null on a symmetric graph, whose rows serve both directions
+    Chunk *tchunks = nullptr;
+    long long ntch = 0;
+    int *tcol = nullptr, *tw = nullptr;
+    const Chunk *in_chunks() const { return tchunks ? tchunks : chunks; }
+    long long in_nch() const { return tchunks ? ntch : nch; }
+    const int *in_col() const { return tchunks ? tcol : col; }
+    const int *in_w() const { return tchunks ? tw : w; }
     int *logv = nullptr;          // (vertex, word) entries of every level (BC)
     uint64_t *logw = nullptr;
     long long log_cap = 0;
     int *part_n = nullptr, *part_x = nullptr;  // per-workgroup frontier size, largest degree
     long long *part_e = nullptr, *tot = nullptr, *h_tot = nullptr;
     int nsrc = 0, levels = 0;     // slots of the last run, levels it swept
-    bool symmetric = true;        // pull levels need every edge stored both ways
+    bool symmetric = true;        // every edge stored both ways (no transposed CSR)
     BCtl *ctl = nullptr, *h_ctl = nullptr;  // device level control (BFS, SSSP)
     hipGraphExec_t gexec = nullptr;         // K captured levels (all level kinds)
     hipGraphExec_t gsparse = nullptr;       // K captured sparse levels (control + fused kernel)
@@ -1128,7 +1139,8 @@ static void batch_free(mg_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
-                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->cw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc};
+                    b->delta, b->bc, b->chunks, b->logv, b->logw, b->cw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc,
+                    b->tchunks, b->tcol, b->tw};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1141,8 +1153,6 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     *out = nullptr;
     if (kind < MG_BATCH_BFS || kind > MG_BATCH_SSSP) return mg_fail(MG_ERR_ARG, "unknown batch kind %d", kind);
     if (max_sources < 1 || max_sources > 64) return mg_fail(MG_ERR_LIMIT, "a batch holds 1..64 sources (asked %d)", max_sources);
-    if (kind == MG_BATCH_SSSP && !g->symmetric)
-        return mg_fail(MG_ERR_ARG, "batched SSSP sweeps pull over in-edges and needs a symmetric graph");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return mg_fail(MG_ERR_DEVICE, "no HIP device available");
     mg_batch *b = new mg_batch();
@@ -1157,15 +1167,37 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "stream"));
     // static edge chunks (a hub's edges are cut into pieces of at most `chunk`)
     const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;
-    std::vector<Chunk> ch;
-    ch.reserve((size_t)g->nv + (size_t)(g->ne / chunk) + 1);
-    for (int v = 0; v < g->nv; ++v) {
-        const int p0 = g->row[v], p1 = g->row[v + 1];
-        if (p1 == p0) continue;
-        const int multi = (p1 - p0) > chunk ? 1 : 0;
-        for (int p = p0; p < p1; p += chunk) ch.push_back(Chunk{v, p, std::min(p1, p + chunk), multi});
-    }
+    auto cut = [&](const int32_t *row, std::vector<Chunk> &ch) {
+        ch.reserve((size_t)g->nv + (size_t)(g->ne / chunk) + 1);
+        for (int v = 0; v < g->nv; ++v) {
+            const int p0 = row[v], p1 = row[v + 1];
+            if (p1 == p0) continue;
+            const int multi = (p1 - p0) > chunk ? 1 : 0;
+            for (int p = p0; p < p1; p += chunk) ch.push_back(Chunk{v, p, std::min(p1, p + chunk), multi});
+        }
+    };
+    std::vector<Chunk> ch, tch;
+    cut(g->row.data(), ch);
     b->nch = (long long)ch.size();
+    // directed: pull levels read in-edges from the transposed CSR (counting sort by
+    // destination; scanning sources in order leaves every row sorted by source)
+    std::vector<int32_t> trow, tcol, tw;
+    if (!g->symmetric && g->nv > 0) {
+        trow.assign((size_t)g->nv + 1, 0);
+        tcol.resize((size_t)g->ne);
+        if (kind == MG_BATCH_SSSP) tw.resize((size_t)g->ne);
+        for (long long p = 0; p < g->ne; ++p) ++trow[(size_t)g->col[p] + 1];
+        for (int v = 0; v < g->nv; ++v) trow[v + 1] += trow[v];
+        std::vector<int32_t> cur(trow.begin(), trow.end() - 1);
+        for (int u = 0; u < g->nv; ++u)
+            for (int p = g->row[u]; p < g->row[u + 1]; ++p) {
+                const int32_t q = cur[g->col[p]]++;
+                tcol[q] = u;
+                if (!tw.empty()) tw[q] = g->w[p];
+            }
+        cut(trow.data(), tch);
+        b->ntch = (long long)tch.size();
+    }
     const long long nv = b->nv, ne = std::max<long long>(1, b->ne), slots = nv * 64 + 64;
     bool ok = hipMalloc(&b->row, sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->col, sizeof(int) * ne) == hipSuccess &&
               hipMalloc(&b->chunks, sizeof(Chunk) * std::max<long long>(1, b->nch)) == hipSuccess &&
@@ -1186,6 +1218,15 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
         double sw = 0.0;
         for (long long p = 0; p < g->ne; ++p) sw += (double)g->w[p];
         b->auto_delta = g->ne > 0 ? std::max(1, (int)(sw / (double)g->ne / 4.0)) : 1;
+    }
+    if (ok && !trow.empty()) {
+        ok = hipMalloc(&b->tchunks, sizeof(Chunk) * std::max<long long>(1, b->ntch)) == hipSuccess &&
+             hipMalloc(&b->tcol, sizeof(int) * ne) == hipSuccess &&
+             (tw.empty() || hipMalloc(&b->tw, sizeof(int) * ne) == hipSuccess);
+        if (ok && ((b->ntch > 0 && hipMemcpy(b->tchunks, tch.data(), sizeof(Chunk) * b->ntch, hipMemcpyHostToDevice) != hipSuccess) ||
+                   (g->ne > 0 && hipMemcpy(b->tcol, tcol.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess) ||
+                   (b->tw && g->ne > 0 && hipMemcpy(b->tw, tw.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess)))
+            return fail(mg_fail(MG_ERR_DEVICE, "batch: upload of the transposed CSR failed"));
     }
     if (ok && kind == MG_BATCH_BC) {
         b->log_cap = nv * 8 + 1024;  // grown on demand
@@ -1278,6 +1319,7 @@ static int run_bfs_bc(mg_batch *b) {
     const long long slots = (long long)nv * 64 + 64;
     const int gv = grid_for(nv, kBB, G);
     const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
+    const int gci = grid_for(b->in_nch(), kBB / MG_WAVE, G);  // pulls walk in-edge chunks
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[0], (long long)nv + 1, 0ull);
     hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->lev, slots, (int)MG_INF);
@@ -1300,14 +1342,13 @@ static int run_bfs_bc(mg_batch *b) {
         bool push = (double)mf < (double)b->ne / alpha;
         if (b->opt.mode == MG_MODE_SPARSE) push = true;
         if (b->opt.mode == MG_MODE_DENSE) push = false;
-        if (!b->symmetric) push = true;
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, Fn, (long long)nv + 1, 0ull);
         if (push) {
             if (bc) hipLaunchKernelGGL((k_mb_push<true>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
             else hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
         } else {
-            if (bc) hipLaunchKernelGGL(k_mb_pull_bc, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, all, b->col, F, b->vis, Fn, b->sigma);
-            else hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, all, b->col, F, b->vis, Fn);
+            if (bc) hipLaunchKernelGGL(k_mb_pull_bc, dim3(gci), dim3(kBB), 0, s, b->in_nch(), b->in_chunks(), all, b->in_col(), F, b->vis, Fn, b->sigma);
+            else hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gci), dim3(kBB), 0, s, (const BCtl *)nullptr, b->in_nch(), b->in_chunks(), all, b->in_col(), F, b->vis, Fn);
         }
         hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->vis, F, Fn, b->lev, it, b->part_n, b->part_e, b->part_x);
         MG_TRY(read_counts(b, gv, &n, &mf));
@@ -1342,7 +1383,7 @@ static LevelRule rule_of(const mg_batch *b) {
     const float beta = b->opt.beta > 0 ? b->opt.beta : 18.0f;
     const int chunk = b->opt.batch_chunk > 0 ? std::max(64, b->opt.batch_chunk) : kCH;
     const int delta = b->kind != MG_BATCH_SSSP ? 0 : b->opt.sssp_delta > 0 ? b->opt.sssp_delta : b->opt.sssp_delta < 0 ? 0 : b->auto_delta;
-    return LevelRule{b->ne, b->nv, alpha, beta, b->opt.mode, b->symmetric ? 1 : 0, chunk, delta};
+    return LevelRule{b->ne, b->nv, alpha, beta, b->opt.mode, chunk, delta};
 }
 
 // BFS (without BC) and SSSP: levels run under the device control block, K at a time
@@ -1353,6 +1394,7 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
     const int G = b->opt.grid_blocks > 0 ? b->opt.grid_blocks : 2048;
     const int gv = grid_for(nv, kBB, G);
     const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
+    const int gci = grid_for(b->in_nch(), kBB / MG_WAVE, G);  // pulls walk in-edge chunks
     const bool sssp = b->kind == MG_BATCH_SSSP;
     hipGraph_t g = nullptr;
     MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1383,15 +1425,15 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
         if (sssp) {
             hipLaunchKernelGGL(k_mb_sssp_push, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col, b->w,
                                b->F[0], b->F[1], b->dist);
-            hipLaunchKernelGGL(k_mb_sssp, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col, b->w, b->F[0],
-                               b->F[1], b->dist);
+            hipLaunchKernelGGL(k_mb_sssp, dim3(gci), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->in_nch(), b->in_chunks(), b->in_col(),
+                               b->in_w(), b->F[0], b->F[1], b->dist);
             hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->F[0], b->F[1], b->part_n,
                                b->part_e, b->part_x);
         } else {
             hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col,
                                b->F[0], b->vis, b->F[1], (double *)nullptr);
-            hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, 0ull, b->col,
-                               b->F[0], b->vis, b->F[1]);
+            hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gci), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->in_nch(), b->in_chunks(), 0ull,
+                               b->in_col(), b->F[0], b->vis, b->F[1]);
             hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->vis, b->F[0], b->F[1],
                                b->lev, 0, b->part_n, b->part_e, b->part_x);
         }

@@ -403,7 +403,7 @@ extern "C" int mg_graph_gen_grid(int width, int height, uint64_t weight_seed, in
 // A CSR built by the caller, taken as it is: vertex ids are kept (no compaction), each
 // row is sorted by column (weights carried along) and a repeated column keeps its
 // first entry, as the reader's stable sort does; the graph counts as symmetric when
-// every stored edge has its reverse.  Weights default to 1.
+// every stored edge has its reverse, with the same weight.  Weights default to 1.
 extern "C" int mg_graph_from_csr(int nv, const int64_t *row, const int32_t *col, const int32_t *w, mg_graph **out) {
     if (!out || !row || (nv > 0 && row[nv] > 0 && !col)) return mg_fail(MG_ERR_ARG, "null argument");
     *out = nullptr;
@@ -443,11 +443,15 @@ extern "C" int mg_graph_from_csr(int nv, const int64_t *row, const int32_t *col,
     g->valid[nv] = nv;
     g->start_p = 0;
     g->flags = 0;
+    // symmetric: every edge's reverse is stored with the same weight (pull levels read
+    // a row as in-edges, SSSP with their weights)
     bool sym = true;
     for (int v = 0; v < nv && sym; ++v)
         for (int32_t p = g->row[v]; p < g->row[v + 1] && sym; ++p) {
             const int32_t u = g->col[p];
-            sym = std::binary_search(g->col.begin() + g->row[u], g->col.begin() + g->row[u + 1], v);
+            const auto b = g->col.begin() + g->row[u], e = g->col.begin() + g->row[u + 1];
+            const auto it = std::lower_bound(b, e, v);
+            sym = it != e && *it == v && g->w[(size_t)(it - g->col.begin())] == g->w[(size_t)p];
         }
     g->symmetric = sym;
     if (sym) g->flags |= MG_READ_SYM;
@@ -503,6 +507,12 @@ extern "C" int mg_graph_info(const mg_graph *g, int32_t *nv, int64_t *ne, int32_
     if (nv) *nv = g->nv;
     if (ne) *ne = g->ne;
     if (start_p) *start_p = g->start_p;
+    return MG_OK;
+}
+
+extern "C" int mg_graph_symmetric(const mg_graph *g, int *symmetric) {
+    if (!g || !symmetric) return mg_fail(MG_ERR_ARG, "null argument");
+    *symmetric = g->symmetric ? 1 : 0;
     return MG_OK;
 }
 

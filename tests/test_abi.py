@@ -57,8 +57,6 @@ def test_null_and_range_errors(mg):
     assert L.mg_build(g._h, 17, None, C.byref(h)) == mg.ERR_ARG
     assert L.mg_batch_build(g._h, 9, 64, None, C.byref(h)) == mg.ERR_ARG
     assert L.mg_batch_build(g._h, mg.BATCH_BFS, 65, None, C.byref(h)) == mg.ERR_LIMIT
-    gd = mg.Graph.rmat(6, 4, seed=1, flags=0)
-    assert L.mg_batch_build(gd._h, mg.BATCH_SSSP, 64, None, C.byref(h)) == mg.ERR_ARG
     with pytest.raises(mg.MGError):
         g.remap(1 << 20)
 
@@ -104,14 +102,12 @@ def test_graph_from_csr(mg):
 
 
 def test_graph_from_csr_symmetry(mg):
-    """the symmetric property is detected: batched SSSP refuses a directed graph on the
-    host (before any device call), a symmetric one gets as far as the device"""
-    sym = mg.Graph.from_csr([0, 1, 3, 4], [1, 0, 2, 1])
-    dirg = mg.Graph.from_csr([0, 1, 2, 2], [1, 2])
-    with pytest.raises(mg.MGError) as e:
-        mg.Batch(dirg, mg.BATCH_SSSP)
-    assert e.value.code == mg.ERR_ARG
-    if mg.device_count() == 0:
-        with pytest.raises(mg.MGError) as e:
-            mg.Batch(sym, mg.BATCH_SSSP)
-        assert e.value.code == mg.ERR_DEVICE
+    """the symmetric property is detected, weights included: an edge whose reverse has
+    another weight makes the graph directed (SSSP pulls read a row's weights as the
+    in-edges')"""
+    assert mg.Graph.from_csr([0, 1, 3, 4], [1, 0, 2, 1]).symmetric
+    assert not mg.Graph.from_csr([0, 1, 2, 2], [1, 2]).symmetric
+    assert mg.Graph.from_csr([0, 1, 3, 4], [1, 0, 2, 1], [5, 5, 7, 7]).symmetric
+    assert not mg.Graph.from_csr([0, 1, 3, 4], [1, 0, 2, 1], [5, 5, 7, 8]).symmetric
+    assert mg.Graph.rmat(8, 4, seed=2).symmetric
+    assert not mg.Graph.rmat(8, 4, seed=2, flags=0).symmetric

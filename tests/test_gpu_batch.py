@@ -96,28 +96,65 @@ def test_batch_sssp(mg, orc, gname):
     b.close()
 
 
-@pytest.mark.parametrize("kind", ["bfs", "bc"])
-def test_batch_directed(mg, orc, kind):
-    """A directed CSR (no MG_READ_SYM) runs push levels only; levels follow out-edges."""
-    g = mg.Graph.rmat(11, 8, seed=13, flags=0)
-    row, col, _ = g.csr()
+def _fan_in(mg, n=3000):
+    """0 -> 1..n -> n+1 -> n+2, directed: the hub n+1 has n in-edges (several
+    1024-edge in-chunks, merged by the pulls) and one out-edge"""
+    src = np.concatenate([np.zeros(n, np.int32), np.arange(1, n + 1, dtype=np.int32), [n + 1]]).astype(np.int32)
+    dst = np.concatenate([np.arange(1, n + 1, dtype=np.int32), np.full(n, n + 1, np.int32), [n + 2]]).astype(np.int32)
+    val = (1 + (np.arange(len(src)) * 7919) % 64).astype(np.int32)
+    return mg.Graph.from_coo(n + 3, src, dst, val, flags=0)
+
+
+DIRECTED_MODES = [("hybrid", dict()), ("sparse", dict(mode=1)), ("dense", dict(mode=2))]
+
+
+@pytest.mark.parametrize("mode", DIRECTED_MODES, ids=[m[0] for m in DIRECTED_MODES])
+@pytest.mark.parametrize("gname", ["rmat", "fan_in"])
+@pytest.mark.parametrize("kind", ["bfs", "sssp", "bc"])
+def test_batch_directed(mg, orc, kind, gname, mode):
+    """A directed CSR (no MG_READ_SYM): pushes follow out-edges, pulls read the
+    transposed CSR; levels, distances and path counts as the oracle's per source"""
+    g = mg.Graph.rmat(11, 8, seed=13, flags=0) if gname == "rmat" else _fan_in(mg)
+    row, col, w = g.csr()
     src = _sources(g, 64, 17)
-    b = mg.Batch(g, mg.BATCH_BFS if kind == "bfs" else mg.BATCH_BC)
+    if gname == "fan_in":
+        src[:3] = [0, 1, g.nv - 2]
+    b = mg.Batch(g, {"bfs": mg.BATCH_BFS, "sssp": mg.BATCH_SSSP, "bc": mg.BATCH_BC}[kind], **mode[1])
+    print(f"directed batch {kind} {gname} mode={mode[0]}", flush=True)
     b.run(src)
     lv = b.levels()
+    sg = b.sigma() if kind == "bc" else None
     for i, s in enumerate(src):
         if kind == "bfs":
             np.testing.assert_array_equal(lv[i], orc.bfs(row, col, int(s)))
+        elif kind == "sssp":
+            np.testing.assert_array_equal(lv[i], orc.sssp_bf(row, col, w, int(s))[0], err_msg=f"source {i}={s}")
         else:
-            l, _, _ = orc.bc_fp64(row, col, int(s))
-            np.testing.assert_array_equal(lv[i], l)
+            l, sig, _ = orc.bc_fp64(row, col, int(s))
+            np.testing.assert_array_equal(lv[i], l, err_msg=f"source {i}={s}")
+            np.testing.assert_array_equal(sg[i], sig, err_msg=f"source {i}={s}")
     b.close()
 
 
-def test_batch_sssp_needs_symmetric(mg):
-    g = mg.Graph.rmat(8, 4, seed=2, flags=0)
-    with pytest.raises(mg.MGError):
-        mg.Batch(g, mg.BATCH_SSSP)
+@pytest.mark.parametrize("mode", [dict(), dict(mode=1)], ids=["hybrid", "sparse"])
+def test_batch_sssp_isolated_sources(mg, orc, mode):
+    """vertices without edges (ids kept by from_csr, the last one included) as sources
+    of queue levels: their changed bits relax nothing"""
+    n = 40
+    nbr = [[v - 1] * (v > 0) + [v + 1] * (v < n - 1) for v in range(n)] + [[], []]
+    row = np.concatenate([[0], np.cumsum([len(x) for x in nbr])]).astype(np.int64)
+    col = np.array([u for x in nbr for u in x], np.int32)
+    u = np.repeat(np.arange(n + 2), np.diff(row))
+    w = (1 + (np.minimum(u, col) * 13 + np.maximum(u, col)) % 7).astype(np.int32)  # same both ways
+    g = mg.Graph.from_csr(row, col, w)
+    row, col, w = g.csr()
+    src = np.array([n, n + 1, 0, n // 2, n - 1, n + 1], np.int32)
+    b = mg.Batch(g, mg.BATCH_SSSP, **mode)
+    b.run(src)
+    d = b.levels()
+    for i, s in enumerate(src):
+        np.testing.assert_array_equal(d[i], orc.sssp_bf(row, col, w, int(s))[0], err_msg=f"source {i}={s}")
+    b.close()
 
 
 @pytest.mark.parametrize("kind", ["bfs", "sssp", "bc"])
