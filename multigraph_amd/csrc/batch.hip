@@ -267,6 +267,33 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bfs(const BCtl *ctl, long long 
     }
 }
 
+// pull, BFS, vertices of at most kLightPull in-edges: a lane per vertex walks its row
+// alone (64 consecutive ids per wave, so the row loads of a step are close together)
+// and stops once its missing bits are covered; k_mb_pull_bfs takes the heavier ones'
+// chunks.  A wave per light vertex would leave most lanes idle and pay a cross-lane
+// reduction per 64 edges for a handful of edges.
+constexpr int kLightPull = 32;
+__global__ __launch_bounds__(kBB) void k_mb_pull_bfs_light(const BCtl *ctl, int nv, const int *__restrict__ row,
+                                                           const int *__restrict__ col, uint64_t all, uint64_t *FA,
+                                                           const uint64_t *__restrict__ vis, uint64_t *FB) {
+    if (!level_runs(ctl, 0)) return;
+    uint64_t *F, *Fn;
+    cur_next(ctl, FA, FB, &F, &Fn);
+    if (ctl) all = ctl->all;
+    for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB) {
+        const int p0 = row[v], p1 = row[v + 1];
+        if (p1 == p0 || p1 - p0 > kLightPull) continue;
+        const uint64_t need = all & ~vis[v];
+        if (!need) continue;
+        uint64_t acc = 0;
+        for (int p = p0; p < p1; ++p) {
+            acc |= F[ld_stream(&col[p])] & need;
+            if (acc == need) break;
+        }
+        if (acc) Fn[v] = acc;
+    }
+}
+
 // pull, BC: lanes = sources; neighbours staged 64 at a time in LDS, then lane s sums
 // sigma[u][s] over neighbours u whose frontier word has bit s
 __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *__restrict__ ch, uint64_t all,
@@ -333,20 +360,34 @@ __global__ __launch_bounds__(kBB) void k_mb_apply(const BCtl *ctl, int nv, const
     if (ctl) it = ctl->it;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB) {
-        const uint64_t f = Fn[v];
-        if (!f) continue;
-        const uint64_t vv = vis[v];
-        const uint64_t nw = f & ~vv;
-        if (nw != f) Fn[v] = nw;
-        if (!nw) continue;
-        vis[v] = vv | nw;
-        ++cnt;
-        const int dg = row[v + 1] - row[v];
-        mf += dg;
-        mx = max(mx, dg);
-        uint64_t m = nw;
-        while (m) { const int s = __builtin_ctzll(m); lev[(long long)v * 64 + s] = it; m &= m - 1; }
+    // a lane per vertex for the words; the level rows are then written by the whole
+    // wave, one vertex at a time (lanes = sources: one coalesced 256-byte row store),
+    // not by each lane walking its own row (64 rows, i.e. 64 cache lines, per store)
+    for (int v0 = blockIdx.x * kBB + (threadIdx.x & ~(MG_WAVE - 1)); v0 < nv; v0 += gridDim.x * kBB) {
+        const int v = v0 + lane();
+        uint64_t nw = 0;
+        if (v < nv) {
+            const uint64_t f = Fn[v];
+            if (f) {
+                const uint64_t vv = vis[v];
+                nw = f & ~vv;
+                if (nw != f) Fn[v] = nw;
+                if (nw) {
+                    vis[v] = vv | nw;
+                    ++cnt;
+                    const int dg = row[v + 1] - row[v];
+                    mf += dg;
+                    mx = max(mx, dg);
+                }
+            }
+        }
+        uint64_t todo = ballot(nw != 0ull);
+        while (todo) {
+            const int l = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint64_t m = (uint64_t)__shfl((long long)nw, l);
+            if ((m >> lane()) & 1ull) lev[(long long)(v0 + l) * 64 + lane()] = it;
+        }
     }
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
 }
@@ -1110,11 +1151,16 @@ struct mg_batch {
     // null on a symmetric graph, whose rows serve both directions
     Chunk *tchunks = nullptr;
     long long ntch = 0;
-    int *tcol = nullptr, *tw = nullptr;
+    int *trow = nullptr, *tcol = nullptr, *tw = nullptr;
     const Chunk *in_chunks() const { return tchunks ? tchunks : chunks; }
     long long in_nch() const { return tchunks ? ntch : nch; }
+    const int *in_row() const { return tchunks ? trow : row; }
     const int *in_col() const { return tchunks ? tcol : col; }
     const int *in_w() const { return tchunks ? tw : w; }
+    // BFS pulls: in-edge chunks of the vertices above kLightPull in-edges only (the
+    // lighter ones are walked a lane per vertex)
+    Chunk *hchunks = nullptr;
+    long long nhch = 0;
     int *logv = nullptr;          // (vertex, word) entries of every level (BC)
     uint64_t *logw = nullptr;
     long long log_cap = 0;
@@ -1140,7 +1186,7 @@ static void batch_free(mg_batch *b) {
     if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
                     b->delta, b->bc, b->chunks, b->logv, b->logw, b->cw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc,
-                    b->tchunks, b->tcol, b->tw};
+                    b->tchunks, b->trow, b->tcol, b->tw, b->hchunks};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1198,6 +1244,13 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
         cut(trow.data(), tch);
         b->ntch = (long long)tch.size();
     }
+    std::vector<Chunk> hch;
+    if (kind == MG_BATCH_BFS)
+        for (const Chunk &c : (trow.empty() ? ch : tch)) {
+            const int32_t *r = trow.empty() ? g->row.data() : trow.data();
+            if (r[c.v + 1] - r[c.v] > kLightPull) hch.push_back(c);
+        }
+    b->nhch = (long long)hch.size();
     const long long nv = b->nv, ne = std::max<long long>(1, b->ne), slots = nv * 64 + 64;
     bool ok = hipMalloc(&b->row, sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->col, sizeof(int) * ne) == hipSuccess &&
               hipMalloc(&b->chunks, sizeof(Chunk) * std::max<long long>(1, b->nch)) == hipSuccess &&
@@ -1219,11 +1272,18 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
         for (long long p = 0; p < g->ne; ++p) sw += (double)g->w[p];
         b->auto_delta = g->ne > 0 ? std::max(1, (int)(sw / (double)g->ne / 4.0)) : 1;
     }
+    if (ok && kind == MG_BATCH_BFS) {
+        ok = hipMalloc(&b->hchunks, sizeof(Chunk) * std::max<long long>(1, b->nhch)) == hipSuccess;
+        if (ok && b->nhch > 0 && hipMemcpy(b->hchunks, hch.data(), sizeof(Chunk) * b->nhch, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(mg_fail(MG_ERR_DEVICE, "batch: chunk upload failed"));
+    }
     if (ok && !trow.empty()) {
         ok = hipMalloc(&b->tchunks, sizeof(Chunk) * std::max<long long>(1, b->ntch)) == hipSuccess &&
+             hipMalloc(&b->trow, sizeof(int) * (nv + 1)) == hipSuccess &&
              hipMalloc(&b->tcol, sizeof(int) * ne) == hipSuccess &&
              (tw.empty() || hipMalloc(&b->tw, sizeof(int) * ne) == hipSuccess);
         if (ok && ((b->ntch > 0 && hipMemcpy(b->tchunks, tch.data(), sizeof(Chunk) * b->ntch, hipMemcpyHostToDevice) != hipSuccess) ||
+                   hipMemcpy(b->trow, trow.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice) != hipSuccess ||
                    (g->ne > 0 && hipMemcpy(b->tcol, tcol.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess) ||
                    (b->tw && g->ne > 0 && hipMemcpy(b->tw, tw.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess)))
             return fail(mg_fail(MG_ERR_DEVICE, "batch: upload of the transposed CSR failed"));
@@ -1320,6 +1380,7 @@ static int run_bfs_bc(mg_batch *b) {
     const int gv = grid_for(nv, kBB, G);
     const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
     const int gci = grid_for(b->in_nch(), kBB / MG_WAVE, G);  // pulls walk in-edge chunks
+    const int gch = grid_for(b->nhch, kBB / MG_WAVE, G);     // BFS pulls: heavy chunks only
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[0], (long long)nv + 1, 0ull);
     hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->lev, slots, (int)MG_INF);
@@ -1348,7 +1409,12 @@ static int run_bfs_bc(mg_batch *b) {
             else hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
         } else {
             if (bc) hipLaunchKernelGGL(k_mb_pull_bc, dim3(gci), dim3(kBB), 0, s, b->in_nch(), b->in_chunks(), all, b->in_col(), F, b->vis, Fn, b->sigma);
-            else hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gci), dim3(kBB), 0, s, (const BCtl *)nullptr, b->in_nch(), b->in_chunks(), all, b->in_col(), F, b->vis, Fn);
+            else {
+                hipLaunchKernelGGL(k_mb_pull_bfs_light, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->in_row(), b->in_col(),
+                                   all, F, b->vis, Fn);
+                hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gch), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nhch, b->hchunks, all,
+                                   b->in_col(), F, b->vis, Fn);
+            }
         }
         hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->vis, F, Fn, b->lev, it, b->part_n, b->part_e, b->part_x);
         MG_TRY(read_counts(b, gv, &n, &mf));
@@ -1395,6 +1461,7 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
     const int gv = grid_for(nv, kBB, G);
     const int gc = grid_for(b->nch, kBB / MG_WAVE, G);
     const int gci = grid_for(b->in_nch(), kBB / MG_WAVE, G);  // pulls walk in-edge chunks
+    const int gch = grid_for(b->nhch, kBB / MG_WAVE, G);     // BFS pulls: heavy chunks only
     const bool sssp = b->kind == MG_BATCH_SSSP;
     hipGraph_t g = nullptr;
     MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1432,7 +1499,9 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
         } else {
             hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col,
                                b->F[0], b->vis, b->F[1], (double *)nullptr);
-            hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gci), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->in_nch(), b->in_chunks(), 0ull,
+            hipLaunchKernelGGL(k_mb_pull_bfs_light, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->in_row(), b->in_col(),
+                               0ull, b->F[0], b->vis, b->F[1]);
+            hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gch), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nhch, b->hchunks, 0ull,
                                b->in_col(), b->F[0], b->vis, b->F[1]);
             hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->vis, b->F[0], b->F[1],
                                b->lev, 0, b->part_n, b->part_e, b->part_x);
