@@ -351,13 +351,23 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
 
 // after a push or pull level: keep only new bits, mark them visited, record levels,
 // and leave the new frontier's size and degree sum as per-workgroup partials
+// With a dense-level log (dlog: dcap words per vertex, one per dense level; BFS runs
+// under the control block), a dense level records each vertex's new bits as one word
+// in its slot, and k_mb_expand writes every level row once, whole, after the run:
+// written level by level, a row takes several partial stores, each a read-modify-write
+// of its lines in memory.  Without a log, or past its dcap slots, the rows are written
+// here.
 __global__ __launch_bounds__(kBB) void k_mb_apply(const BCtl *ctl, int nv, const int *__restrict__ row, uint64_t *__restrict__ vis,
                                                   uint64_t *FA, uint64_t *FB, int *__restrict__ lev, int it,
-                                                  int *__restrict__ part_n, long long *__restrict__ part_e, int *__restrict__ part_x) {
+                                                  int *__restrict__ part_n, long long *__restrict__ part_e, int *__restrict__ part_x,
+                                                  uint64_t *__restrict__ dlog, int *__restrict__ dlev, int dcap) {
     if (!level_runs(ctl, -1)) return;
     uint64_t *F, *Fn;
     cur_next(ctl, FA, FB, &F, &Fn);
     if (ctl) it = ctl->it;
+    const int slot = ctl && dlog ? ctl->ndense - 1 : dcap;  // k_mb_ctl counted this level
+    uint64_t *lw = slot >= 0 && slot < dcap ? dlog + (long long)slot * (nv + 1) : nullptr;
+    if (lw && blockIdx.x == 0 && threadIdx.x == 0) dlev[slot] = it;
     int cnt = 0, mx = 0;
     long long mf = 0;
     // a lane per vertex for the words; the level rows are then written by the whole
@@ -381,6 +391,10 @@ __global__ __launch_bounds__(kBB) void k_mb_apply(const BCtl *ctl, int nv, const
                 }
             }
         }
+        if (lw) {
+            if (v < nv) lw[v] = nw;
+            continue;
+        }
         uint64_t todo = ballot(nw != 0ull);
         while (todo) {
             const int l = __builtin_ctzll(todo);
@@ -390,6 +404,42 @@ __global__ __launch_bounds__(kBB) void k_mb_apply(const BCtl *ctl, int nv, const
         }
     }
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
+}
+
+// After a BFS run with a dense-level log: every level row written once (lanes =
+// sources): a bit set at a logged dense level takes that level; a bit never visited
+// is MG_INF; another visited bit keeps the value its sparse level, seed or unlogged
+// dense level stored (its lane does not store).  A lane per vertex loads the words
+// (coalesced); the wave then writes the 64 rows one by one from lane-broadcast copies.
+constexpr int kDenseLog = 16;  // logged dense levels per BFS run
+__global__ __launch_bounds__(kBB) void k_mb_expand(int nv, int nd, const uint64_t *__restrict__ vis,
+                                                   const uint64_t *__restrict__ dlog, const int *__restrict__ dlev,
+                                                   int *__restrict__ lev) {
+    int dl[kDenseLog];
+#pragma unroll
+    for (int d = 0; d < kDenseLog; ++d) dl[d] = d < nd ? dlev[d] : 0;
+    for (int v0 = blockIdx.x * kBB + (threadIdx.x & ~(MG_WAVE - 1)); v0 < nv; v0 += gridDim.x * kBB) {
+        const int v = v0 + lane();
+        const uint64_t vw = v < nv ? vis[v] : 0ull;
+        uint64_t dw[kDenseLog];
+#pragma unroll
+        for (int d = 0; d < kDenseLog; ++d) dw[d] = (d < nd && v < nv) ? dlog[(long long)d * (nv + 1) + v] : 0ull;
+        const int nl = min(MG_WAVE, nv - v0);
+        for (int l = 0; l < nl; ++l) {  // l is wave-uniform: broadcasts by readlane
+            const uint64_t bv = bcast64(vw, l);
+            int lv = -1;
+#pragma unroll
+            for (int d = 0; d < kDenseLog; ++d) {
+                if (d >= nd) break;
+                const uint64_t w = bcast64(dw[d], l);
+                if (lv < 0 && ((w >> lane()) & 1ull)) lv = dl[d];
+            }
+            // a visited bit of no logged level already holds its level: that lane
+            // stores nothing (no read of the row)
+            const bool visited = (bv >> lane()) & 1ull;
+            if (!visited || lv >= 0) lev[(long long)(v0 + l) * 64 + lane()] = visited ? lv : (int)MG_INF;
+        }
+    }
 }
 
 // size and degree sum of the nonzero words of the next frontier (without a control
@@ -1161,6 +1211,10 @@ struct mg_batch {
     // lighter ones are walked a lane per vertex)
     Chunk *hchunks = nullptr;
     long long nhch = 0;
+    // BFS: new-bit words of the first kDenseLog dense levels ([level][nv + 1]) and their
+    // level numbers, expanded into the level rows after the run (k_mb_expand)
+    uint64_t *dlog = nullptr;
+    int *dlev = nullptr;
     int *logv = nullptr;          // (vertex, word) entries of every level (BC)
     uint64_t *logw = nullptr;
     long long log_cap = 0;
@@ -1186,7 +1240,7 @@ static void batch_free(mg_batch *b) {
     if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
                     b->delta, b->bc, b->chunks, b->logv, b->logw, b->cw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc,
-                    b->tchunks, b->trow, b->tcol, b->tw, b->hchunks};
+                    b->tchunks, b->trow, b->tcol, b->tw, b->hchunks, b->dlog, b->dlev};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1273,7 +1327,9 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
         b->auto_delta = g->ne > 0 ? std::max(1, (int)(sw / (double)g->ne / 4.0)) : 1;
     }
     if (ok && kind == MG_BATCH_BFS) {
-        ok = hipMalloc(&b->hchunks, sizeof(Chunk) * std::max<long long>(1, b->nhch)) == hipSuccess;
+        ok = hipMalloc(&b->hchunks, sizeof(Chunk) * std::max<long long>(1, b->nhch)) == hipSuccess &&
+             hipMalloc(&b->dlog, sizeof(uint64_t) * kDenseLog * (nv + 1)) == hipSuccess &&
+             hipMalloc(&b->dlev, sizeof(int) * kDenseLog) == hipSuccess;
         if (ok && b->nhch > 0 && hipMemcpy(b->hchunks, hch.data(), sizeof(Chunk) * b->nhch, hipMemcpyHostToDevice) != hipSuccess)
             return fail(mg_fail(MG_ERR_DEVICE, "batch: chunk upload failed"));
     }
@@ -1416,7 +1472,8 @@ static int run_bfs_bc(mg_batch *b) {
                                    b->in_col(), F, b->vis, Fn);
             }
         }
-        hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->vis, F, Fn, b->lev, it, b->part_n, b->part_e, b->part_x);
+        hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->row, b->vis, F, Fn, b->lev, it, b->part_n, b->part_e, b->part_x,
+                           (uint64_t *)nullptr, (int *)nullptr, 0);
         MG_TRY(read_counts(b, gv, &n, &mf));
         if (bc && n > 0) { long long c = 0; MG_TRY(compact(b, Fn, off.back(), &c)); off.push_back(off.back() + c); }
         cur ^= 1;
@@ -1504,7 +1561,7 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
             hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gch), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nhch, b->hchunks, 0ull,
                                b->in_col(), b->F[0], b->vis, b->F[1]);
             hipLaunchKernelGGL(k_mb_apply, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->vis, b->F[0], b->F[1],
-                               b->lev, 0, b->part_n, b->part_e, b->part_x);
+                               b->lev, 0, b->part_n, b->part_e, b->part_x, b->dlog, b->dlev, b->dlog ? kDenseLog : 0);
         }
         sparse_level();
     }
@@ -1534,8 +1591,9 @@ static int run_levels(mg_batch *b) {
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->F[0], (long long)nv + 1, 0ull);
+    // BFS level rows need no clearing: k_mb_expand writes every one after the run
     if (sssp) hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->dist, slots, (int)MG_INF);
-    else hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->lev, slots, (int)MG_INF);
+    else if (!b->dlog) hipLaunchKernelGGL(k_fill_i32b, dim3(grid_for(slots, 256, 8192)), dim3(256), 0, s, b->lev, slots, (int)MG_INF);
     hipLaunchKernelGGL(k_ms_seed, dim3(grid_for(b->nseeds, 256, 4096)), dim3(256), 0, s, b->nseeds, b->src,
                        b->slotted ? b->slot : nullptr, b->vis, b->F[0], sssp ? (int *)nullptr : b->lev, (double *)nullptr,
                        sssp ? b->dist : (int *)nullptr);
@@ -1565,6 +1623,10 @@ static int run_levels(mg_batch *b) {
         ++trips;
         if (b->h_ctl->done) break;
     }
+    if (!sssp && b->dlog)
+        hipLaunchKernelGGL(k_mb_expand, dim3(gv), dim3(kBB), 0, s, nv, std::min(b->h_ctl->ndense, kDenseLog), b->vis, b->dlog,
+                           b->dlev, b->lev);
+    MG_HIP(hipGetLastError());
     b->levels = b->h_ctl->it;
     b->stats[0] = b->levels;
     b->stats[1] = b->h_ctl->ndense;

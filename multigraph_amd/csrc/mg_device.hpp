@@ -23,6 +23,11 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
 }
 
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// lane l's 64-bit value in every lane (l wave-uniform)
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int l) {
+    const uint32_t lo = (uint32_t)readlane((int)(uint32_t)v, l), hi = (uint32_t)readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
 
 // Wave-aggregated append: one atomicAdd per wave for all lanes with `pred`;
 // returns this lane's slot (or -1).  64-lane re-derivation of atomicAggInc
