@@ -135,6 +135,33 @@ constexpr int kCH = 1024;  // default edges per chunk (mg_options.batch_chunk)
 #endif
 constexpr int kHitStep = MG_HIT_STEP;  // matched edges whose row gathers a wave issues together
 
+// Wave w walks chunks w, w + nw, w + 2 nw, ... (nw waves in the grid) 64 at a time: each
+// lane loads one chunk and its word (word_of(chunk); 0 = nothing to do), and the wave
+// then runs body(chunk, word) for the nonzero ones in turn, from lane broadcasts.  A
+// level whose frontier touches few chunks costs one load round per 64 chunks instead
+// of one per chunk; the stride keeps a hub's consecutive chunks on different waves.
+template <class WordOf, class Body>
+__device__ __forceinline__ void for_active_chunks(long long nch, const Chunk *__restrict__ ch, WordOf word_of, Body body) {
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long base = wave; base < nch; base += nw * MG_WAVE) {
+        const long long c = base + (long long)lane() * nw;
+        Chunk k{0, 0, 0, 0};
+        uint64_t word = 0ull;
+        if (c < nch) {
+            k = ch[c];
+            word = word_of(k);
+        }
+        uint64_t act = ballot(word != 0ull);
+        while (act) {
+            const int l = __builtin_ctzll(act);
+            act &= act - 1;
+            const Chunk kl{readlane(k.v, l), readlane(k.p0, l), readlane(k.p1, l), readlane(k.multi, l)};
+            body(kl, bcast64(word, l));
+        }
+    }
+}
+
 // Device-side level control (BFS and SSSP runs): the level kernels read the current
 // level from here, so K levels can be captured in one hipGraph and replayed past the
 // end of the run (every kernel returns at once when `done` is set).
@@ -193,12 +220,9 @@ __global__ __launch_bounds__(kBB) void k_mb_push(const BCtl *ctl, long long nch,
     if (!level_runs(ctl, 1)) return;
     uint64_t *F, *Fn;
     cur_next(ctl, FA, FB, &F, &Fn);
-    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
-    const long long nw = ((long long)gridDim.x * kBB) >> 6;
-    for (long long c = wave; c < nch; c += nw) {
-        const Chunk k = ch[c];
-        const uint64_t fu = F[k.v];
-        if (!fu) continue;
+    __shared__ int s_w[kBB];
+    __shared__ uint64_t s_m[kBB];
+    for_active_chunks(nch, ch, [&](const Chunk &k) { return F[k.v]; }, [&](const Chunk &k, const uint64_t fu) {
         if (!kBC) {
             for (int p = k.p0 + lane(); p < k.p1; p += MG_WAVE) {
                 const int w = col[p];
@@ -209,8 +233,6 @@ __global__ __launch_bounds__(kBB) void k_mb_push(const BCtl *ctl, long long nch,
             // lanes load 64 edges and their visited words at once and mark the next
             // frontier; then the wave walks the edges with new bits, four at a time,
             // lanes = sources adding sigma[u][s] into sigma[w][s]
-            __shared__ int s_w[kBB];
-            __shared__ uint64_t s_m[kBB];
             const int wl = threadIdx.x & ~(MG_WAVE - 1);
             const double su = ((fu >> lane()) & 1ull) ? sigma[(long long)k.v * 64 + lane()] : 0.0;
             for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
@@ -235,7 +257,7 @@ __global__ __launch_bounds__(kBB) void k_mb_push(const BCtl *ctl, long long nch,
                 }
             }
         }
-    }
+    });
 }
 
 // pull, BFS: a wave per chunk of a vertex still missing bits; lanes or the frontier
@@ -247,12 +269,7 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bfs(const BCtl *ctl, long long 
     uint64_t *F, *Fn;
     cur_next(ctl, FA, FB, &F, &Fn);
     if (ctl) all = ctl->all;
-    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
-    const long long nw = ((long long)gridDim.x * kBB) >> 6;
-    for (long long c = wave; c < nch; c += nw) {
-        const Chunk k = ch[c];
-        const uint64_t need = all & ~vis[k.v];
-        if (!need) continue;
+    for_active_chunks(nch, ch, [&](const Chunk &k) { return all & ~vis[k.v]; }, [&](const Chunk &k, const uint64_t need) {
         uint64_t acc = 0;
         for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
             const int p = p0 + lane();
@@ -264,7 +281,7 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bfs(const BCtl *ctl, long long 
             if (k.multi) atomicOr((unsigned long long *)&Fn[k.v], (unsigned long long)acc);
             else Fn[k.v] = acc;
         }
-    }
+    });
 }
 
 // pull, BFS, vertices of at most kLightPull in-edges: a lane per vertex walks its row
@@ -303,12 +320,7 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
     __shared__ int s_u[kBB];
     __shared__ uint64_t s_f[kBB];
     const int wl = threadIdx.x & ~(MG_WAVE - 1);
-    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
-    const long long nw = ((long long)gridDim.x * kBB) >> 6;
-    for (long long c = wave; c < nch; c += nw) {
-        const Chunk k = ch[c];
-        const uint64_t need = all & ~vis[k.v];
-        if (!need) continue;
+    for_active_chunks(nch, ch, [&](const Chunk &k) { return all & ~vis[k.v]; }, [&](const Chunk &k, const uint64_t need) {
         uint64_t acc = 0;
         double sg = 0.0;
         for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
@@ -336,7 +348,7 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
                 for (int q = 0; q < kHitStep; ++q) sg += t[q];
             }
         }
-        if (!acc) continue;
+        if (!acc) return;
         if ((acc >> lane()) & 1ull) {
             const long long kv = (long long)k.v * 64 + lane();
             if (k.multi) atomicAdd(&sigma[kv], sg);
@@ -346,7 +358,7 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
             if (k.multi) atomicOr((unsigned long long *)&Fn[k.v], (unsigned long long)acc);
             else Fn[k.v] = acc;
         }
-    }
+    });
 }
 
 // after a push or pull level: keep only new bits, mark them visited, record levels,
@@ -1011,12 +1023,7 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
     __shared__ int s_u[kBB];
     __shared__ uint64_t s_f[kBB];
     const int wl = threadIdx.x & ~(MG_WAVE - 1);
-    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
-    const long long nw = ((long long)gridDim.x * kBB) >> 6;
-    for (long long c = wave; c < nch; c += nw) {
-        const Chunk k = ch[c];
-        const uint64_t word = E[k.v];
-        if (!word) continue;
+    for_active_chunks(nch, ch, [&](const Chunk &k) { return E[k.v]; }, [&](const Chunk &k, const uint64_t word) {
         const bool mine = (word >> lane()) & 1ull;
         const double sv = mine ? sigma[(long long)k.v * 64 + lane()] : 0.0;
         KSum acc;
@@ -1054,7 +1061,7 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
                 cw[kv] = (float)((1.0 + (double)dv) / sv);
             }
         }
-    }
+    });
 }
 
 // bc[v] += sum over slots s of delta[v][s], except the slots v is a source of (level 0)
@@ -1136,12 +1143,7 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp_push(const BCtl *ctl, long long
     if (!level_runs(ctl, 1)) return;
     uint64_t *C, *Cn;
     cur_next(ctl, CA, CB, &C, &Cn);
-    const long long wave = ((long long)blockIdx.x * kBB + threadIdx.x) >> 6;
-    const long long nw = ((long long)gridDim.x * kBB) >> 6;
-    for (long long c = wave; c < nch; c += nw) {
-        const Chunk k = ch[c];
-        const uint64_t cu = C[k.v];
-        if (!cu) continue;
+    for_active_chunks(nch, ch, [&](const Chunk &k) { return C[k.v]; }, [&](const Chunk &k, const uint64_t cu) {
         const bool mine = (cu >> lane()) & 1ull;
         const int du = mine ? dist[(long long)k.v * 64 + lane()] : (int)MG_INF;
         // edges kHitStep at a time: their distance-row reads are in flight together
@@ -1163,7 +1165,7 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp_push(const BCtl *ctl, long long
                 if (lane() == 0 && m) atomicOr((unsigned long long *)&Cn[w[q]], (unsigned long long)m);
             }
         }
-    }
+    });
 }
 
 // vertex-major [v][64] -> source-major [s][v]
