@@ -58,6 +58,8 @@ struct Ctl {
     long long mu;      // degree sum of unexplored vertices
     double resid;      // PageRank FP64 L1 residual of the last iteration
     long long edges;   // edges traversed (statistics)
+    int seq;           // host mirror only: the level whose control block it holds
+    int pad_;
 };
 
 struct Params {
@@ -192,6 +194,7 @@ struct mg_engine {
     int *aux = nullptr;        // small device counters (BC split-vertex chunk count)
     mg::Ctl *ctl = nullptr;
     mg::Ctl *h_ctl = nullptr;  // pinned mirror
+    int ctl_seq = 0;           // levels whose control block k_ctl mirrored (eager steps)
     mg::Params P{};
     // CC: upper-triangular CSR
     int *urow = nullptr, *ucol = nullptr;

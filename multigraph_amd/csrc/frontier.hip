@@ -17,6 +17,8 @@
 //     sampling, process.cu:1161-1255).
 // All kernels are grid-stride and read their sizes from the device control block,
 // so a run of levels can be captured in a hipGraph and replayed with no host sync.
+#include <chrono>
+
 #include "engine.hpp"
 #include "user_function.hpp"
 #include "lb.hpp"
@@ -44,21 +46,31 @@ __global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int
     c.iter = 1; c.mode = MODE_TD; c.in_begin = 0; c.in_n = 1; c.out_begin = out_begin; c.out_n = 0;
     c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0; c.msg_iter = -1;
     c.lb = deg_src > kSmallMF ? 1 : 0; c.lb_total = 0; c.lb_chunks = 0; c.need_queue = 0; c.qsrc = 0;
-    c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0;
+    c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0; c.seq = 0; c.pad_ = 0;
     *ctl = c;
 }
 
-__global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf) {
+// mirror: the host's pinned copy of the control block (eager levels).  It gets the new
+// block, then, after a system-scope fence, its sequence number: the host spins on that
+// number instead of copying the block back and blocking on the stream.
+__global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf, Ctl *mirror, int seq) {
     Ctl c = *ctl;
-    if (c.mode == MODE_DONE) return;
-    if (lvl_off) {
-        lvl_off[c.iter] = c.out_begin;
-        lvl_off[c.iter + 1] = c.out_begin + c.out_n;
-        lvl_mf[c.iter] = c.out_mf;  // degree sum of the level (the BC backward pass picks its direction by it)
+    if (c.mode != MODE_DONE) {
+        if (lvl_off) {
+            lvl_off[c.iter] = c.out_begin;
+            lvl_off[c.iter + 1] = c.out_begin + c.out_n;
+            lvl_mf[c.iter] = c.out_mf;  // degree sum of the level (the BC backward pass picks its direction by it)
+        }
+        c.edges += c.mf;
+        advance(c, p);
+        *ctl = c;
     }
-    c.edges += c.mf;
-    advance(c, p);
-    *ctl = c;
+    if (mirror) {
+        c.seq = seq - 1;
+        *mirror = c;
+        __threadfence_system();
+        __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // relax one edge src->u (edge index p); returns 1 when u joins the next frontier
@@ -890,7 +902,8 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMalloc(&e->part_mf, sizeof(long long) * (max_grid(e) + 8)));
     MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
     MG_HIP(hipMalloc(&e->aux, sizeof(int) * 16));
-    MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    // coherent: k_ctl writes the host's copy of the control block directly
+    MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocCoherent));
     // a directed graph's pull levels read in-edges: transpose the CSR once (counting
     // sort by destination; scanning sources in order leaves every row sorted by source)
     std::vector<int32_t> hrow_t, hcol_t;
@@ -1128,7 +1141,8 @@ static int launch_level(mg_engine *e, bool known) {
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
     }
-    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf);
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf, known ? e->h_ctl : (Ctl *)nullptr,
+                       known ? ++e->ctl_seq : 0);
     if (!known) MG_TRY(launch_after_ctl(e));
     MG_HIP(hipGetLastError());
     return MG_OK;
@@ -1152,11 +1166,25 @@ static int launch_after_ctl(mg_engine *e) {
     return MG_OK;
 }
 
+// wait for the control block of level `seq` in the pinned mirror (written by k_ctl);
+// after ~2 ms of spinning, block on the stream instead, which also reports a failed
+// kernel, and copy the block back if the mirror still lags
+static int wait_ctl(mg_engine *e, int seq) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned k = 0;; ++k) {
+        if (__atomic_load_n(&e->h_ctl->seq, __ATOMIC_ACQUIRE) == seq) return MG_OK;
+        if ((k & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+    }
+    MG_HIP(hipStreamSynchronize(e->stream));
+    if (__atomic_load_n(&e->h_ctl->seq, __ATOMIC_ACQUIRE) == seq) return MG_OK;
+    MG_HIP(hipMemcpy(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    return MG_OK;
+}
+
 int mg_frontier_level(mg_engine *e) {
     if (e->done) return MG_OK;
     MG_TRY(launch_level(e, true));
-    MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
-    MG_HIP(hipStreamSynchronize(e->stream));
+    MG_TRY(wait_ctl(e, e->ctl_seq));
     const Ctl &c = *e->h_ctl;
     if (c.mode != MODE_DONE && (c.need_queue || (e->algo == MG_SSSP && c.mode == MODE_BU && c.convert)))
         MG_TRY(launch_after_ctl(e));
