@@ -88,7 +88,9 @@ typedef struct mg_options {
                                graphs, which measures faster on MI355X)                 */
     int    pull_layout;     /* SSSP dense sweeps: 0, the default, over the partitioned
                                layout (distance messages, 16-bit weights; symmetric graphs
-                               with weights in [0, 65535]), 1 over the CSR               */
+                               with weights in [0, 65535]) with 16-bit messages until a
+                               distance above 65534 must be sent, then 32-bit; 1 over the
+                               CSR; 2 over the layout with 32-bit messages throughout   */
     int    sssp_delta;      /* batched SSSP: sparse sweeps relax only the (vertex, source)
                                pairs below a distance threshold that rises by this much
                                per level, the others wait (0: a quarter of the mean edge

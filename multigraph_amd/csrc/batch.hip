@@ -442,9 +442,10 @@ __global__ __launch_bounds__(kBB) void k_mb_expand(int nv, int nd, const uint64_
             int lv = -1;
 #pragma unroll
             for (int d = 0; d < kDenseLog; ++d) {
-                if (d >= nd) break;
-                const uint64_t w = bcast64(dw[d], l);
-                if (lv < 0 && ((w >> lane()) & 1ull)) lv = dl[d];
+                if (d < nd) {  // uniform
+                    const uint64_t w = bcast64(dw[d], l);
+                    if (lv < 0 && ((w >> lane()) & 1ull)) lv = dl[d];
+                }
             }
             // a visited bit of no logged level already holds its level: that lane
             // stores nothing (no read of the row)

@@ -59,7 +59,8 @@ struct Ctl {
     double resid;      // PageRank FP64 L1 residual of the last iteration
     long long edges;   // edges traversed (statistics)
     int seq;           // host mirror only: the level whose control block it holds
-    int pad_;
+    int sat;           // SSSP layout sweeps with 16-bit messages: a finite distance above
+                       // 65534 could not be sent (the host widens the messages)
 };
 
 struct Params {
@@ -195,6 +196,10 @@ struct mg_engine {
     mg::Ctl *ctl = nullptr;
     mg::Ctl *h_ctl = nullptr;  // pinned mirror
     int ctl_seq = 0;           // levels whose control block k_ctl mirrored (eager steps)
+    // SSSP layout sweeps: 16-bit distance messages while every finite distance sent
+    // fits (pb16 for the current run, pb16_ok when the engine allows them), 32-bit
+    // otherwise; pb_force makes the next sweep rebuild every message
+    bool pb16 = false, pb16_ok = false, pb_force = false;
     mg::Params P{};
     // CC: upper-triangular CSR
     int *urow = nullptr, *ucol = nullptr;

@@ -46,7 +46,7 @@ __global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int
     c.iter = 1; c.mode = MODE_TD; c.in_begin = 0; c.in_n = 1; c.out_begin = out_begin; c.out_n = 0;
     c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0; c.msg_iter = -1;
     c.lb = deg_src > kSmallMF ? 1 : 0; c.lb_total = 0; c.lb_chunks = 0; c.need_queue = 0; c.qsrc = 0;
-    c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0; c.seq = 0; c.pad_ = 0;
+    c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0; c.seq = 0; c.sat = 0;
     *ctl = c;
 }
 
@@ -942,7 +942,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
                                e->pnch, e->pcstart);
         // dense sweeps over the partitioned layout (symmetric graphs, weights in 16 bits),
         // else over the CSR
-        bool w16 = g->symmetric && e->opt.pull_layout == 0 && nv >= 2;
+        bool w16 = g->symmetric && e->opt.pull_layout != 1 && nv >= 2;
         int wmax = 0;
         for (long long p = 0; w16 && p < ne; ++p) {
             w16 = g->w[p] >= 0 && g->w[p] <= 0xFFFF;
@@ -952,6 +952,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
             e->pb.w8 = wmax <= 0xFF;
             MG_TRY(mg_pb_build(e, g));
             e->sssp_pb = true;
+            e->pb16_ok = e->opt.pull_layout == 0;  // 2: 32-bit messages throughout
         }
     } else {
         const long long words = (nv + 63) / 64 + 1;
@@ -1030,6 +1031,8 @@ int mg_frontier_reset(mg_engine *e, int source) {
     e->iter = 1;
     e->done = (nv == 0);
     e->levels = 1;
+    e->pb16 = e->pb16_ok;
+    e->pb_force = false;
     if (nv > 0) {
         MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
         MG_HIP(hipStreamSynchronize(s));
@@ -1186,6 +1189,10 @@ int mg_frontier_level(mg_engine *e) {
     MG_TRY(launch_level(e, true));
     MG_TRY(wait_ctl(e, e->ctl_seq));
     const Ctl &c = *e->h_ctl;
+    if (e->pb16 && c.sat) {  // a distance above 65534 met a 16-bit message: widen them
+        e->pb16 = false;
+        e->pb_force = true;
+    }
     if (c.mode != MODE_DONE && (c.need_queue || (e->algo == MG_SSSP && c.mode == MODE_BU && c.convert)))
         MG_TRY(launch_after_ctl(e));
     e->iter = c.iter;
@@ -1199,7 +1206,12 @@ int mg_frontier_capture(mg_engine *e, int levels, hipGraphExec_t *out) {
     hipGraph_t g = nullptr;
     MG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = MG_OK;
+    // captured SSSP sweeps use 32-bit messages: no host step between them could widen
+    // 16-bit ones
+    const bool narrow = e->pb16;
+    e->pb16 = false;
     for (int i = 0; i < levels && rc == MG_OK; ++i) rc = launch_level(e, false);
+    e->pb16 = narrow;
     hipError_t ce = hipStreamEndCapture(e->stream, &g);
     if (rc != MG_OK) { if (g) hipGraphDestroy(g); return rc; }
     if (ce != hipSuccess) return mg_fail(MG_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ce));

@@ -30,6 +30,7 @@
 // S edge slots: 6.5 B * M in the scatter, 4 B * M + 2.5 B * S in the gather, plus
 // 20 B per vertex.  On RMAT graphs M is a half to three quarters of S.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 #include "engine.hpp"
 #include "user_function.hpp"
@@ -278,6 +279,51 @@ __device__ __forceinline__ void scatter_part(int a0, int a1, const T *__restrict
         // streaming stores: the messages are read once, by the next gather
         st_stream_f4(&msg[base], m[0], m[1], m[2], m[3]);
         st_stream_f4(&msg[base + 4], m[4], m[5], m[6], m[7]);
+    };
+    int i = a0 + (int)threadIdx.x * kG;
+    if (i < a1) {
+        uint4 ia, ib;
+        int ba, bb;
+        ld(i, ia, ba);
+        for (;;) {
+            ld(i + step, ib, bb);
+            emit(i, ia, ba);
+            i += step;
+            if (i >= a1) break;
+            ld(i + step, ia, ba);
+            emit(i, ib, bb);
+            i += step;
+            if (i >= a1) break;
+        }
+    }
+}
+
+// SSSP with 16-bit messages: as scatter_part, from int distances staged in LDS,
+// each narrowed to 16 bits (65535 stands for "unreached or above 65534": it relaxes
+// nothing), 8 messages per 16-byte streaming store (message runs start at multiples
+// of 8 slots)
+__device__ __forceinline__ uint32_t narrow16(int d) { return (uint32_t)min((unsigned)d, 0xFFFFu); }
+template <int NT>
+__device__ __forceinline__ void scatter_part16(int a0, int a1, const int *__restrict__ s_d, const uint16_t *__restrict__ a_src,
+                                               const int *__restrict__ amap, uint16_t *__restrict__ msg) {
+    const int step = NT * kG;
+    auto ld = [&](int i, uint4 &id, int &base) {
+        const int c = min(i, a1 - kG);
+        id = ld_stream_u4(&a_src[c]);
+        base = amap[c / kG];
+    };
+    auto emit = [&](int i, const uint4 &id, int base) {
+        if (i >= a1) return;
+        const uint32_t w[4] = {id.x, id.y, id.z, id.w};
+        int o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t a = w[k] & 0xFFFFu, b = w[k] >> 16;
+            const uint32_t ma = a != 0xFFFFu ? narrow16(s_d[a]) : 0xFFFFu;
+            const uint32_t mb = b != 0xFFFFu ? narrow16(s_d[b]) : 0xFFFFu;
+            o[k] = (int)(ma | (mb << 16));
+        }
+        st_stream_f4((int *)&msg[base], o[0], o[1], o[2], o[3]);
     };
     int i = a0 + (int)threadIdx.x * kG;
     if (i < a1) {
@@ -561,24 +607,41 @@ __global__ void k_pb_zero_f64(double *p, long long n) {
 // messages from the current distances: every source partition before a pull sweep
 // that no pull sweep prepared (when = 0), or the partitions whose destination side was
 // split, after the gather (when = 1); unit = {partition, a0, a1, 0}
-__global__ __launch_bounds__(kBT) void k_ps_scatter(Ctl *ctl, int when, const int4 *__restrict__ units, int nunits, int P,
-                                                    int nv, int np, const int *__restrict__ inv, const int *__restrict__ dist,
+// (force: rebuild every message whatever the last sweep prepared, after the width
+// changed).  MT = uint16_t: 16-bit messages; a finite distance above 65534 is sent as
+// 65535 (relaxes nothing), so its vertex is stamped with the level (it stays in the
+// frontier) and ctl->sat tells the host to continue with 32-bit messages.
+template <class MT>
+__global__ __launch_bounds__(kBT) void k_ps_scatter(Ctl *ctl, int when, int force, const int4 *__restrict__ units, int nunits,
+                                                    int P, int nv, int np, const int *__restrict__ inv,
+                                                    const int *__restrict__ dist, int *__restrict__ vfc,
                                                     const uint16_t *__restrict__ a_src, const int *__restrict__ amap,
-                                                    int *__restrict__ msgbuf, long long npairs) {
+                                                    MT *__restrict__ msgbuf, long long npairs) {
     if (ctl->mode != MODE_BU) return;
     const int iter = ctl->iter;
-    if (when == 0 && ctl->msg_iter == iter) return;
-    int *out = msgbuf;
+    if (when == 0 && !force && ctl->msg_iter == iter) return;
+    MT *out = msgbuf;
     extern __shared__ int s_dist[];
+    bool sat = false;
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int4 un = units[u];
         const int v0 = un.x * P;
         const int pv = part_count(un.x, nv, np);
-        for (int k = threadIdx.x; k < pv; k += kBT) s_dist[k] = dist[inv[v0 + k]];
+        for (int k = threadIdx.x; k < pv; k += kBT) {
+            const int v = inv[v0 + k];
+            const int d = dist[v];
+            s_dist[k] = d;
+            if (sizeof(MT) == 2 && d != (int)MG_INF && d >= 0xFFFF) {
+                vfc[v] = iter;
+                sat = true;
+            }
+        }
         __syncthreads();
-        scatter_part<kBT, int>(un.y, un.z, s_dist, a_src, amap, out);
+        if constexpr (sizeof(MT) == 2) scatter_part16<kBT>(un.y, un.z, s_dist, a_src, amap, out);
+        else scatter_part<kBT, int>(un.y, un.z, s_dist, a_src, amap, out);
         __syncthreads();
     }
+    if (sat) ctl->sat = 1;
 }
 
 // a segment of the SSSP gather: the PageRank segment plus the lane's 8 slot weights
@@ -588,7 +651,8 @@ struct SegW {
     int m[kStageM];
 };
 
-__device__ __forceinline__ void load_msg_w(SegW &sg, const SegHdr &h, const uint4 &wt, const int *__restrict__ msg,
+template <class MT>
+__device__ __forceinline__ void load_msg_w(SegW &sg, const SegHdr &h, const uint4 &wt, const MT *__restrict__ msg,
                                            long long npairs) {
     const long long q0 = h.ga;
     const long long q1 = min(npairs, (long long)h.gb + 1);
@@ -598,15 +662,22 @@ __device__ __forceinline__ void load_msg_w(SegW &sg, const SegHdr &h, const uint
     sg.wt = wt;
     sg.base = h.pad ? 0 : h.gl - h.ga;
 #pragma unroll
-    for (int k = 0; k < kStageM; ++k) sg.m[k] = ld_stream(&msg[q0 + min(lane() + k * MG_WAVE, sg.n - 1)]);
+    for (int k = 0; k < kStageM; ++k) {
+        if constexpr (sizeof(MT) == 2) {
+            const uint32_t m = msg[q0 + min(lane() + k * MG_WAVE, sg.n - 1)];
+            sg.m[k] = m == 0xFFFFu ? (int)MG_INF : (int)m;
+        } else {
+            sg.m[k] = ld_stream(&msg[q0 + min(lane() + k * MG_WAVE, sg.n - 1)]);
+        }
+    }
 }
 
 // WT: uint8_t weights when every weight fits a byte (a slot's 8 weights are one 8-byte
-// load), else uint16_t (one 16-byte load)
-template <class WT>
+// load), else uint16_t (one 16-byte load); MT: the message width (see k_ps_scatter)
+template <class WT, class MT>
 __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
                                                    const uint16_t *__restrict__ b_dst, const WT *__restrict__ b_w,
-                                                   const int *__restrict__ gbase, int *__restrict__ msgbuf, long long npairs,
+                                                   const int *__restrict__ gbase, MT *__restrict__ msgbuf, long long npairs,
                                                    const int *__restrict__ apart, const uint16_t *__restrict__ a_src,
                                                    const int *__restrict__ amap, const int *__restrict__ inv,
                                                    int *__restrict__ dist, int *__restrict__ vfc) {
@@ -616,8 +687,9 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
     // this sweep reads, so partitions gathered later already relax from them.  Any mix
     // of old and new messages is an upper bound of the true distances, so the result
     // is exact whatever the order; only the number of sweeps depends on it.
-    const int *msg = msgbuf;
-    int *out = msgbuf;
+    const MT *msg = msgbuf;
+    MT *out = msgbuf;
+    bool sat = false;
     extern __shared__ int s_min[];  // P + 1 minima, then kStageW staged messages per wave
     const int wid = threadIdx.x / MG_WAVE;
     int *s_msg = s_min + P + 1 + wid * kStageW;
@@ -723,6 +795,10 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
                         vfc[v] = iter;
                         d = best;
                     }
+                    if (sizeof(MT) == 2 && d != (int)MG_INF && d >= 0xFFFF) {  // cannot be sent: see k_ps_scatter
+                        vfc[v] = iter;
+                        sat = true;
+                    }
                     cl[j] = d;
                 }
             }
@@ -733,10 +809,12 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
                 if (k < pv) s_min[k] = cl[j];
             }
             __syncthreads();
-            scatter_part<kBG, int>(apart[un.x], apart[un.x + 1], s_min, a_src, amap, out);
+            if constexpr (sizeof(MT) == 2) scatter_part16<kBG>(apart[un.x], apart[un.x + 1], s_min, a_src, amap, out);
+            else scatter_part<kBG, int>(apart[un.x], apart[un.x + 1], s_min, a_src, amap, out);
         }
         __syncthreads();
     }
+    if (sat) ctl->sat = 1;
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl->msg_iter = iter + 1;
 }
 
@@ -1037,10 +1115,14 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
         L.nS = (int)us.size();
         MG_HIP(hipMalloc(&L.unitsS, sizeof(int4) * std::max<size_t>(1, us.size())));
         if (!us.empty()) MG_HIP(hipMemcpy(L.unitsS, us.data(), sizeof(int4) * us.size(), hipMemcpyHostToDevice));
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(int) * P)));
+        const int lds_sc = (int)(sizeof(int) * P);
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_scatter<int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_sc));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_scatter<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_sc));
         const int lds_ps = (int)(sizeof(int) * (P + 1) + sizeof(int) * kStage);
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint8_t, int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint16_t, int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint8_t, uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint16_t, uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
     }
     MG_HIP(hipMalloc(&L.unitsA, sizeof(int4) * std::max<size_t>(1, ua.size())));
     MG_HIP(hipMalloc(&L.unitsB, sizeof(int4) * std::max<size_t>(1, ub.size())));
@@ -1091,25 +1173,35 @@ int mg_ps_sweep(mg_engine *e) {
     hipStream_t s = e->stream;
     const int P = L.P;
     const int G = e->opt.grid_blocks > 0 ? e->opt.grid_blocks : 2048;
-    int *msg = (int *)L.msg;
-    if (L.nS > 0)
-        hipLaunchKernelGGL(k_ps_scatter, dim3(grid_for(L.nS, 1, G)), dim3(kBT), sizeof(int) * P, s, e->ctl, 0, L.unitsS, L.nS, P, e->nv,
-                           L.np, L.inv, e->level, L.a_src, L.amap, msg, L.pairs);
+    const int force = e->pb_force ? 1 : 0;
+    e->pb_force = false;
     int slot = -1;
     // one full relaxation pass: 12E + 8V (+4)
     const double alg = 12.0 * (double)e->ne + 8.0 * e->nv + 4.0;
-    if (e->prof) mg_prof_begin(e, "k_ps_gather", alg, &slot);
     const size_t lds = sizeof(int) * (P + 1) + sizeof(int) * kStage;
-    if (L.w8)
-        hipLaunchKernelGGL(k_ps_gather<uint8_t>, dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P, e->nv, L.np,
-                           L.b_dst, (const uint8_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv, e->level, e->vfc);
-    else
-        hipLaunchKernelGGL(k_ps_gather<uint16_t>, dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P, e->nv, L.np,
-                           L.b_dst, (const uint16_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv, e->level, e->vfc);
-    if (e->prof) mg_prof_end(e, slot);
-    if (L.nA > 0)
-        hipLaunchKernelGGL(k_ps_scatter, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(int) * P, s, e->ctl, 1, L.unitsA, L.nA, P, e->nv,
-                           L.np, L.inv, e->level, L.a_src, L.amap, msg, L.pairs);
+    auto sweep = [&](auto *msg) {
+        using MT = std::remove_pointer_t<decltype(msg)>;
+        if (L.nS > 0)
+            hipLaunchKernelGGL(k_ps_scatter<MT>, dim3(grid_for(L.nS, 1, G)), dim3(kBT), sizeof(int) * P, s, e->ctl, 0, force, L.unitsS,
+                               L.nS, P, e->nv, L.np, L.inv, e->level, e->vfc, L.a_src, L.amap, msg, L.pairs);
+        if (e->prof) mg_prof_begin(e, "k_ps_gather", alg, &slot);
+        if (L.w8)
+            hipLaunchKernelGGL((k_ps_gather<uint8_t, MT>), dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P,
+                               e->nv, L.np, L.b_dst, (const uint8_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv,
+                               e->level, e->vfc);
+        else
+            hipLaunchKernelGGL((k_ps_gather<uint16_t, MT>), dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P,
+                               e->nv, L.np, L.b_dst, (const uint16_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv,
+                               e->level, e->vfc);
+        if (e->prof) mg_prof_end(e, slot);
+        if (L.nA > 0)
+            hipLaunchKernelGGL(k_ps_scatter<MT>, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(int) * P, s, e->ctl, 1, 0, L.unitsA,
+                               L.nA, P, e->nv, L.np, L.inv, e->level, e->vfc, L.a_src, L.amap, msg, L.pairs);
+    };
+    // 16-bit messages live in the second half of the message allocation (PageRank's
+    // second buffer; pairs is a multiple of 8, so it starts 32-byte aligned)
+    if (e->pb16) sweep((uint16_t *)((float *)L.msg + std::max<long long>(L.pairs, kG)));
+    else sweep((int *)L.msg);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }

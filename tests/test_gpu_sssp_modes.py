@@ -7,9 +7,9 @@ pytestmark = pytest.mark.gpu
 
 MODES = [dict(), dict(mode=1), dict(mode=2), dict(graph_levels=3), dict(mode=2, graph_levels=2), dict(alpha=1000.0),
          dict(pull_layout=1), dict(mode=2, pull_layout=1), dict(mode=2, pull_layout=1, graph_levels=4), dict(part_log2=6),
-         dict(mode=2, part_log2=5)]
+         dict(mode=2, part_log2=5), dict(pull_layout=2), dict(mode=2, pull_layout=2)]
 IDS = ["hybrid", "push", "pull", "graph", "pull-graph", "eager-pull-switch", "csr", "csr-pull", "csr-pull-graph",
-       "small-parts", "pull-tiny-parts"]
+       "small-parts", "pull-tiny-parts", "wide-msgs", "wide-msgs-pull"]
 
 
 @pytest.mark.parametrize("opts", MODES, ids=IDS)
@@ -81,4 +81,37 @@ def test_sssp_weight_widths(mg, orc, wmax):
     for s in (0, g.nv // 2):
         e.run(s)
         np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.sssp_bf(row, col, w, s)[0], err_msg=f"wmax {wmax} src {s}")
+    e.close()
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(mode=2), dict(mode=2, graph_levels=3), dict(mode=2, pull_layout=2),
+                                  dict(mode=2, part_log2=6)],
+                         ids=["hybrid", "pull", "pull-graph", "pull-wide", "pull-small-parts"])
+def test_sssp_message_widths(mg, orc, opts):
+    """layout sweeps send 16-bit distance messages until a finite distance above 65534
+    must be sent: that vertex is stamped (it stays in the frontier) and the run goes
+    on with 32-bit messages; captured sweeps and pull_layout=2 use 32-bit throughout.
+    Weights up to 30000 push distances past 65534 within a few hops; a 1,500-vertex
+    path with weight-64 edges does it after ~1,000 hops, mid-run."""
+    rng = np.random.default_rng(41)
+    n = 2000
+    src = rng.integers(0, n, 16000)
+    dst = rng.integers(0, n, 16000)
+    val = rng.integers(1, 30001, 16000)
+    k = 1500
+    src = np.concatenate([src, np.arange(n, n + k - 1)])
+    dst = np.concatenate([dst, np.arange(n + 1, n + k)])
+    val = np.concatenate([val, np.full(k - 1, 64)])
+    src = np.concatenate([src, [0]])
+    dst = np.concatenate([dst, [n]])
+    val = np.concatenate([val, [5]])
+    g = mg.Graph.from_coo(n + k, src, dst, val)
+    row, col, w = g.csr()
+    e = mg.Engine(g, mg.SSSP, **opts)
+    for s in (0, g.nv - 1, 7):
+        print(f"sssp widths {opts} src={s}", flush=True)
+        e.run(s)
+        ref, _ = orc.sssp_bf(row, col, w, s)
+        assert ref[ref != 0x64646464].max() > 65535
+        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref, err_msg=f"{opts} source {s}")
     e.close()
