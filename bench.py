@@ -79,7 +79,7 @@ def measured_traffic(kernel, scale):
         return None
     with open(path) as f:
         ks = json.load(f).get("kernels", {})
-    # the trace names template instantiations ("k_ps_gather<unsigned char>")
+    # the trace names template instantiations ("k_ps_gather<unsigned char, unsigned short>")
     k = ks.get(kernel) or next((v for n, v in ks.items() if n.split("<")[0] == kernel), None)
     return float(k["bytes"]) if k else None
 

@@ -92,6 +92,21 @@ def counters(d, grid=0):
     return {k: list(v.values()) for k, v in out.items()}
 
 
+def split_names(spec):
+    """comma-separated kernel names; commas inside template arguments belong to the name"""
+    out, depth, cur = [], 0, ""
+    for ch in spec:
+        depth += (ch == "<") - (ch == ">")
+        if ch == "," and depth == 0:
+            out.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur.strip())
+    return out
+
+
 def traffic(a):
     fetch, write = counters(a.fetch, a.grid), counters(a.write, a.grid)
     cf, cw = counters(a.copy_fetch), counters(a.copy_write)
@@ -104,7 +119,7 @@ def traffic(a):
     print(f"copy calibration: {a.copy_mib} MiB read + written per launch; FETCH_SIZE reads "
           f"{med(cf['k_copy']) * 1024 / known:.4f}x and WRITE_SIZE {med(cw['k_copy']) * 1024 / known:.4f}x of it "
           f"(scale factors {f_scale:.4f}, {w_scale:.4f})\n")
-    want = a.kernels.split(",") if a.kernels else sorted(set(fetch) | set(write))
+    want = split_names(a.kernels) if a.kernels else sorted(set(fetch) | set(write))
     res = {"copy_mib": a.copy_mib, "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
     print("| kernel | dispatches | fetched MB / launch | written MB / launch | HBM MB / launch |")
     print("|---|---:|---:|---:|---:|")
