@@ -57,7 +57,17 @@ COMPLEX = """%%MatrixMarket matrix coordinate complex general
 """
 
 
-@pytest.mark.parametrize("text", [GENERAL, SYMREAL, COMPLEX], ids=["general", "symreal", "complex"])
+# Hermitian header: read as symmetric (read_input.cu:31), real part of each value
+HERMITIAN = """%%MatrixMarket matrix coordinate complex Hermitian
+5 5 4
+2 1 3.5 -2.0
+3 1 -7.75 1.0
+4 3 12.0 0.5
+5 2 1.25 0.0
+"""
+
+
+@pytest.mark.parametrize("text", [GENERAL, SYMREAL, COMPLEX, HERMITIAN], ids=["general", "symreal", "complex", "hermitian"])
 @pytest.mark.parametrize("flags", ["SYM", "NONE", "HEADER", "SYM|CC"])
 @pytest.mark.parametrize("source", [0, 2, 5])
 def test_reader_matches_restatement(mg, tmp_path, text, flags, source):
