@@ -146,9 +146,13 @@ __global__ void k_cc_init(int nv, int *__restrict__ parent) {
 
 // Edge-balanced hooking: workgroup c takes edges [c*kEPB, (c+1)*kEPB) of the
 // upper-triangular CSR; owners come from chunk_owners (lb.hpp).
+constexpr int kGiantSamples = 512;  // of k_cc_sample's 1024: the giant tree is "most" of the graph
+// gate (optional, k_cc_sample's output): nothing to do when the sampled giant tree
+// held at least half the samples (k_cc_link_rest hooked the edges instead)
 __global__ __launch_bounds__(kBlock) void k_cc_hook_lb(int nch, long long ne, const int *__restrict__ off,
                                                        const int *__restrict__ adj, const int *__restrict__ cstart,
-                                                       int *__restrict__ parent) {
+                                                       int *__restrict__ parent, const int *__restrict__ gate) {
+    if (gate && gate[1] >= kGiantSamples) return;
     __shared__ int s_own[kEPB];
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const int e0 = c * kEPB;
@@ -208,10 +212,13 @@ __global__ __launch_bounds__(1024) void k_cc_sample(int nv, int *__restrict__ pa
     }
 }
 
-// hook the remaining neighbours of every vertex outside the sampled giant tree
+// hook the remaining neighbours of every vertex outside the sampled giant tree, when
+// that tree held at least half the samples (giant = {root, samples}); otherwise
+// k_cc_hook_lb hooks every edge
 __global__ __launch_bounds__(kBlock) void k_cc_link_rest(int nv, int k, const int *__restrict__ row,
                                                          const int *__restrict__ col, const int *__restrict__ giant,
                                                          int *__restrict__ parent) {
+    if (giant[1] < kGiantSamples) return;
     __shared__ int s_wv[kBlock];
     __shared__ int s_wn;
     const int c = giant[0];
@@ -441,7 +448,7 @@ int mg_cc_step(mg_engine *e, int *more) {
     const int nv = e->nv;
     int slot = -1;
     bool hooked = false;
-    if (e->cc_afforest) {
+    if (e->cc_afforest) {  // no host round trip: the last two hooks read the sample on the device
         const int k = 2;
         if (e->prof) mg_prof_begin(e, "k_cc_link_min", 16.0 * nv, &slot);
         hipLaunchKernelGGL(k_cc_link_min, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->row, e->col, e->level);
@@ -452,21 +459,15 @@ int mg_cc_step(mg_engine *e, int *more) {
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->level);
         hipLaunchKernelGGL(k_cc_sample, dim3(1), dim3(1024), 0, s, nv, e->level, e->aux);
-        int best[2] = {0, 0};
-        MG_HIP(hipMemcpyAsync(best, e->aux, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-        MG_HIP(hipStreamSynchronize(s));
-        e->cc_giant_frac = best[1] / 1024.0;
-        if (best[1] >= 512) {
-            if (e->prof) mg_prof_begin(e, "k_cc_link_rest", 8.0 * nv, &slot);
-            hipLaunchKernelGGL(k_cc_link_rest, dim3(grid_for(nv, kBlock, G)), dim3(kBlock), 0, s, nv, k, e->row, e->col, e->aux, e->level);
-            if (e->prof) mg_prof_end(e, slot);
-            hooked = true;
-        }
+        if (e->prof) mg_prof_begin(e, "k_cc_link_rest", 8.0 * nv, &slot);
+        hipLaunchKernelGGL(k_cc_link_rest, dim3(grid_for(nv, kBlock, G)), dim3(kBlock), 0, s, nv, k, e->row, e->col, e->aux, e->level);
+        if (e->prof) mg_prof_end(e, slot);
+        hooked = true;
     }
-    if (!hooked && e->chunk_cap > 0) {
+    if (e->chunk_cap > 0) {  // after the sampling pass: only when its giant tree was too small
         if (e->prof) mg_prof_begin(e, "k_cc_hook_lb", 4.0 * e->ne_u + 4.0 * (nv + 1) + 8.0 * e->ne_u, &slot);
         hipLaunchKernelGGL(k_cc_hook_lb, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, (int)e->chunk_cap, e->ne_u,
-                           e->urow, e->ucol, e->cstart, e->level);
+                           e->urow, e->ucol, e->cstart, e->level, hooked ? (const int *)e->aux : (const int *)nullptr);
         if (e->prof) mg_prof_end(e, slot);
     }
     if (e->prof) mg_prof_begin(e, "k_cc_compress", 8.0 * nv, &slot);

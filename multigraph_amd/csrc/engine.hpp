@@ -205,7 +205,6 @@ struct mg_engine {
     int *urow = nullptr, *ucol = nullptr;
     long long ne_u = 0;
     bool cc_afforest = false;
-    double cc_giant_frac = 0.0;
     // PageRank
     float *rank = nullptr, *cval = nullptr;
     int *deg = nullptr;
