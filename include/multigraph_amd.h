@@ -216,6 +216,10 @@ int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32_t *sources
  * from HIP events around every launch on the engine stream when profiling is on)
  * and algorithmic bytes.  name is the kernel's function name. */
 int mg_set_profiling(mg_engine *e, int on);
+/* Profile only the kernels whose name starts with prefix (NULL or "": all), so a timed
+ * run pays events around those launches alone.  No reference counterpart (the
+ * reference times whole iterations, process.cu:1300-1305). */
+int mg_set_profile_filter(mg_engine *e, const char *prefix);
 int mg_kernel_stats(mg_engine *e, int idx, char *name, int name_len, int64_t *launches,
                     double *ms, double *bytes);
 

@@ -101,6 +101,7 @@ _SIGS = {
     "mg_batch_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, _I32P, C.c_int, C.POINTER(Options),
                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mg_set_profiling": (C.c_int, [_P, C.c_int]),
+    "mg_set_profile_filter": (C.c_int, [_P, C.c_char_p]),
     "mg_kernel_stats": (C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
 }
@@ -321,7 +322,9 @@ class Engine:
         _check(lib().mg_get_result(self._h, which, out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 
-    def set_profiling(self, on: bool) -> None:
+    def set_profiling(self, on: bool, only: str | None = None) -> None:
+        """per-kernel HIP-event accounting; `only`: a kernel-name prefix to restrict it to"""
+        _check(lib().mg_set_profile_filter(self._h, only.encode() if only else None))
         _check(lib().mg_set_profiling(self._h, 1 if on else 0))
 
     def kernel_stats(self) -> dict:

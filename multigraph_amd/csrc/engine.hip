@@ -273,6 +273,8 @@ extern "C" int mg_get_result(mg_engine *e, int which, void *dst, int64_t bytes) 
 
 // --------------------------------------------------------------------------- profiling
 int mg_prof_begin(mg_engine *e, const char *name, double bytes, int *slot) {
+    *slot = -1;
+    if (!e->prof_only.empty() && strncmp(name, e->prof_only.c_str(), e->prof_only.size()) != 0) return MG_OK;
     int idx = -1;
     for (size_t i = 0; i < e->stats.size(); ++i) if (e->stats[i].name == name) { idx = (int)i; break; }
     if (idx < 0) { e->stats.push_back(KStat{name, 0, 0.0, 0.0}); idx = (int)e->stats.size() - 1; }
@@ -316,6 +318,14 @@ extern "C" int mg_set_profiling(mg_engine *e, int on) {
     if (!e) return mg_fail(MG_ERR_ARG, "null engine");
     MG_TRY(mg_prof_flush(e));
     e->prof = on != 0;
+    e->stats.clear();
+    return MG_OK;
+}
+
+extern "C" int mg_set_profile_filter(mg_engine *e, const char *prefix) {
+    if (!e) return mg_fail(MG_ERR_ARG, "null engine");
+    MG_TRY(mg_prof_flush(e));
+    e->prof_only = prefix ? prefix : "";
     e->stats.clear();
     return MG_OK;
 }

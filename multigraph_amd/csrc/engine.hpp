@@ -226,6 +226,7 @@ struct mg_engine {
     int gexec_levels = 0;
     // profiling
     bool prof = false;
+    std::string prof_only;     // profiling: only kernels whose name starts with this (empty: all)
     std::vector<mg::KStat> stats;
     std::vector<hipEvent_t> ev_pool;
     struct Pending { int stat; hipEvent_t a, b; double bytes; };
