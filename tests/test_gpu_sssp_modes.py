@@ -115,3 +115,23 @@ def test_sssp_message_widths(mg, orc, opts):
         assert ref[ref != 0x64646464].max() > 65535
         np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref, err_msg=f"{opts} source {s}")
     e.close()
+
+
+def test_profile_filter(mg, orc):
+    """per-kernel accounting: every kernel by default, only the named prefix with a
+    filter; the run's result is the same either way"""
+    g = mg.Graph.rmat(14, 16, seed=3)
+    row, col, w = g.csr()
+    s = g.max_degree_vertex()
+    e = mg.Engine(g, mg.SSSP)
+    e.set_profiling(True)
+    e.run(s)
+    full = e.kernel_stats()
+    assert "k_ps_gather" in full and len(full) > 1
+    e.set_profiling(True, only="k_ps_gather")
+    e.run(s)
+    only = e.kernel_stats()
+    assert set(only) == {"k_ps_gather"} and only["k_ps_gather"]["launches"] == full["k_ps_gather"]["launches"]
+    np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.sssp_bf(row, col, w, s)[0])
+    e.set_profiling(False)
+    e.close()
