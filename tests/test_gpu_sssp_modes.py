@@ -123,7 +123,7 @@ def test_profile_filter(mg, orc):
     g = mg.Graph.rmat(14, 16, seed=3)
     row, col, w = g.csr()
     s = g.max_degree_vertex()
-    e = mg.Engine(g, mg.SSSP)
+    e = mg.Engine(g, mg.SSSP, mode=2)  # every level a layout sweep
     e.set_profiling(True)
     e.run(s)
     full = e.kernel_stats()
