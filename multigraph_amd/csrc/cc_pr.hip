@@ -185,7 +185,6 @@ __global__ void k_cc_link_second(int nv, const int *__restrict__ row, const int 
 // the most frequent root among 1024 evenly spaced vertices (bitonic sort in LDS)
 __global__ __launch_bounds__(1024) void k_cc_sample(int nv, int *__restrict__ parent, int *__restrict__ out) {
     __shared__ int s[1024];
-    __shared__ int s_best, s_cnt;
     const int t = threadIdx.x;
     const long long v = ((long long)t * 2654435761ll) % nv;
     s[t] = CCHooks::find_ro(parent, (int)v);
@@ -206,7 +205,6 @@ __global__ __launch_bounds__(1024) void k_cc_sample(int nv, int *__restrict__ pa
             if (s[i] == cur) ++cc; else { cur = s[i]; cc = 1; }
             if (cc > bc) { bc = cc; best = cur; }
         }
-        s_best = best; s_cnt = bc;
         out[0] = best;
         out[1] = bc;
     }

@@ -28,7 +28,7 @@
 
 
 // ------------------------------------------------------------------ certificates
-static int verify_levels(const csc_package *inp, const std::vector<int> &L, int src, const int *w) {
+[[maybe_unused]] static int verify_levels(const csc_package *inp, const std::vector<int> &L, int src, const int *w) {
     const int nv = inp->nv;
     if (nv == 0) return -1;
     if (L[src] != 0) return src;
@@ -46,7 +46,7 @@ static int verify_levels(const csc_package *inp, const std::vector<int> &L, int 
     return -1;
 }
 
-static int verify_cc(const csc_package *inp, const std::vector<int> &lab) {
+[[maybe_unused]] static int verify_cc(const csc_package *inp, const std::vector<int> &lab) {
     for (int v = 0; v < inp->nv; ++v) {
         if (lab[v] < 0 || lab[v] > v || lab[lab[v]] != lab[v]) return v;
         for (int p = inp->csc_v[v]; p < inp->csc_v[v + 1]; ++p)
