@@ -367,11 +367,24 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
             int steps = alone ? e - b : 0;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
-            for (int k = 0; k < steps; ++k) {
-                if (b + k < e && alone) {
-                    const int u = ld_stream(&col[b + k]);
-                    if (bm_test(fr, u)) { found = true; s += sigma[u]; }
+            // four edges per step, their column, bitmap and path-count loads issued
+            // together (three dependent loads per edge otherwise serialise)
+            for (int k = 0; k < steps; k += 4) {
+                int u[4];
+                bool f[4];
+                double sv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    f[j] = alone && b + k + j < e;
+                    u[j] = f[j] ? ld_stream(&col[b + k + j]) : 0;
                 }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) f[j] = f[j] && bm_test(fr, u[j]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sv[j] = f[j] ? sigma[u[j]] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (f[j]) { found = true; s += sv[j]; }
             }
             uint64_t heavy = ballot(!alone);
             while (heavy) {
