@@ -21,6 +21,7 @@
 //     distance at v dropped in the last sweep; lane s relaxes dist[v][s] from the
 //     neighbours whose changed bit s is set.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 #include "engine.hpp"
@@ -177,6 +178,7 @@ struct BCtl {
     int qn[2];    // queue lengths of F[0], F[1]
     int qvalid[2];    // the queue of F[b] lists its nonzero words
     int ndense, nsparse, npersist;  // levels run: dense, sparse, sparse by persistent launches
+    int seq;      // host mirror only: the host check whose block it holds
     int thr;      // SSSP: sparse sweeps relax the (vertex, source) pairs below this distance
     long long n, mf;  // size and degree sum of the last produced frontier
     unsigned long long all;  // the slots in use
@@ -549,6 +551,16 @@ __global__ void k_mb_ctl_sparse(BCtl *ctl, const int *__restrict__ pn, const lon
     if (!c.done && (!c.sparse || c.compact || c.fill)) { ctl->need_full = 1; return; }
     if (!c.done) ++c.nsparse;
     *ctl = c;
+}
+
+// the host's pinned copy of the control block after a replay: the block, then (after a
+// system-scope fence) the check's sequence number, on which the host spins
+__global__ void k_mb_mirror(const BCtl *ctl, BCtl *mirror, int seq) {
+    BCtl c = *ctl;
+    c.seq = seq - 1;
+    *mirror = c;
+    __threadfence_system();
+    __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---- sparse levels (queue of the frontier's vertices) -------------------------------
@@ -1233,6 +1245,7 @@ struct mg_batch {
     int persist_grid = 0;         // workgroups of its cooperative launch (0: not available)
     long long stats[5] = {0, 0, 0, 0, 0};  // of the last BFS / SSSP run (mg_batch_stats)
     int auto_delta = 0;           // SSSP near-far band by default: a quarter of the mean weight
+    int mirror_seq = 0;           // host checks of the control block so far (k_mb_mirror)
     int cus = 256;                // compute units of the device
 };
 
@@ -1321,7 +1334,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
               hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess &&
               hipMalloc(&b->ctl, sizeof(BCtl)) == hipSuccess && hipMalloc(&b->pc, sizeof(PCtl)) == hipSuccess &&
               hipMalloc(&b->Q[0], sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->Q[1], sizeof(int) * (nv + 1)) == hipSuccess &&
-              hipHostMalloc(&b->h_ctl, sizeof(BCtl), hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc(&b->h_ctl, sizeof(BCtl), hipHostMallocCoherent) == hipSuccess;  // written by k_mb_mirror
     if (ok && kind != MG_BATCH_SSSP) ok = hipMalloc(&b->lev, sizeof(int) * slots) == hipSuccess;
     if (ok && kind == MG_BATCH_SSSP) {
         ok = hipMalloc(&b->w, sizeof(int) * ne) == hipSuccess && hipMalloc(&b->dist, sizeof(int) * slots) == hipSuccess;
@@ -1621,8 +1634,21 @@ static int run_levels(mg_batch *b) {
         } else {
             MG_HIP(hipGraphLaunch(sparse_next ? b->gsparse : b->gexec, s));
         }
-        MG_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s));
-        MG_HIP(hipStreamSynchronize(s));
+        // the control block through the pinned mirror: spin on its sequence number
+        // (after ~2 ms, block on the stream, which also reports a failed kernel)
+        const int seq = ++b->mirror_seq;
+        hipLaunchKernelGGL(k_mb_mirror, dim3(1), dim3(1), 0, s, (const BCtl *)b->ctl, b->h_ctl, seq);
+        MG_HIP(hipGetLastError());
+        {
+            const auto t0 = std::chrono::steady_clock::now();
+            for (unsigned k = 0; __atomic_load_n(&b->h_ctl->seq, __ATOMIC_ACQUIRE) != seq; ++k)
+                if ((k & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+                    MG_HIP(hipStreamSynchronize(s));
+                    if (__atomic_load_n(&b->h_ctl->seq, __ATOMIC_ACQUIRE) != seq)
+                        MG_HIP(hipMemcpy(b->h_ctl, b->ctl, sizeof(BCtl), hipMemcpyDeviceToHost));
+                    break;
+                }
+        }
         ++trips;
         if (b->h_ctl->done) break;
     }
