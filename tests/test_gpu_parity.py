@@ -13,7 +13,7 @@ SMALL = {}
 def graphs(mg):
     if SMALL:
         return SMALL
-    SMALL["rmat16"] = mg.Graph.rmat(16, 16, seed=1)                     # the C1 workload
+    SMALL["rmat16"] = mg.Graph.rmat(16, 16, seed=1)                     # BFS from vertex 0, RMAT-16
     SMALL["rmat12_sparse"] = mg.Graph.rmat(12, 2, seed=5)               # many components
     SMALL["grid64"] = mg.Graph.grid(64, 48)
     n = 300
