@@ -1236,7 +1236,6 @@ struct mg_batch {
     int *part_n = nullptr, *part_x = nullptr;  // per-workgroup frontier size, largest degree
     long long *part_e = nullptr, *tot = nullptr, *h_tot = nullptr;
     int nsrc = 0, levels = 0;     // slots of the last run, levels it swept
-    bool symmetric = true;        // every edge stored both ways (no transposed CSR)
     BCtl *ctl = nullptr, *h_ctl = nullptr;  // device level control (BFS, SSSP)
     hipGraphExec_t gexec = nullptr;         // K captured levels (all level kinds)
     hipGraphExec_t gsparse = nullptr;       // K captured sparse levels (control + fused kernel)
@@ -1276,7 +1275,6 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     b->nv = g->nv;
     b->ne = g->ne;
     b->max_sources = max_sources;
-    b->symmetric = g->symmetric;
     if (opt) b->opt = *opt; else mg_default_options(&b->opt);
     if (b->opt.grid_blocks > 4096) b->opt.grid_blocks = 4096;  // per-workgroup partial arrays hold 4096
     auto fail = [&](int rc) { batch_free(b); return rc; };

@@ -191,7 +191,6 @@ struct mg_engine {
     long long *part_mf = nullptr;
     int *lvl_off = nullptr;    // BC level log offsets (levels + 2)
     long long *lvl_mf = nullptr;  // BC: degree sum of every level
-    bool symmetric = true;     // every stored edge has its reverse (rows double as in-edges)
     int *aux = nullptr;        // small device counters (BC split-vertex chunk count)
     mg::Ctl *ctl = nullptr;
     mg::Ctl *h_ctl = nullptr;  // pinned mirror

@@ -1010,7 +1010,6 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     P.log_queue = (e->algo == MG_SSSP) ? 0 : 1;
     // pull levels read in-edges: the CSR itself when symmetric, else its transpose
     P.forced = e->opt.mode;
-    e->symmetric = g->symmetric;
     P.sssp = e->algo == MG_SSSP ? 1 : 0;
     P.need_log = e->algo == MG_BC ? 1 : 0;
     P.huge = e->opt.huge_degree;
