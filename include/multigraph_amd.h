@@ -211,6 +211,21 @@ int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32_t *sources
                       const int32_t *slots, int reps, mg_options *best, float *ms_default,
                       float *ms_tuned);
 
+/* ---- allocation-free entry point ------------------------------------------- */
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The BFS
+ * of mg_build + mg_run (same kernels, same levels) over a CSR the caller already holds on
+ * the current device: row[nv + 1] int32 offsets, col[ne] int32, symmetric unless mode is
+ * MG_MODE_SPARSE (pull levels read rows as in-edges); levels land in the caller's device
+ * array level[nv] (MG_INF = unreached).  The queue, bitmaps, degree-scan tables, partial
+ * counters and control block live in one fixed device array of the library (about 40 MB,
+ * held by every GPU the library runs on until the process ends), so the call allocates
+ * nothing; nv above 4,194,304 or ne above 2^31 - 1 is refused with MG_ERR_LIMIT.  `stream`
+ * is a hipStream_t (NULL: the null stream); the call returns when the levels are written.
+ * Calls that overlap on one GPU share that array and race: serialise them. */
+int mg_bfs_noalloc(int nv, int64_t ne, const int32_t *row, const int32_t *col, int source, int mode,
+                   int32_t *level, void *stream);
+
 /* ---- instrumentation -------------------------------------------------------- */
 /* Per-kernel accounting of the last mg_run: launches, summed device time (ms,
  * from HIP events around every launch on the engine stream when profiling is on)

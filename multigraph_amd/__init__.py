@@ -100,6 +100,7 @@ _SIGS = {
                               C.POINTER(C.c_float)]),
     "mg_batch_autotune": (C.c_int, [_P, C.c_int, C.c_int, _I32P, _I32P, C.c_int, C.POINTER(Options),
                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mg_bfs_noalloc": (C.c_int, [C.c_int, C.c_int64, _P, _P, C.c_int, C.c_int, _P, _P]),
     "mg_set_profiling": (C.c_int, [_P, C.c_int]),
     "mg_set_profile_filter": (C.c_int, [_P, C.c_char_p]),
     "mg_kernel_stats": (C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
@@ -427,3 +428,13 @@ def batch_autotune(graph: Graph, kind: int, sources, slots=None, reps: int = 3, 
     _check(lib().mg_batch_autotune(graph._h, kind, len(src), _ptr(src), _ptr(sl) if sl is not None else None, reps,
                                    C.byref(o), C.byref(a), C.byref(b)))
     return o, a.value, b.value
+
+
+def bfs_noalloc(nv: int, ne: int, d_row: int, d_col: int, source: int, d_level: int, mode: int = MODE_HYBRID,
+                stream: int = 0) -> None:
+    """BFS levels into the device buffer d_level (nv int32) over the device CSR d_row / d_col
+    (int32), with no device allocation: the library's fixed scratch array holds the run state.
+    Pointers are device addresses as integers; calls on one device must not overlap.  Pull
+    levels read rows as in-edges: a directed CSR needs mode=MODE_SPARSE."""
+    _check(lib().mg_bfs_noalloc(nv, ne, C.c_void_p(d_row), C.c_void_p(d_col), source, mode, C.c_void_p(d_level),
+                                C.c_void_p(stream) if stream else None))

@@ -194,6 +194,7 @@ struct mg_engine {
     int *aux = nullptr;        // small device counters (BC split-vertex chunk count)
     mg::Ctl *ctl = nullptr;
     mg::Ctl *h_ctl = nullptr;  // pinned mirror
+    bool mirror = true;        // k_ctl writes h_ctl itself (false: h_ctl is pageable, copied back)
     int ctl_seq = 0;           // levels whose control block k_ctl mirrored (eager steps)
     // SSSP layout sweeps: 16-bit distance messages while every finite distance sent
     // fits (pb16 for the current run, pb16_ok when the engine allows them), 32-bit

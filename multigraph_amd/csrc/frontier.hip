@@ -18,6 +18,7 @@
 // All kernels are grid-stride and read their sizes from the device control block,
 // so a run of levels can be captured in a hipGraph and replayed with no host sync.
 #include <chrono>
+#include <mutex>
 
 #include "engine.hpp"
 #include "user_function.hpp"
@@ -327,7 +328,7 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // The new frontier is the bitmap itself; its size and degree sum leave as one
 // partial per workgroup (no global atomics), summed by k_reduce_parts.
 template <bool kBC>
-__global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__restrict__ row,
+__global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, const int *__restrict__ row,
                                                  const int *__restrict__ col, const int *__restrict__ orow,
                                                  int *__restrict__ level,
                                                  double *__restrict__ sigma, uint64_t *__restrict__ vis,
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, const int *__
             b = row[v];
             e = row[v + 1];
             deg = e - b;
-            if (e - b > kPullHuge) e = b;  // left to k_pull_huge
+            if (e - b > cut) e = b;  // left to k_pull_huge
             if (!kBC) {
                 for (int p = b; p < e; ++p)
                     if (bm_test(fr, ld_stream(&col[p]))) { found = true; break; }
@@ -1137,10 +1138,13 @@ static int launch_level(mg_engine *e, bool known) {
         const double bytes = known ? (8.0 * nv / 64.0 * 3 + 4.0 * (double)c.mu + 8.0 * nv) : 0.0;
         if (e->prof) mg_prof_begin(e, "k_pull", bytes, &slot);
         const int *orow = e->trow ? e->row : nullptr;
+        // rows above kPullHuge edges go to k_pull_huge's chunks; an engine without
+        // chunks (none was that long, or no table was built) walks every row here
+        const int cut = e->pull_ch ? kPullHuge : 0x7FFFFFFF;
         if (e->algo == MG_BC)
-            hipLaunchKernelGGL((k_pull<true>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            hipLaunchKernelGGL((k_pull<true>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         else
-            hipLaunchKernelGGL((k_pull<false>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            hipLaunchKernelGGL((k_pull<false>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         if (e->pull_nch > 0) {
             const int gh = grid_for(e->pull_nch, kBlock / MG_WAVE, G);
             if (e->algo == MG_BC)
@@ -1156,8 +1160,9 @@ static int launch_level(mg_engine *e, bool known) {
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
     }
-    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf, known ? e->h_ctl : (Ctl *)nullptr,
-                       known ? ++e->ctl_seq : 0);
+    const bool mirror = known && e->mirror;
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf, mirror ? e->h_ctl : (Ctl *)nullptr,
+                       mirror ? ++e->ctl_seq : 0);
     if (!known) MG_TRY(launch_after_ctl(e));
     MG_HIP(hipGetLastError());
     return MG_OK;
@@ -1290,5 +1295,125 @@ int mg_bc_backward_impl(mg_engine *e) {
         hipLaunchKernelGGL(k_bc_back_fin, dim3(64), dim3(256), 0, s, e->chunks, nch, e->dacc, e->sigma, e->delta, e->bc_cw);
     }
     MG_HIP(hipGetLastError());
+    return MG_OK;
+}
+
+// ------------------------------------------------------------ allocation-free BFS
+// mg_bfs_noalloc: the BFS of mg_build + mg_run for callers that cannot allocate.  The
+// CSR and the level array are the caller's device buffers; everything else the level
+// kernels need (queue, degree-scan tables, bitmaps, partial counters, control block)
+// lives in one fixed device array, carved as below.  The array's address is a host
+// stand-in until hipGetSymbolAddress resolves it on the current device, once per device.
+namespace {
+constexpr int kNaMaxNv = 1 << 22;                 // vertices
+constexpr long long kNaMaxNe = 0x7FFFFFFFLL;      // edges (int32 row offsets)
+constexpr int kNaGrid = 2048;                     // workgroups of the grid-stride kernels
+constexpr size_t na_al(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kNaQ = na_al(sizeof(int) * ((size_t)kNaMaxNv + 2));                 // queue / level log
+constexpr size_t kNaEoff = kNaQ;                                                      // tile-local degree offsets
+constexpr size_t kNaTsum = na_al(sizeof(int) * ((size_t)(kNaMaxNv + 2) / kTile + 2)); // tile sums
+constexpr size_t kNaCst = na_al(sizeof(int) * ((size_t)(kNaMaxNe / kEPB) + 4));      // chunk owners
+constexpr size_t kNaBm = na_al(sizeof(uint64_t) * ((size_t)kNaMaxNv / 64 + 1));      // one bitmap
+constexpr size_t kNaPn = na_al(sizeof(int) * (kNaGrid + 8));
+constexpr size_t kNaPm = na_al(sizeof(long long) * (kNaGrid + 8));
+constexpr size_t kNaCtl = na_al(sizeof(Ctl));
+constexpr size_t kNaBytes = kNaQ + kNaEoff + kNaTsum + kNaCst + 3 * kNaBm + kNaPn + kNaPm + kNaCtl;
+constexpr int kNaMaxDev = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+static std::mutex g_na_mu;
+static void *g_na_ptr[kNaMaxDev];
+
+// device address of `scratch` on the current device (resolved once per device)
+static int na_scratch(int dev, unsigned char **out) {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return mg_fail(MG_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(err));
+        g_na_ptr[dev] = p;
+    }
+    *out = (unsigned char *)g_na_ptr[dev];
+    return MG_OK;
+}
+
+extern "C" int mg_bfs_noalloc(int nv, int64_t ne, const int32_t *row, const int32_t *col, int source, int mode,
+                              int32_t *level, void *stream) {
+    if (nv < 0 || ne < 0) return mg_fail(MG_ERR_ARG, "negative size (nv %d, ne %lld)", nv, (long long)ne);
+    if (nv > kNaMaxNv || ne > kNaMaxNe)
+        return mg_fail(MG_ERR_LIMIT, "mg_bfs_noalloc takes at most %d vertices and %lld edges (got %d, %lld): its scratch "
+                       "is a fixed %zu-byte device array", kNaMaxNv, kNaMaxNe, nv, (long long)ne, kNaBytes);
+    if (nv == 0) return MG_OK;
+    if (!row || !level || (ne > 0 && !col)) return mg_fail(MG_ERR_ARG, "null device pointer");
+    if (source < 0 || source >= nv) return mg_fail(MG_ERR_ARG, "source %d outside [0,%d)", source, nv);
+    if (mode != MG_MODE_HYBRID && mode != MG_MODE_SPARSE && mode != MG_MODE_DENSE)
+        return mg_fail(MG_ERR_ARG, "unknown mode %d", mode);
+    int dev = 0;
+    MG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDev) return mg_fail(MG_ERR_LIMIT, "device %d beyond %d", dev, kNaMaxDev);
+    unsigned char *base = nullptr;
+    MG_TRY(na_scratch(dev, &base));
+
+    // an engine on the stack whose buffers are the caller's and the scratch array's
+    mg_engine eng;
+    mg_engine *e = &eng;
+    Ctl hc{};
+    e->algo = MG_BFS;
+    e->nv = nv;
+    e->ne = ne;
+    e->stream = (hipStream_t)stream;
+    e->row = const_cast<int *>(row);
+    e->col = const_cast<int *>(col);
+    e->level = level;
+    unsigned char *p = base;
+    e->q = (int *)p;             p += kNaQ;
+    e->eoff = (int *)p;          p += kNaEoff;
+    e->tsum = (int *)p;          p += kNaTsum;
+    e->cstart = (int *)p;        p += kNaCst;
+    e->bm[0] = (uint64_t *)p;    p += kNaBm;
+    e->bm[1] = (uint64_t *)p;    p += kNaBm;
+    e->vis = (uint64_t *)p;      p += kNaBm;
+    e->part_n = (int *)p;        p += kNaPn;
+    e->part_mf = (long long *)p; p += kNaPm;
+    e->ctl = (Ctl *)p;
+    e->qcap = (long long)nv + 2;
+    e->h_ctl = &hc;
+    e->mirror = false;
+    e->opt.mode = mode;
+    e->opt.alpha = 15.0f;
+    e->opt.beta = 18.0f;
+    e->opt.grid_blocks = kNaGrid;
+    Params &P = e->P;
+    P.nv = nv;
+    P.ne = ne;
+    P.log_queue = 1;
+    P.forced = mode;
+    P.sssp = 0;
+    P.need_log = 0;
+    P.alpha = e->opt.alpha;
+    P.beta = e->opt.beta;
+
+    // reset (resetValue): every level unreached, the source at level 0, queue {source}
+    hipStream_t s = e->stream;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, level, (long long)nv, (int)MG_INF);
+    int rr[2] = {0, 0};
+    MG_HIP(hipMemcpyAsync(rr, row + source, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(k_seed, dim3(1), dim3(1), 0, s, e->ctl, level, e->q, (double *)nullptr, source, rr[1] - rr[0], ne, 1);
+    MG_HIP(hipGetLastError());
+    MG_HIP(hipMemcpyAsync(&hc, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+    // levels: the control block comes back by a copy into host memory after each level
+    for (;;) {
+        MG_TRY(launch_level(e, true));
+        MG_HIP(hipMemcpyAsync(&hc, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        MG_HIP(hipStreamSynchronize(s));
+        if (hc.mode == MODE_DONE) break;
+        if (hc.need_queue) MG_TRY(launch_after_ctl(e));
+    }
+    MG_HIP(hipStreamSynchronize(s));
     return MG_OK;
 }
