@@ -15,16 +15,24 @@ The JSON line also carries
     `kernels` and checks that it is the dominant one;
   * cpu_baseline: the oracle's single-thread restatement of code/validate.cu (SSSP
     Bellman-Ford + CC label propagation) on the same graph, timed on this host;
-  * per_algo: BFS, BC, PR_T, PR_D on RMAT-22 with GTEPS and the fraction of HBM peak
-    their algorithmic bytes imply;
+  * per_algo: SSSP, CC, BFS, BC, PR_T, PR_D on RMAT-22 with GTEPS, the single-thread
+    CPU restatement of the same algorithm (code/validate.cu) timed beside each, and
+    equiv_frac: one full pass's algorithmic bytes (SSSP 12E+8V, CC 16E_u+8V, BFS 8E+8V,
+    BC 36E+24V) over the run time and the 8 TB/s peak.  It is a yardstick, not a
+    bandwidth: a direction-optimised BFS or an Afforest CC touches far less than a full
+    pass (measured bandwidths come from rocprofv3 counters, see profiles/).  A PageRank
+    iteration is one full pass, so its hbm_frac_per_iter (8E+16V) is a bandwidth;
   * batched: 64 sources (uniform over the vertices, seed 3) run as one batch vs 64
-    single-source runs, BFS, BC and SSSP, on RMAT-22;
+    single-source runs, BFS, BC and SSSP, on RMAT-22, every row checked against the
+    oracle;
   * pagerank_rmat24: PR_T and PR_D at damping 0.85, tolerance 1e-7, at most 100
-    iterations on RMAT-24 (their error against FP64 is covered by the parity tests);
+    iterations on RMAT-24, with relative L1 and max-abs error against FP64 run for the
+    same number of iterations;
   * uneven_batch: 32 graphs (RMAT-14..20, grids 256^2..2048^2) in one batched run,
-    4 BFS / SSSP sources each, default options vs autotuned (about two minutes of
-    the default run, most of it the SSSP tuning; --no-uneven skips it).
-Inputs are resident in HBM before every timed region.
+    4 BFS / SSSP sources each, default options vs autotuned, every slot checked against
+    the oracle (--no-uneven skips it).
+Inputs are resident in HBM before every timed region.  The oracle (oracle/) is used
+only as the checker and for the CPU columns.
 """
 import argparse
 import json
@@ -88,39 +96,66 @@ def measured_traffic(kernel, scale):
 
 
 def rmat(mg, scale):
+    from multigraph_amd import workloads
+
     t0 = time.time()
-    g = mg.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=1, weight_seed=7)
+    g = workloads.rmat(mg, scale)
     log(f"RMAT-{scale}: nv={g.nv} ne={g.ne} built on the host in {time.time() - t0:.1f}s")
     return g
 
 
-def per_algo_extras(mg, np, g, src, nv, ne):
+def cpu_col(edges, seconds, what):
+    """single-thread CPU figure of the oracle's restatement of code/validate.cu"""
+    return {"gteps": edges / seconds / 1e9, "s": seconds, "cores": 1, "kind": "port", "what": what}
+
+
+def per_algo_extras(mg, np, orc, g, src, nv, ne, row, col):
     out = {}
     e = mg.Engine(g, mg.BFS)
     e.run(src)
     t = float(np.median([e.run(src) for _ in range(5)]))
-    out["bfs"] = {"ms": t, "gteps": gteps(ne, t), "levels": e.iterations(), "hbm_frac": frac(8.0 * ne + 8.0 * nv, t)}
+    ok = bool(np.array_equal(e.result(mg.RES_LEVEL), orc.bfs(row, col, src)))
+    t0 = time.perf_counter()
+    orc.bfs(row, col, src)
+    tc = time.perf_counter() - t0
+    out["bfs"] = {"ms": t, "gteps": gteps(ne, t), "levels": e.iterations(), "equiv_frac": frac(8.0 * ne + 8.0 * nv, t),
+                  "matches_oracle": ok, "cpu": cpu_col(ne, tc, "queue BFS, validate.cu:106-127")}
     e.close()
     e = mg.Engine(g, mg.BC)
     e.run(src)
     t = float(np.median([e.run(src) for _ in range(3)]))
-    out["bc"] = {"ms": t, "gteps": gteps(2 * ne, t), "hbm_frac": frac(36.0 * ne + 24.0 * nv, t)}
+    t0 = time.perf_counter()
+    lv, sg, dl = orc.bc_fp64(row, col, src)
+    tc = time.perf_counter() - t0
+    d = e.result(mg.RES_DELTA).astype(np.float64)
+    rel = float((np.abs(d - dl) / np.maximum(1.0, np.abs(dl))).max())
+    ok = bool(np.array_equal(e.result(mg.RES_LEVEL), lv) and np.array_equal(e.result(mg.RES_SIGMA), sg)) and rel <= 1e-6
+    out["bc"] = {"ms": t, "gteps": gteps(2 * ne, t), "equiv_frac": frac(36.0 * ne + 24.0 * nv, t),
+                 "delta_max_rel_err": rel, "matches_oracle": ok,
+                 "cpu": cpu_col(2 * ne, tc, "Brandes forward + backward, validate.cu:149-189 (FP64)")}
     e.close()
-    for name, algo in (("pr_t", mg.PR_T), ("pr_d", mg.PR_D)):
+    for name, algo, variant in (("pr_t", mg.PR_T, 0), ("pr_d", mg.PR_D, 1)):
         e = mg.Engine(g, algo, pr_tolerance=1e-7, pr_max_iter=100)
         e.run(0)
         ms = e.run(0)
         its = max(e.iterations(), 1)
+        # CPU: the reference's float rule for a bounded sample of 5 iterations
+        t0 = time.perf_counter()
+        orc.pr_f32(row, col, variant, 0.85, 1e-7, -5)
+        tc = (time.perf_counter() - t0) / 5
         out[name] = {"ms": ms, "iterations": its, "ms_per_iter": ms / its, "gteps": gteps(ne, ms),
-                     "gteps_per_iter": gteps(ne, ms / its), "hbm_frac_per_iter": frac(8.0 * ne + 16.0 * nv, ms / its)}
+                     "gteps_per_iter": gteps(ne, ms / its), "hbm_frac_per_iter": frac(8.0 * ne + 16.0 * nv, ms / its),
+                     "cpu": cpu_col(ne, tc, "one iteration (mean of 5) of the float rule, validate.cu:34-91")}
         e.close()
     return out
 
 
-def batched(mg, np, g, nv, ne):
-    """64 sources (seed 3) as one batch vs 64 single-source runs"""
-    rng = np.random.default_rng(3)
-    src = rng.integers(0, nv, size=64).astype(np.int32)
+def batched(mg, np, orc, g, nv, ne, row, col, w):
+    """64 sources (seed 3) as one batch vs 64 single-source runs; every row of the batch
+    against the oracle (levels, sigma and the summed dependencies for BC)"""
+    from multigraph_amd import workloads
+
+    src = workloads.batch_sources(nv)
     out = {"sources": "64 uniform over the compacted vertices, numpy default_rng(3)"}
     for name, kind, algo, te in (("bfs", mg.BATCH_BFS, mg.BFS, 1), ("bc", mg.BATCH_BC, mg.BC, 2),
                                  ("sssp", mg.BATCH_SSSP, mg.SSSP, 1)):
@@ -129,44 +164,58 @@ def batched(mg, np, g, nv, ne):
         tb = float(np.median([b.run(src) for _ in range(3)]))
         levels = b.sweeps()
         lv = b.levels()
+        rec = {}
+        if name == "bfs":
+            ok = bool(np.array_equal(lv, orc.bfs_multi(row, col, src)))
+        elif name == "sssp":
+            ok = bool(np.array_equal(lv, orc.sssp_dijkstra_multi(row, col, w, src)))
+        else:
+            rl, rs, rbc = orc.bc_fp64_multi(row, col, src)
+            bc = b.bc().astype(np.float64)
+            err = float((np.abs(bc - rbc) / np.maximum(1.0, np.abs(rbc))).max())
+            ok = bool(np.array_equal(lv, rl) and np.array_equal(b.sigma(), rs)) and err <= 1e-5
+            rec["bc_sum_max_rel_err"] = err
+            del rl, rs
         e = mg.Engine(g, algo)
         e.run(int(src[0]))
-        ts = []
-        ok = True
-        for i, s in enumerate(src):
-            ts.append(e.run(int(s)))
-            if i < 4:  # the batch's rows match single-source runs, bit for bit
-                ok = ok and bool(np.array_equal(e.result(mg.RES_LEVEL), lv[i]))
+        ts = [e.run(int(s)) for s in src]
         tsum = float(np.sum(ts))
-        rec = {"batch_ms": tb, "singles_ms": tsum, "batched_gteps": gteps(64 * te * ne, tb),
-               "per_source_gteps": gteps(64 * te * ne, tsum), "speedup": tsum / tb, "levels": levels,
-               "rows_match_singles": ok}
+        rec.update({"batch_ms": tb, "singles_ms": tsum, "batched_gteps": gteps(64 * te * ne, tb),
+                    "per_source_gteps": gteps(64 * te * ne, tsum), "speedup": tsum / tb, "levels": levels,
+                    "rows_match_oracle": ok})
         if name == "bfs":
-            rec["hbm_frac"] = frac((12.0 * ne + 20.0 * nv) * levels, tb)
+            rec["equiv_frac"] = frac((12.0 * ne + 20.0 * nv) * levels, tb)
         b.close()
         e.close()
         out[name] = rec
-        log(f"batched {name}: {tb:.2f} ms vs {tsum:.2f} ms of singles ({tsum / tb:.2f}x)")
+        log(f"batched {name}: {tb:.2f} ms vs {tsum:.2f} ms of singles ({tsum / tb:.2f}x), oracle {'ok' if ok else 'MISMATCH'}")
     return out
 
 
-def uneven(mg, np):
+def uneven_oracle(np, orc, graphs, offs, src, slots, lv, kind):
+    """every slot of every member graph against the oracle"""
+    for gi, p in enumerate(graphs):
+        row, col, w = p.csr()
+        o = offs[gi]
+        loc = src[4 * gi:4 * gi + 4] - o
+        ref = orc.bfs_multi(row, col, loc) if kind == "bfs" else orc.sssp_dijkstra_multi(row, col, w, loc)
+        for j in range(4):
+            if not np.array_equal(lv[slots[4 * gi + j], o:o + p.nv], ref[j]):
+                return False
+    return True
+
+
+def uneven(mg, np, orc):
     """32 graphs in one batch: RMAT-14..20 (4 seeds each) and 4-neighbour grids 256^2 ..
     2048^2 joined as a disjoint union, 4 sources per graph sharing 4 slots of one run;
     default options vs the autotuner's"""
+    from multigraph_amd import workloads
+
     t0 = time.time()
-    graphs = [mg.Graph.rmat(sc, 16, seed=100 + 10 * k + sc, weight_seed=7) for k in range(4) for sc in range(14, 21)]
-    graphs += [mg.Graph.grid(side, side, weight_seed=7) for side in (256, 512, 1024, 2048)]
-    u = mg.Graph.union(graphs)
-    rng = np.random.default_rng(5)
-    src, slots, off = [], [], 0
-    for g in graphs:
-        src += (rng.integers(0, g.nv, 4) + off).tolist()
-        slots += [0, 1, 2, 3]
-        off += g.nv
+    u, graphs, offs, src, slots = workloads.uneven(mg)
     ne = u.ne
     log(f"uneven batch: {len(graphs)} graphs, nv={u.nv} ne={ne} ({time.time() - t0:.1f}s)")
-    out = {"graphs": "RMAT-14..20 x 4 seeds + grids 256^2, 512^2, 1024^2, 2048^2", "nv": u.nv, "ne": ne,
+    out = {"graphs": workloads.UNEVEN_DESC, "nv": u.nv, "ne": ne,
            "sources": "4 per graph, numpy default_rng(5), slots 0..3 shared across graphs"}
     for name, kind in (("bfs", mg.BATCH_BFS), ("sssp", mg.BATCH_SSSP)):
         b = mg.Batch(u, kind)
@@ -185,9 +234,10 @@ def uneven(mg, np):
         t_tun = float(np.median([b.run(src, slots) for _ in range(3)]))
         same = bool(np.array_equal(b.levels(), lv_def))
         b.close()
+        ok = uneven_oracle(np, orc, graphs, offs, src, slots, lv_def, name)
         out[name] = {"untuned_ms": t_def, "tuned_ms": t_tun, "untuned_gteps": gteps(4 * ne, t_def),
                      "tuned_gteps": gteps(4 * ne, t_tun), "levels": levels, "tuning_s": tune_s,
-                     "results_equal": same,
+                     "results_equal": same, "slots_match_oracle": ok,
                      "tuned_options": {"alpha": opt.alpha, "beta": opt.beta, "batch_chunk": opt.batch_chunk,
                                        "grid_blocks": opt.grid_blocks, "graph_levels": opt.graph_levels,
                                        "persist_levels": opt.persist_levels, "sssp_delta": opt.sssp_delta},
@@ -196,12 +246,14 @@ def uneven(mg, np):
     return out
 
 
-def pagerank_big(mg, np, scale):
-    """timing only: the ranks' error against FP64 is measured by the PageRank parity tests"""
+def pagerank_big(mg, np, orc, scale):
+    """time per iteration, and the ranks' relative L1 and max-abs error against the FP64
+    oracle run for the same number of iterations"""
     g = rmat(mg, scale)
     nv, ne = g.nv, g.ne
+    row, col, _ = g.csr()
     out = {"graph": f"RMAT-{scale}", "nv": nv, "ne": ne}
-    for name, algo in (("pr_t", mg.PR_T), ("pr_d", mg.PR_D)):
+    for name, algo, variant in (("pr_t", mg.PR_T, 0), ("pr_d", mg.PR_D, 1)):
         e = mg.Engine(g, algo, pr_tolerance=1e-7, pr_max_iter=100)
         e.run(0)
         e.set_profiling(True)
@@ -210,13 +262,19 @@ def pagerank_big(mg, np, scale):
         e.set_profiling(False)
         ms = e.run(0)
         its = max(e.iterations(), 1)
+        r = e.result(mg.RES_RANK).astype(np.float64)
+        ref, _ = orc.pr_fp64_pull(row, col, variant, 0.85, 1e-7, -its)
+        l1 = float(np.abs(r - ref).sum() / np.abs(ref).sum())
+        mx = float(np.abs(r - ref).max())
+        del r, ref
         rec = {"ms": ms, "iterations": its, "ms_per_iter": ms / its, "gteps_per_iter": gteps(ne, ms / its),
-               "hbm_frac_per_iter": frac(8.0 * ne + 16.0 * nv, ms / its),
+               "hbm_frac_per_iter": frac(8.0 * ne + 16.0 * nv, ms / its), "l1_rel_vs_fp64": l1,
+               "max_abs_vs_fp64": mx,
                "kernels": {k: {"avg_ms": v["ms"] / max(v["launches"], 1), "launches": v["launches"]}
                            for k, v in ks.items()}, "profiled_ms": ms_prof}
         e.close()
         out[name] = rec
-        log(f"{name} RMAT-{scale}: {ms / its:.3f} ms/iter over {its} iterations")
+        log(f"{name} RMAT-{scale}: {ms / its:.3f} ms/iter over {its} iterations, L1 {l1:.2e}, max abs {mx:.2e}")
     return out
 
 
@@ -288,19 +346,22 @@ def main():
     kt = kernel_table([sssp, cc])
     dominant_ok = max(kt, key=lambda k: kt[k]["ms"]) == DOMINANT
 
-    # correctness of what was timed: SSSP certificate, CC edge consistency
+    # correctness of what was timed, against the oracle: SSSP distances (Dijkstra) and
+    # CC labels (union-find, min label per component), bit-exact
+    from oracle import oracle as orc
+
     d = sssp.result(mg.RES_LEVEL)
     lab = cc.result(mg.RES_LEVEL)
-    srcs = np.repeat(np.arange(nv, dtype=np.int64), np.diff(row))
-    ok = bool(d[src] == 0 and np.all(d[col] <= d[srcs].astype(np.int64) + w) and np.all(lab[col] == lab[srcs]))
+    ok = bool(np.array_equal(d, orc.sssp_dijkstra(row, col, w, src)))
+    ok = ok and bool(np.array_equal(lab, orc.cc_unionfind(row, col)[0]))
     sssp.set_profiling(False)
     cc.set_profiling(False)
 
     t_sssp, t_cc = float(np.mean(ms_sssp)), float(np.mean(ms_cc))
     per_algo = {
         "sssp": {"ms": t_sssp, "gteps": gteps(ne, t_sssp), "iterations": sssp_iters,
-                 "hbm_frac_one_pass": frac(12.0 * ne + 8.0 * nv, t_sssp)},
-        "cc": {"ms": t_cc, "gteps": gteps(ne_u, t_cc), "hbm_frac_one_round": frac(16.0 * ne_u + 8.0 * nv, t_cc)},
+                 "equiv_frac": frac(12.0 * ne + 8.0 * nv, t_sssp)},
+        "cc": {"ms": t_cc, "gteps": gteps(ne_u, t_cc), "equiv_frac": frac(16.0 * ne_u + 8.0 * nv, t_cc)},
     }
     sssp.close()
     cc.close()
@@ -309,31 +370,31 @@ def main():
     pr_big = None
     uneven_batch = None
     if not args.no_extra:
-        per_algo.update(per_algo_extras(mg, np, g, src, nv, ne))
+        per_algo.update(per_algo_extras(mg, np, orc, g, src, nv, ne, row, col))
         log("per-algorithm figures done")
-        batch = batched(mg, np, g, nv, ne)
+        batch = batched(mg, np, orc, g, nv, ne, row, col, w)
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as orc
-
         t1 = time.perf_counter()
         ref, sweeps = orc.sssp_bf(row, col, w, src)
         t2 = time.perf_counter()
-        _, ncomp, _ = orc.cc_labelprop(row, col)
+        lp, ncomp, _ = orc.cc_labelprop(row, col)
         t3 = time.perf_counter()
-        ok = ok and bool(np.array_equal(ref, d))
+        ok = ok and bool(np.array_equal(ref, d)) and bool(np.array_equal(orc.canon(lp), orc.canon(lab)))
         cpu = {"value": (ne + ne_u) / (t3 - t1) / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
                "sample": f"the full step on the same RMAT-{args.scale} graph: Bellman-Ford SSSP ({sweeps} sweeps, "
                          f"{t2 - t1:.2f}s) + label-propagation CC ({t3 - t2:.2f}s), code/validate.cu restated, "
                          f"1 thread on {cpu_model()}"}
+        per_algo["sssp"]["cpu"] = cpu_col(ne, t2 - t1, "in-place Bellman-Ford, validate.cu:131-146")
+        per_algo["cc"]["cpu"] = cpu_col(ne_u, t3 - t2, "label propagation + relabel, validate.cu:223-306")
         log("cpu baseline done")
 
-    del g, row, col, w, srcs, d, lab
+    del g, row, col, w, d, lab
     if not args.no_extra and args.pr_scale > 0:
-        pr_big = pagerank_big(mg, np, args.pr_scale)
+        pr_big = pagerank_big(mg, np, orc, args.pr_scale)
     if not args.no_extra and not args.no_uneven:
-        uneven_batch = uneven(mg, np)
+        uneven_batch = uneven(mg, np, orc)
 
     line = {
         "metric": METRIC,

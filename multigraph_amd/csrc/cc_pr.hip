@@ -444,7 +444,7 @@ int mg_cc_step(mg_engine *e, int *more) {
     hipStream_t s = e->stream;
     const int G = e->opt.grid_blocks > 0 ? e->opt.grid_blocks : 2048;
     const int nv = e->nv;
-    int slot = -1;
+    int slot = -1, rest_slot = -1;
     bool hooked = false;
     if (e->cc_afforest) {  // no host round trip: the last two hooks read the sample on the device
         const int k = 2;
@@ -457,22 +457,32 @@ int mg_cc_step(mg_engine *e, int *more) {
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->level);
         hipLaunchKernelGGL(k_cc_sample, dim3(1), dim3(1024), 0, s, nv, e->level, e->aux);
-        if (e->prof) mg_prof_begin(e, "k_cc_link_rest", 8.0 * nv, &slot);
+        if (e->prof) mg_prof_begin(e, "k_cc_link_rest", 8.0 * nv, &rest_slot);
         hipLaunchKernelGGL(k_cc_link_rest, dim3(grid_for(nv, kBlock, G)), dim3(kBlock), 0, s, nv, k, e->row, e->col, e->aux, e->level);
-        if (e->prof) mg_prof_end(e, slot);
+        if (e->prof) mg_prof_end(e, rest_slot);
         hooked = true;
     }
+    int hook_slot = -1;
     if (e->chunk_cap > 0) {  // after the sampling pass: only when its giant tree was too small
-        if (e->prof) mg_prof_begin(e, "k_cc_hook_lb", 4.0 * e->ne_u + 4.0 * (nv + 1) + 8.0 * e->ne_u, &slot);
+        if (e->prof) mg_prof_begin(e, "k_cc_hook_lb", 4.0 * e->ne_u + 4.0 * (nv + 1) + 8.0 * e->ne_u, &hook_slot);
         hipLaunchKernelGGL(k_cc_hook_lb, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, (int)e->chunk_cap, e->ne_u,
                            e->urow, e->ucol, e->cstart, e->level, hooked ? (const int *)e->aux : (const int *)nullptr);
-        if (e->prof) mg_prof_end(e, slot);
+        if (e->prof) mg_prof_end(e, hook_slot);
     }
     if (e->prof) mg_prof_begin(e, "k_cc_compress", 8.0 * nv, &slot);
     hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, s, nv, e->level);
     if (e->prof) mg_prof_end(e, slot);
     MG_HIP(hipGetLastError());
     MG_HIP(hipStreamSynchronize(s));
+    if (e->prof && hooked) {
+        // of k_cc_link_rest and k_cc_hook_lb, the sample lets one work and the other
+        // return at once: the one that returned moved no bytes
+        int aux2[2] = {0, 0};
+        MG_HIP(hipMemcpy(aux2, e->aux, sizeof(aux2), hipMemcpyDeviceToHost));
+        const bool giant = aux2[1] >= kGiantSamples;
+        if (giant && hook_slot >= 0) e->pending[hook_slot].bytes = 0.0;
+        if (!giant && rest_slot >= 0) e->pending[rest_slot].bytes = 0.0;
+    }
     e->iter += 1;
     e->levels = 1;
     e->edges_traversed = e->ne_u;

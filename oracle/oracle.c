@@ -20,11 +20,15 @@
  *                      path / star / complete-graph closed forms
  *   orc_pr_f32/_fp64   regular graphs converge to 1, float rule vs double rule
  *                      agree to float precision, hand-iterated 3-vertex example
+ *   orc_*_multi,       OpenMP drivers of the functions above for the full-size parity
+ *   orc_pr_fp64_pull   tests: bit-identical to the single-thread functions (checked on
+ *                      small graphs), so they inherit those functions' pins
  */
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <omp.h>
 
 #define ORC_INF 0x64646464 /* IUNUSED, common.h:37-39 */
 
@@ -74,8 +78,7 @@ int orc_sssp_bf(int nv, const int32_t *row, const int32_t *col, const int32_t *w
 int orc_sssp_dijkstra(int nv, const int32_t *row, const int32_t *col, const int32_t *w, int src, int32_t *dist) {
     for (int i = 0; i < nv; ++i) dist[i] = ORC_INF;
     if (nv == 0) return 0;
-    int64_t ne = row[nv];
-    int64_t cap = ne + 1;
+    int64_t cap = (int64_t)nv + 1;  /* grows by doubling: pushes can reach ne */
     int64_t *hk = (int64_t *)malloc(sizeof(int64_t) * (size_t)cap);
     if (!hk) return -1;
     int64_t hn = 0;
@@ -306,4 +309,106 @@ int orc_pr_fp64(int nv, const int32_t *row, const int32_t *col, int variant, dou
     }
     free(nr); free(mark); free(c);
     return it;
+}
+
+/* ---- OpenMP drivers for the full-size parity tests --------------------------------
+ * Each is the single-thread function above, run on several threads, with results that
+ * are bit-identical to it (pinned so on small graphs by the tests). */
+
+int orc_threads(void) { return omp_get_max_threads(); }
+
+/* row s of level[n][nv] = orc_bfs(src[s]) */
+int orc_bfs_multi(int nv, const int32_t *row, const int32_t *col, int n, const int32_t *src, int32_t *level) {
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+    for (int s = 0; s < n; ++s)
+        if (orc_bfs(nv, row, col, src[s], level + (size_t)s * nv) < 0) bad = 1;
+    return bad ? -1 : 0;
+}
+
+/* row s of dist[n][nv] = orc_sssp_dijkstra(src[s]) */
+int orc_sssp_dijkstra_multi(int nv, const int32_t *row, const int32_t *col, const int32_t *w, int n,
+                            const int32_t *src, int32_t *dist) {
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+    for (int s = 0; s < n; ++s)
+        if (orc_sssp_dijkstra(nv, row, col, w, src[s], dist + (size_t)s * nv) < 0) bad = 1;
+    return bad ? -1 : 0;
+}
+
+/* rows s of level / sigma [n][nv] = orc_bc_fp64(src[s]); bc[v] = sum over s of delta_s[v]
+ * with each source's own entry (delta_s[src[s]]) left out, summed in source order */
+int orc_bc_fp64_multi(int nv, const int32_t *row, const int32_t *col, int n, const int32_t *src, int32_t *level,
+                      double *sigma, double *bc) {
+    double *dl = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1) * (size_t)(nv > 0 ? nv : 1));
+    if (!dl) return -1;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+    for (int s = 0; s < n; ++s) {
+        double *d = dl + (size_t)s * nv;
+        if (orc_bc_fp64(nv, row, col, src[s], level + (size_t)s * nv, sigma + (size_t)s * nv, d) < 0) bad = 1;
+        else d[src[s]] = 0.0;
+    }
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < nv; ++v) {
+        double a = 0.0;
+        for (int s = 0; s < n; ++s) a += dl[(size_t)s * nv + v];
+        bc[v] = a;
+    }
+    free(dl);
+    return bad ? -1 : 0;
+}
+
+/* orc_pr_fp64 in pull form over the transposed CSR, vertices on OpenMP threads.  The
+ * transpose is built by a counting sort that scans sources in increasing order, so every
+ * destination adds its contributions in the same order as orc_pr_fp64's push loop does:
+ * the ranks are bit-identical to it. */
+int orc_pr_fp64_pull(int nv, const int32_t *row, const int32_t *col, int variant, double damping, double tol,
+                     int max_iter, double *r) {
+    const size_t n1 = (size_t)(nv > 0 ? nv : 1);
+    const int64_t ne = nv > 0 ? row[nv] : 0;
+    int64_t *trow = (int64_t *)calloc(n1 + 1, sizeof(int64_t));
+    int32_t *tcol = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ne > 0 ? ne : 1));
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * n1);
+    double *nr = (double *)malloc(sizeof(double) * n1);
+    double *c = (double *)malloc(sizeof(double) * n1);
+    int32_t *mark = (int32_t *)malloc(sizeof(int32_t) * n1);
+    int rc = -1;
+    if (!trow || !tcol || !cur || !nr || !c || !mark) goto out;
+    for (int64_t j = 0; j < ne; ++j) ++trow[col[j] + 1];
+    for (int v = 0; v < nv; ++v) trow[v + 1] += trow[v];
+    for (int v = 0; v < nv; ++v) cur[v] = trow[v];
+    for (int u = 0; u < nv; ++u)
+        for (int32_t j = row[u]; j < row[u + 1]; ++j) tcol[cur[col[j]]++] = u;
+    for (int i = 0; i < nv; ++i) { r[i] = 0.0; mark[i] = 1; }
+    int it = 0;
+    for (;;) {
+        int flag = 0;
+        ++it;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nv; ++i) {
+            int deg = row[i + 1] - row[i];
+            c[i] = deg > 0 ? r[i] / (double)deg : 0.0;
+        }
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : flag)
+        for (int i = 0; i < nv; ++i) {
+            double a = 0.0;
+            for (int64_t j = trow[i]; j < trow[i + 1]; ++j) a += c[tcol[j]];
+            double f1 = (1.0 - damping) + a * damping;
+            double f2 = r[i];
+            if (variant == 0) {
+                if (fabs(f1 - f2) > tol) { r[i] = f1; flag = 1; }
+            } else {
+                if (mark[i] == 1 && fabs(f1 - f2) > tol * f2) { r[i] = f1; flag = 1; }
+                else { r[i] = f1; mark[i] = 0; }
+            }
+        }
+        if (max_iter < 0) { if (it >= -max_iter) break; continue; }
+        if (!flag) break;
+        if (max_iter > 0 && it >= max_iter) break;
+    }
+    rc = it;
+out:
+    free(trow); free(tcol); free(cur); free(nr); free(c); free(mark);
+    return rc;
 }

@@ -39,6 +39,11 @@ def lib():
         L.orc_bc_fp64.argtypes = [C.c_int, I, I, C.c_int, I, D, D]
         L.orc_pr_f32.argtypes = [C.c_int, I, I, C.c_int, C.c_float, C.c_float, C.c_int, F]
         L.orc_pr_fp64.argtypes = [C.c_int, I, I, C.c_int, C.c_double, C.c_double, C.c_int, D]
+        L.orc_threads.argtypes = []
+        L.orc_bfs_multi.argtypes = [C.c_int, I, I, C.c_int, I, I]
+        L.orc_sssp_dijkstra_multi.argtypes = [C.c_int, I, I, I, C.c_int, I, I]
+        L.orc_bc_fp64_multi.argtypes = [C.c_int, I, I, C.c_int, I, I, D, D]
+        L.orc_pr_fp64_pull.argtypes = [C.c_int, I, I, C.c_int, C.c_double, C.c_double, C.c_int, D]
         _lib = L
     return _lib
 
@@ -126,4 +131,63 @@ def pr_fp64(row, col, variant, damping=0.85, tol=0.01, max_iter=0):
     nv = len(row) - 1
     r = np.empty(nv, np.float64)
     it = lib().orc_pr_fp64(nv, _p(row), _p(col), variant, damping, tol, max_iter, _p(r, C.c_double))
+    return r, it
+
+
+# ---- OpenMP drivers (bit-identical to the single-thread functions above) ----------
+def threads():
+    """OpenMP threads the multi-source and pull functions use"""
+    return lib().orc_threads()
+
+
+def _srcs(sources):
+    return np.ascontiguousarray(sources, np.int32)
+
+
+def bfs_multi(row, col, sources):
+    """[n][nv] levels: row s = bfs(row, col, sources[s])"""
+    row, col = _csr(row, col)
+    src = _srcs(sources)
+    nv = len(row) - 1
+    out = np.empty((len(src), nv), np.int32)
+    if lib().orc_bfs_multi(nv, _p(row), _p(col), len(src), _p(src), _p(out)) < 0:
+        raise MemoryError("orc_bfs_multi")
+    return out
+
+
+def sssp_dijkstra_multi(row, col, w, sources):
+    """[n][nv] distances: row s = sssp_dijkstra(row, col, w, sources[s])"""
+    row, col = _csr(row, col)
+    w = np.ascontiguousarray(w, np.int32)
+    src = _srcs(sources)
+    nv = len(row) - 1
+    out = np.empty((len(src), nv), np.int32)
+    if lib().orc_sssp_dijkstra_multi(nv, _p(row), _p(col), _p(w), len(src), _p(src), _p(out)) < 0:
+        raise MemoryError("orc_sssp_dijkstra_multi")
+    return out
+
+
+def bc_fp64_multi(row, col, sources):
+    """(levels [n][nv], sigma [n][nv], bc [nv]): rows as bc_fp64 per source; bc sums the
+    sources' dependencies, each source's own entry left out"""
+    row, col = _csr(row, col)
+    src = _srcs(sources)
+    nv = len(row) - 1
+    lv = np.empty((len(src), nv), np.int32)
+    sg = np.empty((len(src), nv), np.float64)
+    bc = np.empty(nv, np.float64)
+    if lib().orc_bc_fp64_multi(nv, _p(row), _p(col), len(src), _p(src), _p(lv), _p(sg, C.c_double),
+                               _p(bc, C.c_double)) < 0:
+        raise MemoryError("orc_bc_fp64_multi")
+    return lv, sg, bc
+
+
+def pr_fp64_pull(row, col, variant, damping=0.85, tol=0.01, max_iter=0):
+    """pr_fp64 in pull form on OpenMP threads: bit-identical ranks"""
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    r = np.empty(nv, np.float64)
+    it = lib().orc_pr_fp64_pull(nv, _p(row), _p(col), variant, damping, tol, max_iter, _p(r, C.c_double))
+    if it < 0:
+        raise MemoryError("orc_pr_fp64_pull")
     return r, it

@@ -250,3 +250,43 @@ def test_planted_mistakes_are_caught(orc):
     lv[5] += 1
     ref_l = nx.single_source_shortest_path_length(Gb, 0)
     assert not np.array_equal(lv, [ref_l[v] for v in range(nb)])
+
+
+# ---- OpenMP drivers: bit-identical to the single-thread functions -----------------
+@pytest.mark.parametrize("name", NAMES)
+def test_multi_drivers_equal_singles(orc, name):
+    G = GRAPHS[name]
+    row, col = csr_of(G)
+    w = weights_of(G, row, col)
+    n = len(row) - 1
+    src = np.array([0, n // 2, n - 1, n // 3, 0], np.int32)  # a repeated source too
+    lv = orc.bfs_multi(row, col, src)
+    dd = orc.sssp_dijkstra_multi(row, col, w, src)
+    blv, bsg, bc = orc.bc_fp64_multi(row, col, src)
+    ref_bc = np.zeros(n)
+    for i, s in enumerate(src):
+        s = int(s)
+        assert np.array_equal(lv[i], orc.bfs(row, col, s))
+        assert np.array_equal(dd[i], orc.sssp_dijkstra(row, col, w, s))
+        l1, s1, d1 = orc.bc_fp64(row, col, s)
+        assert np.array_equal(blv[i], l1) and np.array_equal(bsg[i], s1)
+        d1 = d1.copy()
+        d1[s] = 0.0
+        ref_bc += d1
+    assert np.array_equal(bc, ref_bc)  # same additions in the same (source) order
+
+
+@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("variant", [0, 1])
+def test_pr_pull_equals_push(orc, name, variant):
+    G = GRAPHS[name]
+    row, col = csr_of(G)
+    for tol, cap in ((1e-7, 100), (0.01, 0), (1e-9, -37)):
+        r1, i1 = orc.pr_fp64(row, col, variant, 0.85, tol, cap)
+        r2, i2 = orc.pr_fp64_pull(row, col, variant, 0.85, tol, cap)
+        assert i1 == i2
+        assert np.array_equal(r1, r2)
+
+
+def test_omp_threads_reported(orc):
+    assert orc.threads() >= 1
