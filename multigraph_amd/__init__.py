@@ -18,7 +18,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmgamd.so")
+# MG_LIBRARY selects another build of the same ABI (e.g. one compiled with a
+# maintainer's own hooks file); the default is the in-tree libmgamd.so
+LIB_PATH = os.environ.get("MG_LIBRARY") or os.path.join(_HERE, "libmgamd.so")
 
 BFS, SSSP, BC, CC, PR_T, PR_D = range(6)
 ALGO_NAMES = {BFS: "BFS", SSSP: "SSSP", BC: "BC", CC: "CC", PR_T: "PR_T", PR_D: "PR_D"}

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 #include "engine.hpp"
+#include "hook_calls.hpp"
 #include "mg_device.hpp"
 
 using namespace mg;
@@ -236,7 +237,10 @@ __global__ __launch_bounds__(kBB) void k_mb_push(const BCtl *ctl, long long nch,
             // frontier; then the wave walks the edges with new bits, four at a time,
             // lanes = sources adding sigma[u][s] into sigma[w][s]
             const int wl = threadIdx.x & ~(MG_WAVE - 1);
-            const double su = ((fu >> lane()) & 1ull) ? sigma[(long long)k.v * 64 + lane()] : 0.0;
+            // the lane's message (BCHooks::get_partial_result: its path count)
+            typename MG_BC_HOOKS::V1T lm;
+            double su = 0.0;
+            if ((fu >> lane()) & 1ull) MG_BC_HOOKS::get_partial_result(&lm, 0, &su, sigma[(long long)k.v * 64 + lane()]);
             for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
                 const int p = p0 + lane();
                 int w = 0;
@@ -315,7 +319,11 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bfs_light(const BCtl *ctl, int 
 
 // pull, BC: lanes = sources; neighbours staged 64 at a time in LDS, then lane s sums
 // sigma[u][s] over neighbours u whose frontier word has bit s
-__global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *__restrict__ ch, uint64_t all,
+// Path counts per lane (lane = source) through BCHooks: every parent in a source's
+// frontier sends get_partial_result(level it - 1, its count), combined with
+// accumulate_nonAtomics; the vertex's new bits are the lanes that received any.
+template <class H>
+__global__ __launch_bounds__(kBB) void k_mb_pull_bc(int it, long long nch, const Chunk *__restrict__ ch, uint64_t all,
                                                     const int *__restrict__ col, const uint64_t *__restrict__ F,
                                                     const uint64_t *__restrict__ vis, uint64_t *__restrict__ Fn,
                                                     double *__restrict__ sigma) {
@@ -324,7 +332,8 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
     const int wl = threadIdx.x & ~(MG_WAVE - 1);
     for_active_chunks(nch, ch, [&](const Chunk &k) { return all & ~vis[k.v]; }, [&](const Chunk &k, const uint64_t need) {
         uint64_t acc = 0;
-        double sg = 0.0;
+        Acc<H> a;
+        acc_init(a);
         for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
             const int p = p0 + lane();
             int u = 0;
@@ -343,18 +352,22 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bc(long long nch, const Chunk *
 #pragma unroll
                 for (int q = 0; q < kHitStep; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
                 double t[kHitStep];
+                bool on[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q) {
+                    on[q] = j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull);
+                    t[q] = on[q] ? sigma[(long long)s_u[wl + j[q]] * 64 + lane()] : 0.0;
+                }
 #pragma unroll
                 for (int q = 0; q < kHitStep; ++q)
-                    t[q] = (j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull)) ? sigma[(long long)s_u[wl + j[q]] * 64 + lane()] : 0.0;
-#pragma unroll
-                for (int q = 0; q < kHitStep; ++q) sg += t[q];
+                    if (on[q]) acc_edge<H>(a, it - 1, t[q], 0, it);
             }
         }
         if (!acc) return;
         if ((acc >> lane()) & 1ull) {
             const long long kv = (long long)k.v * 64 + lane();
-            if (k.multi) atomicAdd(&sigma[kv], sg);
-            else sigma[kv] = sg;
+            if (k.multi) atomicAdd(&sigma[kv], a.s);
+            else sigma[kv] = a.s;
         }
         if (lane() == 0) {
             if (k.multi) atomicOr((unsigned long long *)&Fn[k.v], (unsigned long long)acc);
@@ -705,7 +718,8 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
 // back into the vertex's next changed word (near-far order: the threshold rises by a
 // fixed band per level, so distances are settled roughly in increasing order instead of
 // being lowered again and again as in plain Bellman-Ford; the result is the same).
-__device__ __forceinline__ void sparse_sssp(int n, int thr, const int *__restrict__ row, const int *__restrict__ col,
+template <class H>
+__device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *__restrict__ row, const int *__restrict__ col,
                                             const int *__restrict__ wgt, uint64_t *C, uint64_t *Cn, const int *Q, int *Qn,
                                             int *qn_next, int *__restrict__ dist, int &cnt, long long &mf, int &mx) {
     const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
@@ -739,10 +753,13 @@ __device__ __forceinline__ void sparse_sssp(int n, int thr, const int *__restric
             if (rem) {
                 const int sl = __builtin_ctzll(rem);
                 w = col[p];
-                const int nd = du + wgt[p];
+                int nd;
+                H::get_partial_result(&nd, du, wgt[p]);
                 int *dw = &dist[(long long)w * 64 + sl];
-                if (nd < *dw && atomicMin(dw, nd) > nd)
+                if (nd < *dw) {
+                    H::accumulate(dw, nd, it);
                     add = atomicOr((unsigned long long *)&Cn[w], 1ull << sl) == 0ull;
+                }
                 if (++p == p1) {
                     p = p0;
                     rem &= rem - 1;
@@ -772,11 +789,12 @@ __device__ __forceinline__ void sparse_sssp(int n, int thr, const int *__restric
             }
             for (int pp = q0; pp < q1; ++pp) {
                 const int w = col[pp];
-                const int nd = dv + wgt[pp];
+                int nd;
+                H::get_partial_result(&nd, dv, wgt[pp]);
                 bool imp = false;
                 if (mine) {
                     int *dw = &dist[(long long)w * 64 + lane()];
-                    if (nd < *dw) imp = atomicMin(dw, nd) > nd;
+                    if (nd < *dw) { H::accumulate(dw, nd, it); imp = true; }
                 }
                 const uint64_t m = ballot(imp);
                 bool add = false;
@@ -825,7 +843,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__
     const int cur = (ctl->it - 1) & 1;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    sparse_sssp(ctl->qn[cur], ctl->thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
+    sparse_sssp<MG_SSSP_HOOKS>(ctl->it, ctl->qn[cur], ctl->thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
                 dist, cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
     zero_rest(nparts, part_n, part_e, part_x);
@@ -932,7 +950,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
             a.pc->mf[ns] = 0;
         }
         if (kSSSP)
-            sparse_sssp((int)n, c.thr, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA,
+            sparse_sssp<MG_SSSP_HOOKS>(it, (int)n, c.thr, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA,
                         cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
         else
             sparse_bfs(it, (int)n, a.row, a.col, cur ? a.FB : a.FA, cur ? a.FA : a.FB, a.vis, cur ? a.QB : a.QA,
@@ -1005,15 +1023,16 @@ __global__ void k_ms_scatter_words(long long n, const int *__restrict__ lv, cons
 // FP64 and stored in FP32 (like delta) once per (vertex, source) when its level is
 // finished (k_ms_cw): a matched successor costs one 256-byte row, not two rows and a
 // division per source
+template <class H>
 __device__ __forceinline__ void ms_term(KSum &acc, double sv, float cw) {
-    const double t = sv * (double)cw;
-    const float hi = (float)t;
-    acc.add(hi);
-    acc.add((float)(t - (double)hi));
+    double t;
+    H::back_term(&t, sv, (double)cw);
+    H::back_accumulate(&acc, t);
 }
 
 // c[v][s] = (1 + delta[v][s]) / sigma[v][s] for the (vertex, word) entries of a finished
 // level whose vertex is split into several chunks (k_mb_back writes the others' c)
+template <class H>
 __global__ void k_ms_cw(long long n, const int *__restrict__ lv, const uint64_t *__restrict__ lw, const int *__restrict__ row,
                         int chunk, const double *__restrict__ sigma, const float *__restrict__ delta, float *__restrict__ cw) {
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1022,13 +1041,16 @@ __global__ void k_ms_cw(long long n, const int *__restrict__ lv, const uint64_t 
         const int v = lv[i];
         if (row[v + 1] - row[v] <= chunk || !((lw[i] >> lane()) & 1ull)) continue;
         const long long kv = (long long)v * 64 + lane();
-        cw[kv] = (float)((1.0 + (double)delta[kv]) / sigma[kv]);
+        double c;
+        H::back_message(&c, delta[kv], sigma[kv]);
+        cw[kv] = (float)c;
     }
 }
 
 // E = words of level d, D = words of level d+1; a wave per chunk of a vertex in E.
 // Lanes load 64 edges' successor words at a time; then only the edges with a common
 // bit are walked (lanes = sources), each one gather of the successor's c row.
+template <class H>
 __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
                                                  const uint64_t *__restrict__ E, const uint64_t *__restrict__ D,
                                                  const double *__restrict__ sigma, float *__restrict__ cw,
@@ -1061,7 +1083,7 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
                     c[q] = (j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull)) ? cw[(long long)s_u[wl + j[q]] * 64 + lane()] : 0.f;
 #pragma unroll
                 for (int q = 0; q < kHitStep; ++q)
-                    if (c[q] != 0.f) ms_term(acc, sv, c[q]);
+                    if (c[q] != 0.f) ms_term<H>(acc, sv, c[q]);
             }
         }
         if (mine) {
@@ -1069,9 +1091,9 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
             if (k.multi) {
                 atomicAdd(&delta[kv], acc.value());
             } else {  // final: also its c for the level above
-                const float dv = acc.value();
-                delta[kv] = dv;
-                cw[kv] = (float)((1.0 + (double)dv) / sv);
+                double c;
+                H::back_update(&delta[kv], &c, acc.value(), sv);
+                cw[kv] = (float)c;
             }
         }
     });
@@ -1093,10 +1115,15 @@ __global__ void k_ms_bc_sum(int nv, int nslots, const int *__restrict__ lev, con
 // ---- SSSP --------------------------------------------------------------------------
 // a wave per chunk, lanes = sources: relax dist[v][s] from neighbours whose changed
 // bit s is set; the vertex's new changed word is the ballot of improvements
+// Hooks per lane: candidates from get_partial_result, combined in a register with
+// accumulate_nonAtomics (starting at initialize's identity), applied with
+// update_condition, or with accumulate (atomicMin) on a vertex split over chunks.
+template <class H>
 __global__ __launch_bounds__(kBB) void k_mb_sssp(const BCtl *ctl, long long nch, const Chunk *__restrict__ ch, const int *__restrict__ col,
                                                  const int *__restrict__ wgt, uint64_t *CA,
                                                  uint64_t *CB, int *__restrict__ dist) {
     if (!level_runs(ctl, 0)) return;
+    const int it = ctl ? ctl->it : 0;
     uint64_t *C, *Cn;
     cur_next(ctl, CA, CB, &C, &Cn);
     __shared__ int s_u[kBB], s_w[kBB];
@@ -1108,7 +1135,8 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp(const BCtl *ctl, long long nch,
         const Chunk k = ch[c];
         const long long kv = (long long)k.v * 64 + lane();
         const int cur = dist[kv];
-        int best = cur;
+        int best;
+        H::initialize(&best);
         for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
             const int p = p0 + lane();
             int u = 0, wu = 0;
@@ -1126,18 +1154,30 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp(const BCtl *ctl, long long nch,
                 int j[kHitStep];
 #pragma unroll
                 for (int q = 0; q < kHitStep; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
-                int cand[kHitStep];
+                int du[kHitStep];
+                bool on[kHitStep];
+#pragma unroll
+                for (int q = 0; q < kHitStep; ++q) {
+                    on[q] = j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull);
+                    du[q] = on[q] ? dist[(long long)s_u[wl + j[q]] * 64 + lane()] : 0;
+                }
 #pragma unroll
                 for (int q = 0; q < kHitStep; ++q)
-                    cand[q] = (j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull))
-                                  ? dist[(long long)s_u[wl + j[q]] * 64 + lane()] + s_w[wl + j[q]] : (int)MG_INF;
-#pragma unroll
-                for (int q = 0; q < kHitStep; ++q) best = min(best, cand[q]);
+                    if (on[q]) {
+                        int cand;
+                        H::get_partial_result(&cand, du[q], s_w[wl + j[q]]);
+                        H::accumulate_nonAtomics(&best, cand, it);
+                    }
             }
         }
         bool imp;
-        if (k.multi) imp = best < cur && atomicMin(&dist[kv], best) > best;
-        else { imp = best < cur; if (imp) dist[kv] = best; }
+        if (k.multi) {  // other chunks of the vertex write too
+            imp = best < cur;
+            if (imp) H::accumulate(&dist[kv], best, it);
+        } else {
+            int old = cur, stamp;
+            imp = H::update_condition(&old, &dist[kv], best, it, &stamp);
+        }
         const uint64_t m = ballot(imp);
         if (lane() == 0 && m) {
             if (k.multi) atomicOr((unsigned long long *)&Cn[k.v], (unsigned long long)m);
@@ -1150,10 +1190,12 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp(const BCtl *ctl, long long nch,
 // lanes = sources; along each edge (u, w) lane s lowers dist[w][s] to
 // dist[u][s] + weight (read first, atomicMin only when it improves) and the
 // improvements are or-ed into w's next changed word
+template <class H>
 __global__ __launch_bounds__(kBB) void k_mb_sssp_push(const BCtl *ctl, long long nch, const Chunk *__restrict__ ch,
                                                       const int *__restrict__ col, const int *__restrict__ wgt, uint64_t *CA,
                                                       uint64_t *CB, int *__restrict__ dist) {
     if (!level_runs(ctl, 1)) return;
+    const int it = ctl ? ctl->it : 0;
     uint64_t *C, *Cn;
     cur_next(ctl, CA, CB, &C, &Cn);
     for_active_chunks(nch, ch, [&](const Chunk &k) { return C[k.v]; }, [&](const Chunk &k, const uint64_t cu) {
@@ -1166,14 +1208,15 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp_push(const BCtl *ctl, long long
             for (int q = 0; q < kHitStep; ++q) {
                 const int p = min(p0 + q, k.p1 - 1);
                 w[q] = col[p];
-                nd[q] = p0 + q < k.p1 ? du + wgt[p] : (int)MG_INF;
+                if (p0 + q < k.p1) H::get_partial_result(&nd[q], du, wgt[p]);
+                else H::initialize(&nd[q]);
             }
 #pragma unroll
             for (int q = 0; q < kHitStep; ++q) dw[q] = mine ? dist[(long long)w[q] * 64 + lane()] : (int)MG_INF;
 #pragma unroll
             for (int q = 0; q < kHitStep; ++q) {
                 bool imp = false;
-                if (mine && nd[q] < dw[q]) imp = atomicMin(&dist[(long long)w[q] * 64 + lane()], nd[q]) > nd[q];
+                if (mine && nd[q] < dw[q]) { H::accumulate(&dist[(long long)w[q] * 64 + lane()], nd[q], it); imp = true; }
                 const uint64_t m = ballot(imp);
                 if (lane() == 0 && m) atomicOr((unsigned long long *)&Cn[w[q]], (unsigned long long)m);
             }
@@ -1478,7 +1521,7 @@ static int run_bfs_bc(mg_batch *b) {
             if (bc) hipLaunchKernelGGL((k_mb_push<true>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
             else hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nch, b->chunks, b->col, F, b->vis, Fn, b->sigma);
         } else {
-            if (bc) hipLaunchKernelGGL(k_mb_pull_bc, dim3(gci), dim3(kBB), 0, s, b->in_nch(), b->in_chunks(), all, b->in_col(), F, b->vis, Fn, b->sigma);
+            if (bc) hipLaunchKernelGGL(k_mb_pull_bc<MG_BC_HOOKS>, dim3(gci), dim3(kBB), 0, s, it, b->in_nch(), b->in_chunks(), all, b->in_col(), F, b->vis, Fn, b->sigma);
             else {
                 hipLaunchKernelGGL(k_mb_pull_bfs_light, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->in_row(), b->in_col(),
                                    all, F, b->vis, Fn);
@@ -1503,8 +1546,8 @@ static int run_bfs_bc(mg_batch *b) {
         if (n0 <= 0) continue;
         hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->E, 0);
         if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], b->D, 0);
-        hipLaunchKernelGGL(k_mb_back, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, b->E, b->D, b->sigma, b->cw, b->delta);
-        hipLaunchKernelGGL(k_ms_cw, dim3(grid_for(n0, 4, 8192)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->row,
+        hipLaunchKernelGGL(k_mb_back<MG_BC_HOOKS>, dim3(gc), dim3(kBB), 0, s, b->nch, b->chunks, b->col, b->E, b->D, b->sigma, b->cw, b->delta);
+        hipLaunchKernelGGL(k_ms_cw<MG_BC_HOOKS>, dim3(grid_for(n0, 4, 8192)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->row,
                            chunk, b->sigma, b->delta, b->cw);
         hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n0, 256, 4096)), dim3(256), 0, s, n0, b->logv + off[l], b->logw + off[l], b->E, 1);
         if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], b->D, 1);
@@ -1561,9 +1604,9 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
         hipLaunchKernelGGL(k_mb_fill_next, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, (const BCtl *)b->ctl, b->F[0],
                            b->F[1], (long long)nv + 1);
         if (sssp) {
-            hipLaunchKernelGGL(k_mb_sssp_push, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col, b->w,
+            hipLaunchKernelGGL(k_mb_sssp_push<MG_SSSP_HOOKS>, dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col, b->w,
                                b->F[0], b->F[1], b->dist);
-            hipLaunchKernelGGL(k_mb_sssp, dim3(gci), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->in_nch(), b->in_chunks(), b->in_col(),
+            hipLaunchKernelGGL(k_mb_sssp<MG_SSSP_HOOKS>, dim3(gci), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->in_nch(), b->in_chunks(), b->in_col(),
                                b->in_w(), b->F[0], b->F[1], b->dist);
             hipLaunchKernelGGL(k_mb_count, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->row, b->F[0], b->F[1], b->part_n,
                                b->part_e, b->part_x);

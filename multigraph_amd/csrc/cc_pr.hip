@@ -1,7 +1,7 @@
 // Connected components (the role of MultiGraph_E_E + MultiGraph_V_V, process.cu:940-1077)
 // and PageRank (the role of MultiGraph_V_E_V_NOTIDEM, processi.cu) for gfx950.
 #include "engine.hpp"
-#include "user_function.hpp"
+#include "hook_calls.hpp"
 #include "lb.hpp"
 
 using namespace mg;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_hook_lb(int nch, long long ne, co
         const int e0 = c * kEPB;
         const int e1 = (int)min((long long)e0 + kEPB, ne);
         chunk_owners(off, cstart[c], cstart[c + 1], e0, e1, s_own);
-        for (int e = e0 + threadIdx.x; e < e1; e += kBlock) CCHooks::fused_update_condition(parent, s_own[e - e0], ld_stream(&adj[e]));
+        for (int e = e0 + threadIdx.x; e < e1; e += kBlock) MG_CC_HOOKS::fused_update_condition(parent, s_own[e - e0], ld_stream(&adj[e]));
         __syncthreads();
     }
 }
@@ -178,7 +178,7 @@ __global__ void k_cc_link_min(int nv, const int *__restrict__ row, const int *__
 __global__ void k_cc_link_second(int nv, const int *__restrict__ row, const int *__restrict__ col, int *__restrict__ parent) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
         const int b = row[v] + 1;
-        if (b < row[v + 1]) CCHooks::fused_update_condition(parent, v, col[b]);
+        if (b < row[v + 1]) MG_CC_HOOKS::fused_update_condition(parent, v, col[b]);
     }
 }
 
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(1024) void k_cc_sample(int nv, int *__restrict__ pa
     __shared__ int s[1024];
     const int t = threadIdx.x;
     const long long v = ((long long)t * 2654435761ll) % nv;
-    s[t] = CCHooks::find_ro(parent, (int)v);
+    s[t] = MG_CC_HOOKS::find_ro(parent, (int)v);
     __syncthreads();
     for (int k = 2; k <= 1024; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -224,18 +224,18 @@ __global__ __launch_bounds__(kBlock) void k_cc_link_rest(int nv, int k, const in
         if (threadIdx.x == 0) s_wn = 0;
         __syncthreads();
         const int v = base + threadIdx.x;
-        if (v < nv && CCHooks::find(parent, v) != c) {
+        if (v < nv && MG_CC_HOOKS::find(parent, v) != c) {
             const int r0 = row[v], b = r0 + k, e = row[v + 1];
             // k_cc_link_min joined v with its smallest neighbour only if that one is smaller
-            if (e > r0 && col[r0] > v) CCHooks::fused_update_condition(parent, v, col[r0]);
+            if (e > r0 && col[r0] > v) MG_CC_HOOKS::fused_update_condition(parent, v, col[r0]);
             if (e - b >= 64) s_wv[atomicAdd(&s_wn, 1)] = v;
-            else for (int p = b; p < e; ++p) CCHooks::fused_update_condition(parent, v, col[p]);
+            else for (int p = b; p < e; ++p) MG_CC_HOOKS::fused_update_condition(parent, v, col[p]);
         }
         __syncthreads();
         const int wn = s_wn;
         for (int i = threadIdx.x >> 6; i < wn; i += kBlock / MG_WAVE) {
             const int u = s_wv[i];
-            for (int p = row[u] + k + lane(); p < row[u + 1]; p += MG_WAVE) CCHooks::fused_update_condition(parent, u, col[p]);
+            for (int p = row[u] + k + lane(); p < row[u + 1]; p += MG_WAVE) MG_CC_HOOKS::fused_update_condition(parent, u, col[p]);
         }
         __syncthreads();
     }
@@ -243,12 +243,17 @@ __global__ __launch_bounds__(kBlock) void k_cc_link_rest(int nv, int k, const in
 
 __global__ void k_cc_compress(int nv, int *__restrict__ parent) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x)
-        CCHooks::update_condition(parent, v);
+        MG_CC_HOOKS::update_condition(parent, v);
 }
 
 // ------------------------------------------------------------------ PageRank (pull)
 // psum[v] = sum over neighbours u of c[u] (c = rank/deg), FP64, degree tiers as in
 // the BC backward pass; then apply the PR_T / PR_D rule.
+// CSR pull for PageRank: sums start at initialize's identity, every in-edge adds its
+// source's message (accumulate_nonAtomics in registers, lanes merged the same way), a
+// split vertex's chunks add theirs with accumulate; k_pr_apply applies update_condition
+// and computes the next messages with get_partial_result.
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_pr_gather(int nv, int huge, const int *__restrict__ row,
                                                       const int *__restrict__ col, const float *__restrict__ c,
                                                       double *__restrict__ psum) {
@@ -261,28 +266,32 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(int nv, int huge, const in
         if (v < nv) {
             const int rb = row[v], re = row[v + 1], deg = re - rb;
             if (deg < 32) {
-                double s = 0.0;
-                for (int p = rb; p < re; ++p) s += (double)c[ld_stream(&col[p])];
+                double s;
+                H::initialize(&s);
+                for (int p = rb; p < re; ++p) H::accumulate_nonAtomics(&s, (double)c[ld_stream(&col[p])]);
                 psum[v] = s;
             } else if (deg < huge) {
                 s_wv[atomicAdd(&s_wn, 1)] = v;
             } else {
-                psum[v] = 0.0;
+                H::initialize(&psum[v]);
             }
         }
         __syncthreads();
         const int wn = s_wn;
         for (int k = threadIdx.x >> 6; k < wn; k += kBlock / MG_WAVE) {
             const int u = s_wv[k];
-            double s = 0.0;
-            for (int p = row[u] + lane(); p < row[u + 1]; p += MG_WAVE) s += (double)c[ld_stream(&col[p])];
-            s = wave_sum(s);
+            double s;
+            H::initialize(&s);
+            for (int p = row[u] + lane(); p < row[u + 1]; p += MG_WAVE) H::accumulate_nonAtomics(&s, (double)c[ld_stream(&col[p])]);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) H::accumulate_nonAtomics(&s, __shfl_xor(s, o));
             if (lane() == 0) psum[u] = s;
         }
         __syncthreads();
     }
 }
 
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_pr_gather_huge(const HugeChunk *__restrict__ chunks, int nch,
                                                            const int *__restrict__ row, const int *__restrict__ col,
                                                            const float *__restrict__ c, double *__restrict__ psum) {
@@ -290,20 +299,24 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather_huge(const HugeChunk *__re
     for (int k = blockIdx.x; k < nch; k += gridDim.x) {
         const HugeChunk hc = chunks[k];
         const int b = row[hc.v] + hc.j * kHugeChunk, e = min(row[hc.v + 1], b + kHugeChunk);
-        double s = 0.0;
-        for (int p = b + threadIdx.x; p < e; p += kBlock) s += (double)c[ld_stream(&col[p])];
-        s = wave_sum(s);
+        double s;
+        H::initialize(&s);
+        for (int p = b + threadIdx.x; p < e; p += kBlock) H::accumulate_nonAtomics(&s, (double)c[ld_stream(&col[p])]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) H::accumulate_nonAtomics(&s, __shfl_xor(s, o));
         if (lane() == 0) s_t[threadIdx.x >> 6] = s;
         __syncthreads();
         if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int i = 0; i < kBlock / MG_WAVE; ++i) t += s_t[i];
-            atomicAdd(&psum[hc.v], t);
+            double t;
+            H::initialize(&t);
+            for (int i = 0; i < kBlock / MG_WAVE; ++i) H::accumulate_nonAtomics(&t, s_t[i]);
+            H::accumulate(&psum[hc.v], t);
         }
         __syncthreads();
     }
 }
 
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_pr_apply(int nv, int variant, float damping, float tol,
                                                      const int *__restrict__ deg, const double *__restrict__ psum,
                                                      float *__restrict__ rank, float *__restrict__ c,
@@ -317,10 +330,9 @@ __global__ __launch_bounds__(kBlock) void k_pr_apply(int nv, int variant, float 
     for (int v = blockIdx.x * kBlock + threadIdx.x; v < nv; v += gridDim.x * kBlock) {
         float f1;
         const float old = rank[v];
-        ch |= PRHooks::update_condition(&rank[v], &active[v], psum[v], variant, damping, tol, &f1);
+        ch |= H::update_condition(&rank[v], &active[v], psum[v], variant, damping, tol, &f1);
         res += fabs((double)f1 - (double)old);
-        const int d = deg[v];
-        c[v] = d > 0 ? rank[v] / (float)d : 0.f;
+        H::get_partial_result(&c[v], rank[v], deg[v]);
     }
     res = block_sum<kBlock / MG_WAVE>(res, s_tmp);
     if (ch) s_ch = 1;
@@ -554,14 +566,14 @@ static int pr_launch_iteration(mg_engine *e) {
     hipLaunchKernelGGL(k_pr_begin, dim3(1), dim3(1), 0, s, e->ctl);
     // algorithmic bytes of one pull iteration: row + col + contribution gather per edge
     if (e->prof) mg_prof_begin(e, "k_pr_gather", 4.0 * (e->nv + 1) + 8.0 * e->ne + 8.0 * e->nv, &slot);
-    hipLaunchKernelGGL(k_pr_gather, dim3(grid_for(e->nv, kBlock, G)), dim3(kBlock), 0, s, e->nv, e->opt.huge_degree, e->row,
+    hipLaunchKernelGGL(k_pr_gather<MG_PR_HOOKS>, dim3(grid_for(e->nv, kBlock, G)), dim3(kBlock), 0, s, e->nv, e->opt.huge_degree, e->row,
                        e->col, e->cval, e->psum);
     if (e->prof) mg_prof_end(e, slot);
     if (e->chunk_cap > 0)
-        hipLaunchKernelGGL(k_pr_gather_huge, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, e->chunks,
+        hipLaunchKernelGGL(k_pr_gather_huge<MG_PR_HOOKS>, dim3(grid_for(e->chunk_cap, 1, G)), dim3(kBlock), 0, s, e->chunks,
                            (int)e->chunk_cap, e->row, e->col, e->cval, e->psum);
     if (e->prof) mg_prof_begin(e, "k_pr_apply", 28.0 * e->nv, &slot);
-    hipLaunchKernelGGL(k_pr_apply, dim3(grid_for(e->nv, kBlock, 4096)), dim3(kBlock), 0, s, e->nv, variant, e->opt.pr_damping,
+    hipLaunchKernelGGL(k_pr_apply<MG_PR_HOOKS>, dim3(grid_for(e->nv, kBlock, 4096)), dim3(kBlock), 0, s, e->nv, variant, e->opt.pr_damping,
                        e->opt.pr_tolerance, e->deg, e->psum, e->rank, e->cval, e->vfc, e->ctl);
     if (e->prof) mg_prof_end(e, slot);
     hipLaunchKernelGGL(k_pr_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->opt.pr_max_iter, e->ne);

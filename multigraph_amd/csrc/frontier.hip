@@ -21,7 +21,7 @@
 #include <mutex>
 
 #include "engine.hpp"
-#include "user_function.hpp"
+#include "hook_calls.hpp"
 #include "lb.hpp"
 
 using namespace mg;
@@ -73,36 +73,6 @@ __global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf, Ctl *
         __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
-
-// relax one edge src->u (edge index p); returns 1 when u joins the next frontier
-template <class H>
-struct Relax;
-
-template <>
-struct Relax<BFSHooks> {
-    __device__ static int go(int sval, double, int u, int, const int *, int *state, int *, double *, int iter) {
-        int msg;
-        BFSHooks::get_partial_result(&msg, sval);
-        return BFSHooks::fused_update_condition(&state[u], &state[u], msg, iter, nullptr);
-    }
-};
-template <>
-struct Relax<SSSPHooks> {
-    __device__ static int go(int sval, double, int u, int p, const int *wgt, int *state, int *vfc, double *, int iter) {
-        int msg;
-        SSSPHooks::get_partial_result(&msg, sval, wgt[p]);
-        return SSSPHooks::fused_update_condition(&state[u], &state[u], msg, iter, &vfc[u]);
-    }
-};
-template <>
-struct Relax<BCHooks> {
-    __device__ static int go(int sval, double s2, int u, int, const int *, int *state, int *, double *sigma, int iter) {
-        int msg;
-        double msg2;
-        BCHooks::get_partial_result(&msg, sval, &msg2, s2);
-        return BCHooks::fused_update_condition(&state[u], &state[u], msg, iter, nullptr, &sigma[u], &sigma[u], msg2);
-    }
-};
 
 template <bool kTrackDeg>
 __device__ __forceinline__ void append(Ctl *ctl, int *q, int out_begin, const int *row, bool win, int u) {
@@ -160,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
                 }
                 const int p = s_rb[lo] + (e - s_off[lo]);
                 u = ld_stream(&col[p]);
-                win = Relax<H>::go(s_val[lo], s_val2[lo], u, p, wgt, state, vfc, sigma, iter);
+                win = relax<H>(s_val[lo], s_val2[lo], u, p, wgt, state, vfc, sigma, iter);
             }
             append<kTrackDeg>(ctl, q, out_begin, row, win != 0, u);
         }
@@ -293,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_push_lb(Ctl *ctl, const int *__restr
                 }
                 const int p = rb + (e - off);
                 u = ld_stream(&col[p]);
-                win = Relax<H>::go(state[v], sigma ? sigma[v] : 0.0, u, p, wgt, state, vfc, sigma, iter);
+                win = relax<H>(state[v], sigma ? sigma[v] : 0.0, u, p, wgt, state, vfc, sigma, iter);
             }
             append<true>(ctl, q, out_begin, row, win != 0, u);
         }
@@ -322,12 +292,16 @@ __global__ __launch_bounds__(kBlock) void k_level2bm(Ctl *ctl, int nv, const int
 
 __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[u >> 6] >> (u & 63)) & 1ull; }
 
-// Pull level for BFS (early exit at the first parent) and for the BC forward pass
-// (all parents, summing their path counts).  row / col are the in-edges; orow, the
-// out-edge rows of a directed graph (null when symmetric), gives the degree sum.
-// The new frontier is the bitmap itself; its size and degree sum leave as one
-// partial per workgroup (no global atomics), summed by k_reduce_parts.
-template <bool kBC>
+// Pull level for BFS and the BC forward pass: every unvisited vertex accumulates the
+// messages of its parents in the current frontier (a parent u of the frontier has
+// level iter - 1; get_partial_result gives its message, accumulate_nonAtomics combines
+// it into the vertex's register accumulator) and applies the result with
+// update_condition.  BFS stops at the first parent found (its hooks say the first
+// message is final); BC scans every parent, summing their path counts.  row / col are
+// the in-edges; orow, the out-edge rows of a directed graph (null when symmetric),
+// gives the degree sum.  The new frontier is the bitmap itself; its size and degree sum
+// leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, const int *__restrict__ row,
                                                  const int *__restrict__ col, const int *__restrict__ orow,
                                                  int *__restrict__ level,
@@ -335,7 +309,9 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
                                                  uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
                                                  int *__restrict__ part_n, long long *__restrict__ part_mf) {
     if (ctl->mode != MODE_BU) return;
+    constexpr bool kSum = !H::kFirstWins;  // every parent contributes
     const int iter = ctl->iter;
+    const int plev = iter - 1;  // level of the frontier's vertices
     const uint64_t *fr = ctl->bm_cur ? bm1 : bm0;
     uint64_t *nx = ctl->bm_cur ? bm0 : bm1;
     const int words = (nv + 63) >> 6;
@@ -346,24 +322,25 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
     for (int w = wave; w < words; w += nwaves) {
         const uint64_t vw = vis[w];
         const int v = (w << 6) + lane();
-        bool found = false;
-        double s = 0.0;
+        Acc<H> a;
+        acc_init(a);
         int b = 0, e = 0, deg = 0;
-        if (vw != ~0ull && !((vw >> lane()) & 1ull)) {
+        const bool open = vw != ~0ull && !((vw >> lane()) & 1ull);
+        if (open) {
             b = row[v];
             e = row[v + 1];
             deg = e - b;
             if (e - b > cut) e = b;  // left to k_pull_huge
-            if (!kBC) {
+            if (!kSum) {
                 for (int p = b; p < e; ++p)
-                    if (bm_test(fr, ld_stream(&col[p]))) { found = true; break; }
+                    if (bm_test(fr, ld_stream(&col[p]))) { acc_edge<H>(a, plev, 0.0, 0, iter); break; }
             }
         }
-        if (kBC) {
-            // BC scans every parent: lanes walk rows of at most kPullLane edges alone,
-            // stepping together for the longest of them; longer rows are walked by the
-            // whole wave, one at a time (lanes over edges, a wave sum).  Path counts
-            // are integers, exact in FP64 whatever the order of the additions.
+        if (kSum) {
+            // lanes walk rows of at most kPullLane edges alone, stepping together for the
+            // longest of them; longer rows are walked by the whole wave, one at a time
+            // (lanes over edges, then the lanes' accumulators merged).  Path counts are
+            // integers, exact in FP64 whatever the order of the additions.
             const bool alone = e - b <= kPullLane;
             int steps = alone ? e - b : 0;
 #pragma unroll
@@ -385,29 +362,30 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
                 for (int j = 0; j < 4; ++j) sv[j] = f[j] ? sigma[u[j]] : 0.0;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (f[j]) { found = true; s += sv[j]; }
+                    if (f[j]) acc_edge<H>(a, plev, sv[j], 0, iter);
             }
             uint64_t heavy = ballot(!alone);
             while (heavy) {
                 const int l = __builtin_ctzll(heavy);
                 heavy &= heavy - 1;
                 const int q0 = __shfl(b, l), q1 = __shfl(e, l);
-                double hs = 0.0;
-                bool hf = false;
+                Acc<H> h;
+                acc_init(h);
                 for (int p = q0 + lane(); p < q1; p += MG_WAVE) {
                     const int u = ld_stream(&col[p]);
-                    if (bm_test(fr, u)) { hf = true; hs += sigma[u]; }
+                    if (bm_test(fr, u)) acc_edge<H>(h, plev, sigma[u], 0, iter);
                 }
-                hs = wave_sum(hs);
-                const bool any = ballot(hf) != 0ull;
-                if (lane() == l) { found = any; s = hs; }
+                h = acc_wave<H>(h, iter);
+                if (lane() == l) a = h;
             }
         }
+        // an unvisited vertex's level is the identity: its register copy is the old value
+        Acc<H> none;
+        acc_init(none);
+        const bool found = open && acc_apply<H>(none.v, &level[v], sigma ? &sigma[v] : nullptr, nullptr, a, iter);
         const uint64_t m = ballot(found);
         if (lane() == 0) { nx[w] = m; if (m) vis[w] = vw | m; }
         if (found) {
-            level[v] = iter;
-            if (kBC) sigma[v] = s;
             cnt += 1;
             mf += orow ? orow[v + 1] - orow[v] : deg;  // out-degree: the next level's push edges
         }
@@ -420,64 +398,81 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
 
 // Unvisited vertices above kPullHuge edges, cut into static kPullChunk-edge chunks
 // (PullChunk {v, p0, p1, k}, k = the vertex's slot): a wave per chunk scans it and
-// adds what it finds into the vertex's slot (found flag, FP64 path-count sum: exact
-// in any order); k_pull_huge_fin then updates level, sigma, bitmaps and counts.
-// Without this split, one lane walks a 160 K-edge hub alone (BC scans every parent).
-template <bool kBC>
+// merges its accumulator into the vertex's slot (hlev, hsum) with accumulate (FP64
+// path-count sums: exact in any order); k_pull_huge_fin then applies the slots to
+// level, sigma, bitmaps and counts and resets them to the identity.  Without this
+// split, one lane walks a 160 K-edge hub alone (BC scans every parent).
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_pull_huge(Ctl *ctl, int nch, const PullChunk *__restrict__ ch,
                                                       const int *__restrict__ col, const double *__restrict__ sigma,
                                                       const uint64_t *__restrict__ vis, const uint64_t *__restrict__ bm0,
-                                                      const uint64_t *__restrict__ bm1, int *__restrict__ hfound,
+                                                      const uint64_t *__restrict__ bm1, int *__restrict__ hlev,
                                                       double *__restrict__ hsum) {
     if (ctl->mode != MODE_BU) return;
+    const int iter = ctl->iter;
     const uint64_t *fr = ctl->bm_cur ? bm1 : bm0;
     const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * kBlock) >> 6;
     for (int c = wave; c < nch; c += nwaves) {
         const PullChunk k = ch[c];
         if (bm_test(vis, k.v)) continue;
+        Acc<H> a;
+        acc_init(a);
         bool found = false;
-        double sum = 0.0;
         for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
             const int p = p0 + lane();
             bool f = false;
             if (p < k.p1) {
                 const int u = ld_stream(&col[p]);
                 f = bm_test(fr, u);
-                if (kBC && f) sum += sigma[u];
+                if (f) acc_edge<H>(a, iter - 1, H::kV2 ? sigma[u] : 0.0, 0, iter);
             }
             const bool any = ballot(f) != 0;
             found = found || any;
-            if (!kBC && any) break;
+            if (H::kFirstWins && any) break;
         }
-        if (kBC) sum = wave_sum(sum);
-        if (lane() == 0 && found) {
-            atomicOr(&hfound[k.k], 1);
-            if (kBC) atomicAdd(&hsum[k.k], sum);
-        }
+        if (!found) continue;
+        a = acc_wave<H>(a, iter);
+        if (lane() == 0) acc_atomic<H>(&hlev[k.k], H::kV2 ? &hsum[k.k] : nullptr, a, iter);
     }
 }
 
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_pull_huge_fin(Ctl *ctl, int nh, const int *__restrict__ hv,
                                                           const int *__restrict__ row, int *__restrict__ level,
                                                           double *__restrict__ sigma, uint64_t *__restrict__ vis,
                                                           uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
-                                                          int *__restrict__ hfound, double *__restrict__ hsum,
-                                                          int *__restrict__ part_n, long long *__restrict__ part_mf,
-                                                          int with_sigma) {
+                                                          int *__restrict__ hlev, double *__restrict__ hsum,
+                                                          int *__restrict__ part_n, long long *__restrict__ part_mf) {
     if (ctl->mode != MODE_BU) return;
+    const int iter = ctl->iter;
     uint64_t *nx = ctl->bm_cur ? bm0 : bm1;
     for (int k = blockIdx.x * kBlock + threadIdx.x; k < nh; k += gridDim.x * kBlock) {
-        if (!hfound[k]) continue;
+        Acc<H> a;
+        a.v = hlev[k];
+        a.s = H::kV2 ? hsum[k] : 0.0;
+        if (acc_is_identity<H>(a)) continue;
         const int v = hv[k];
-        level[v] = ctl->iter;
-        if (with_sigma) sigma[v] = hsum[k];
+        Acc<H> none;
+        acc_init(none);
+        hlev[k] = none.v;  // the slot back to the identity for the next level
+        if (H::kV2) hsum[k] = none.s;
+        if (bm_test(vis, v) || !acc_apply<H>(none.v, &level[v], H::kV2 ? &sigma[v] : nullptr, nullptr, a, iter)) continue;
         atomicOr((unsigned long long *)&nx[v >> 6], 1ull << (v & 63));
         atomicOr((unsigned long long *)&vis[v >> 6], 1ull << (v & 63));
         atomicAdd(&part_n[0], 1);
         atomicAdd((unsigned long long *)&part_mf[0], (unsigned long long)(row[v + 1] - row[v]));
-        hfound[k] = 0;
-        hsum[k] = 0.0;
+    }
+}
+
+// slots of the split pull vertices at the identity (once, at build)
+template <class H>
+__global__ void k_pull_slots_init(int nh, int *__restrict__ hlev, double *__restrict__ hsum) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nh; k += gridDim.x * blockDim.x) {
+        Acc<H> a;
+        acc_init(a);
+        hlev[k] = a.v;
+        hsum[k] = a.s;
     }
 }
 
@@ -571,10 +566,13 @@ __global__ __launch_bounds__(kBlock) void k_bmq_write(Ctl *ctl, int words, const
 // redundant relaxations it would skip.
 constexpr int kPE = kEPB / kBlock;  // consecutive edges per lane (8)
 
-__device__ __forceinline__ void sssp_flush(int *s_best, int k, int best) {
-    if (best != (int)MG_INF) atomicMin(&s_best[k], best);
-}
-
+// Hooks: a lane's running minimum is its register accumulator (initialize,
+// get_partial_result per edge, accumulate_nonAtomics); an owner run is flushed into the
+// owner's LDS slot with accumulate (LDS atomicMin); an owner whose row lies inside the
+// chunk is applied with update_condition, one that straddles a chunk boundary (another
+// workgroup may update it too) with fused_update_condition.  Unreached sources
+// (distance MG_INF) send nothing.
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long long ne, const int *__restrict__ row,
                                                       const int *__restrict__ col, const int *__restrict__ wgt,
                                                       const int *__restrict__ cstart, int *__restrict__ dist,
@@ -583,6 +581,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
     const int iter = ctl->iter;
     __shared__ int s_row[kEPB + 2];
     __shared__ int s_best[kEPB + 1];
+    Acc<H> ident;
+    acc_init(ident);
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const int e0 = c * kEPB;
         const int e1 = (int)min((long long)e0 + kEPB, ne);
@@ -610,39 +610,43 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
                 if (e >= e1 || du[j] == (int)MG_INF) continue;
                 int lo = u0, hi = u1;
                 while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (row[mid] <= e) lo = mid; else hi = mid - 1; }
-                const int cand = du[j] + wv[j];
-                if (atomicMin(&dist[lo], cand) > cand) vfc[lo] = iter;
+                Acc<H> a = ident;
+                acc_edge<H>(a, du[j], 0.0, wv[j], iter);
+                acc_apply_atomic<H>(&dist[lo], nullptr, &vfc[lo], a, iter);
             }
             continue;  // uniform: nown is the same for the whole workgroup
         }
         for (int k = threadIdx.x; k <= nown; k += kBlock) s_row[k] = row[u0 + k];
-        for (int k = threadIdx.x; k < nown; k += kBlock) s_best[k] = MG_INF;
+        for (int k = threadIdx.x; k < nown; k += kBlock) s_best[k] = ident.v;
         __syncthreads();
         if (eb < e1) {
             int lo = 0, hi = nown - 1;  // owner of edge eb: the last k with s_row[k] <= eb
             while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_row[mid] <= eb) lo = mid; else hi = mid - 1; }
-            int k = lo, next = s_row[lo + 1], best = MG_INF;
+            int k = lo, next = s_row[lo + 1];
+            Acc<H> a = ident;
 #pragma unroll
             for (int j = 0; j < kPE; ++j) {
                 const int e = eb + j;
                 if (e < e1) {
-                    while (e >= next) { sssp_flush(s_best, k, best); best = MG_INF; ++k; next = s_row[k + 1]; }
-                    if (du[j] != (int)MG_INF) best = min(best, du[j] + wv[j]);
+                    while (e >= next) {
+                        if (a.v != ident.v) acc_atomic<H>(&s_best[k], nullptr, a, iter);
+                        a = ident;
+                        ++k;
+                        next = s_row[k + 1];
+                    }
+                    if (du[j] != (int)MG_INF) acc_edge<H>(a, du[j], 0.0, wv[j], iter);
                 }
             }
-            sssp_flush(s_best, k, best);
+            if (a.v != ident.v) acc_atomic<H>(&s_best[k], nullptr, a, iter);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < nown; k += kBlock) {
-            const int best = s_best[k];
+            Acc<H> a = ident;
+            a.v = s_best[k];
+            if (a.v == ident.v) continue;
             const int o = u0 + k;
-            if (best == (int)MG_INF || best >= dist[o]) continue;
-            if (s_row[k] < e0 || s_row[k + 1] > e1) {
-                if (atomicMin(&dist[o], best) > best) vfc[o] = iter;
-            } else {
-                dist[o] = best;
-                vfc[o] = iter;
-            }
+            if (s_row[k] < e0 || s_row[k + 1] > e1) acc_apply_atomic<H>(&dist[o], nullptr, &vfc[o], a, iter);
+            else acc_apply<H>(dist[o], &dist[o], nullptr, &vfc[o], a, iter);
         }
         __syncthreads();
     }
@@ -695,16 +699,18 @@ __device__ __forceinline__ bool bc_next(int l, int w, const uint8_t *__restrict_
 // tiers: a lane per vertex below 32 edges, a wave per vertex below 2048, and the
 // whole grid (4096-edge chunks, FP64 partials) above.  Sums are Neumaier-compensated
 // FP32 over terms split into two floats, so each term keeps ~48 bits.
-// term sigma[v] * c[w] with c[w] = (1 + delta[w]) / sigma[w], stored in FP64 once per
-// vertex when its delta is final (one gather per successor edge, no division)
+// The per-edge rule, its compensated sum and the vertex update are BCHooks' backward
+// hooks (back_term, back_accumulate, back_update): c[w] = (1 + delta[w]) / sigma[w] is
+// stored in FP64 once per vertex when its delta is final, so a successor edge costs one
+// gather and a multiply, no division.
+template <class H>
 __device__ __forceinline__ void bc_term(KSum &acc, double sv, double cw) {
-    const double t = sv * cw;
-    float hi = (float)t;
-    float lo = (float)(t - (double)hi);
-    acc.add(hi);
-    acc.add(lo);
+    double t;
+    H::back_term(&t, sv, cw);
+    H::back_accumulate(&acc, t);
 }
 
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
                                                     const int *__restrict__ row, const int *__restrict__ col,
                                                     const int *__restrict__ level, const uint8_t *__restrict__ lev8,
@@ -726,11 +732,9 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
                 KSum acc;
                 for (int p = rb; p < re; ++p) {
                     const int wv = col[p];
-                    if (bc_next(l, wv, lev8, level)) bc_term(acc, sv, cw[wv]);
+                    if (bc_next(l, wv, lev8, level)) bc_term<H>(acc, sv, cw[wv]);
                 }
-                const float dv = acc.value();
-                delta[v] = dv;
-                cw[v] = (1.0 + (double)dv) / sv;
+                H::back_update(&delta[v], &cw[v], acc.value(), sv);
             } else if (deg < 2048) {
                 s_wv[atomicAdd(&s_wn, 1)] = v;
             } else {
@@ -748,19 +752,18 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
             KSum acc;
             for (int p = rb + lane(); p < re; p += MG_WAVE) {
                 const int wv = ld_stream(&col[p]);
-                if (bc_next(l, wv, lev8, level)) bc_term(acc, sv, cw[wv]);
+                if (bc_next(l, wv, lev8, level)) bc_term<H>(acc, sv, cw[wv]);
             }
             acc = wave_ksum(acc);
             if (lane() == 0) {
-                const float dv = acc.value();
-                delta[v] = dv;
-                cw[v] = (1.0 + (double)dv) / sv;
+                H::back_update(&delta[v], &cw[v], acc.value(), sv);
             }
         }
         __syncthreads();
     }
 }
 
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__restrict__ row, const int *__restrict__ col,
                                                          const int *__restrict__ level, const uint8_t *__restrict__ lev8,
                                                          const double *__restrict__ sigma,
@@ -776,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
         KSum acc;
         for (int p = b + threadIdx.x; p < e; p += kBlock) {
             const int wv = ld_stream(&col[p]);
-            if (bc_next(l, wv, lev8, level)) bc_term(acc, sv, cw[wv]);
+            if (bc_next(l, wv, lev8, level)) bc_term<H>(acc, sv, cw[wv]);
         }
         acc = wave_ksum(acc);
         if (lane() == 0) { s_s[threadIdx.x >> 6] = acc.s; s_c[threadIdx.x >> 6] = acc.c; }
@@ -790,15 +793,14 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
     }
 }
 
+template <class H>
 __global__ void k_bc_back_fin(const HugeChunk *__restrict__ chunks, int *__restrict__ nchunks, double *__restrict__ dacc,
                               const double *__restrict__ sigma, float *__restrict__ delta, double *__restrict__ cw) {
     const int nch = *nchunks;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nch; c += gridDim.x * blockDim.x) {
         if (chunks[c].j != 0) continue;  // the first chunk of each vertex finalizes it
         const int v = chunks[c].v;
-        const float dv = (float)dacc[v];
-        delta[v] = dv;
-        cw[v] = (1.0 + (double)dv) / sigma[v];
+        H::back_update(&delta[v], &cw[v], (float)dacc[v], sigma[v]);
         dacc[v] = 0.0;
     }
 }
@@ -809,6 +811,7 @@ __global__ void k_bc_back_fin(const HugeChunk *__restrict__ chunks, int *__restr
 // than level l (from a hub, the widest level sits above a thin one).  Tiers as in the
 // pull: a lane per vertex below 32 edges, a wave below 2048, kHugeChunk-edge chunks
 // above (k_bc_push_huge).
+template <class H>
 __global__ __launch_bounds__(kBlock) void k_bc_push(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
                                                     const int *__restrict__ row, const int *__restrict__ col,
                                                     const int *__restrict__ level, const uint8_t *__restrict__ lev8,
@@ -872,15 +875,16 @@ __global__ __launch_bounds__(kBlock) void k_bc_push_huge(int l, const int *__res
 }
 
 // level l after a push: delta = sigma * sum, its c for the level above, sums cleared
+template <class H>
 __global__ void k_bc_push_fin(int l, const int *__restrict__ lvl_off, const int *__restrict__ q, const double *__restrict__ sigma,
                               double *__restrict__ dacc, float *__restrict__ delta, double *__restrict__ cw) {
     const int b = lvl_off[l], n = lvl_off[l + 1] - b;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int v = q[b + i];
         const double sv = sigma[v];
-        const float dv = (float)(sv * dacc[v]);
-        delta[v] = dv;
-        cw[v] = (1.0 + (double)dv) / sv;
+        double t;  // the successors' c summed by the push: delta = sigma[v] * sum
+        H::back_term(&t, sv, dacc[v]);
+        H::back_update(&delta[v], &cw[v], (float)t, sv);
         dacc[v] = 0.0;
     }
 }
@@ -991,8 +995,12 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
             MG_HIP(hipMalloc(&e->pull_sum, sizeof(double) * hv.size()));
             MG_HIP(hipMemcpy(e->pull_ch, pc.data(), sizeof(PullChunk) * pc.size(), hipMemcpyHostToDevice));
             MG_HIP(hipMemcpy(e->pull_hv, hv.data(), sizeof(int) * hv.size(), hipMemcpyHostToDevice));
-            MG_HIP(hipMemset(e->pull_found, 0, sizeof(int) * hv.size()));
-            MG_HIP(hipMemset(e->pull_sum, 0, sizeof(double) * hv.size()));
+            if (e->algo == MG_BC)
+                hipLaunchKernelGGL(k_pull_slots_init<MG_BC_HOOKS>, dim3(grid_for((long long)hv.size(), 256, 64)), dim3(256), 0,
+                                   e->stream, (int)hv.size(), e->pull_found, e->pull_sum);
+            else
+                hipLaunchKernelGGL(k_pull_slots_init<MG_BFS_HOOKS>, dim3(grid_for((long long)hv.size(), 256, 64)), dim3(256), 0,
+                                   e->stream, (int)hv.size(), e->pull_found, e->pull_sum);
         }
     }
     if (e->algo == MG_BC) {
@@ -1080,13 +1088,13 @@ static int launch_level(mg_engine *e, bool known) {
             if (e->prof) mg_prof_begin(e, "k_push", bytes, &slot);
             switch (e->algo) {
             case MG_BFS:
-                hipLaunchKernelGGL((k_push<BFSHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                hipLaunchKernelGGL((k_push<MG_BFS_HOOKS, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
                 break;
             case MG_SSSP:
-                hipLaunchKernelGGL((k_push<SSSPHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                hipLaunchKernelGGL((k_push<MG_SSSP_HOOKS, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
                 break;
             default:
-                hipLaunchKernelGGL((k_push<BCHooks, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                hipLaunchKernelGGL((k_push<MG_BC_HOOKS, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
                 break;
             }
             if (e->prof) mg_prof_end(e, slot);
@@ -1103,13 +1111,13 @@ static int launch_level(mg_engine *e, bool known) {
             if (e->prof) mg_prof_begin(e, "k_push_lb", bytes, &slot);
             switch (e->algo) {
             case MG_BFS:
-                hipLaunchKernelGGL((k_push_lb<BFSHooks>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                hipLaunchKernelGGL((k_push_lb<MG_BFS_HOOKS>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
                 break;
             case MG_SSSP:
-                hipLaunchKernelGGL((k_push_lb<SSSPHooks>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                hipLaunchKernelGGL((k_push_lb<MG_SSSP_HOOKS>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
                 break;
             default:
-                hipLaunchKernelGGL((k_push_lb<BCHooks>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                hipLaunchKernelGGL((k_push_lb<MG_BC_HOOKS>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
                 break;
             }
             if (e->prof) mg_prof_end(e, slot);
@@ -1125,7 +1133,7 @@ static int launch_level(mg_engine *e, bool known) {
             MG_TRY(mg_ps_sweep(e));
         } else {
         if (e->prof) mg_prof_begin(e, "k_pull_sssp", bytes, &slot);
-        hipLaunchKernelGGL(k_pull_sssp, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->in_row(),
+        hipLaunchKernelGGL(k_pull_sssp<MG_SSSP_HOOKS>, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->in_row(),
                            e->in_col(), e->in_w(), e->pcstart, e->level, e->vfc);
         if (e->prof) mg_prof_end(e, slot);
         }
@@ -1142,20 +1150,23 @@ static int launch_level(mg_engine *e, bool known) {
         // chunks (none was that long, or no table was built) walks every row here
         const int cut = e->pull_ch ? kPullHuge : 0x7FFFFFFF;
         if (e->algo == MG_BC)
-            hipLaunchKernelGGL((k_pull<true>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            hipLaunchKernelGGL((k_pull<MG_BC_HOOKS>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         else
-            hipLaunchKernelGGL((k_pull<false>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
         if (e->pull_nch > 0) {
             const int gh = grid_for(e->pull_nch, kBlock / MG_WAVE, G);
-            if (e->algo == MG_BC)
-                hipLaunchKernelGGL((k_pull_huge<true>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
+            const int gf = grid_for(e->pull_nh, kBlock, 64);
+            if (e->algo == MG_BC) {
+                hipLaunchKernelGGL((k_pull_huge<MG_BC_HOOKS>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
                                    e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
-            else
-                hipLaunchKernelGGL((k_pull_huge<false>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
+                hipLaunchKernelGGL((k_pull_huge_fin<MG_BC_HOOKS>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, e->row,
+                                   e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
+            } else {
+                hipLaunchKernelGGL((k_pull_huge<MG_BFS_HOOKS>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
                                    e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
-            hipLaunchKernelGGL(k_pull_huge_fin, dim3(grid_for(e->pull_nh, kBlock, 64)), dim3(kBlock), 0, s, e->ctl, e->pull_nh,
-                               e->pull_hv, e->row, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum,
-                               e->part_n, e->part_mf, e->algo == MG_BC ? 1 : 0);
+                hipLaunchKernelGGL((k_pull_huge_fin<MG_BFS_HOOKS>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, e->row,
+                                   e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
+            }
         }
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
@@ -1276,23 +1287,23 @@ int mg_bc_backward_impl(mg_engine *e) {
             const int n1 = off[l + 2] - off[l + 1];
             if (e->prof) mg_prof_begin(e, "k_bc_push", 0.0, &slot);
             if (n1 > 0) {
-                hipLaunchKernelGGL(k_bc_push, dim3(grid_for(n1, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->in_row(),
+                hipLaunchKernelGGL(k_bc_push<MG_BC_HOOKS>, dim3(grid_for(n1, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->in_row(),
                                    e->in_col(), e->level, e->lev8, e->bc_cw, e->dacc, e->chunks, nch);
                 hipLaunchKernelGGL(k_bc_push_huge, dim3(G), dim3(kBlock), 0, s, l, e->in_row(), e->in_col(), e->level, e->lev8, e->bc_cw,
                                    e->chunks, nch, e->dacc);
             }
-            hipLaunchKernelGGL(k_bc_push_fin, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, l, e->lvl_off, e->q, e->sigma, e->dacc,
+            hipLaunchKernelGGL(k_bc_push_fin<MG_BC_HOOKS>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, l, e->lvl_off, e->q, e->sigma, e->dacc,
                                e->delta, e->bc_cw);
             if (e->prof) mg_prof_end(e, slot);
             continue;
         }
         if (e->prof) mg_prof_begin(e, "k_bc_back", 0.0, &slot);
-        hipLaunchKernelGGL(k_bc_back, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
+        hipLaunchKernelGGL(k_bc_back<MG_BC_HOOKS>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
                            e->level, e->lev8, e->sigma, e->delta, e->bc_cw, e->chunks, nch);
         if (e->prof) mg_prof_end(e, slot);
-        hipLaunchKernelGGL(k_bc_back_huge, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->lev8, e->sigma, e->bc_cw,
+        hipLaunchKernelGGL(k_bc_back_huge<MG_BC_HOOKS>, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->lev8, e->sigma, e->bc_cw,
                            e->chunks, nch, e->dacc);
-        hipLaunchKernelGGL(k_bc_back_fin, dim3(64), dim3(256), 0, s, e->chunks, nch, e->dacc, e->sigma, e->delta, e->bc_cw);
+        hipLaunchKernelGGL(k_bc_back_fin<MG_BC_HOOKS>, dim3(64), dim3(256), 0, s, e->chunks, nch, e->dacc, e->sigma, e->delta, e->bc_cw);
     }
     MG_HIP(hipGetLastError());
     return MG_OK;

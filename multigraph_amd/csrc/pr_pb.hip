@@ -33,7 +33,7 @@
 #include <type_traits>
 #include <vector>
 #include "engine.hpp"
-#include "user_function.hpp"
+#include "hook_calls.hpp"
 
 using namespace mg;
 
@@ -353,6 +353,7 @@ __device__ __forceinline__ float *msg_out(const Ctl *ctl, float *msg, long long 
 
 // messages of partitions whose destination side was split over several gather units
 // (their ranks are final only after k_pb_apply_split); unit = {partition, a0, a1, 0}
+template <class H>
 __global__ __launch_bounds__(kBT) void k_pb_scatter_msg(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
                                                         const float *__restrict__ rank, const int *__restrict__ deg,
                                                         const uint16_t *__restrict__ a_src, const int *__restrict__ amap,
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(kBT) void k_pb_scatter_msg(Ctl *ctl, const int4 *__
         const int pv = part_count(un.x, nv, np);
         for (int k = threadIdx.x; k < pv; k += kBT) {
             float c;
-            PRHooks::get_partial_result(&c, rank[v0 + k], deg[v0 + k]);
+            H::get_partial_result(&c, rank[v0 + k], deg[v0 + k]);
             s_c[k] = c;
         }
         __syncthreads();
@@ -376,12 +377,13 @@ __global__ __launch_bounds__(kBT) void k_pb_scatter_msg(Ctl *ctl, const int4 *__
 }
 
 // apply the PR rule at v from its FP64 sum; returns (changed, |f1 - old|)
+template <class H>
 __device__ __forceinline__ int pr_apply(int v, double sum, int variant, float damping, float tol,
                                         const int *__restrict__ deg, float *__restrict__ rank, int *__restrict__ active,
                                         double *res) {
     float f1;
     const float old = rank[v];
-    const int ch = PRHooks::update_condition(&rank[v], &active[v], sum, variant, damping, tol, &f1);
+    const int ch = H::update_condition(&rank[v], &active[v], sum, variant, damping, tol, &f1);
     *res += fabs((double)f1 - (double)old);
     return ch;
 }
@@ -439,6 +441,11 @@ __device__ __forceinline__ void load_msg(Seg &sg, const SegHdr &h, const float *
 
 constexpr int kPer = 16;  // vertices per gather thread: P <= kPer * kBG
 
+// Hooks: the LDS accumulators start at initialize's identity, every edge adds its
+// message with accumulate (LDS FP64 atomic add), the partition's two hubs sum in
+// registers with accumulate_nonAtomics, the workgroup applies update_condition and
+// writes the next messages from get_partial_result.
+template <class H>
 __global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
                                                    int variant, float damping, float tol,
                                                    const uint16_t *__restrict__ b_dst, const int *__restrict__ gbase,
@@ -460,7 +467,7 @@ __global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restr
         const int4 un = units[u];
         const int v0 = un.x * P;
         const int pv = part_count(un.x, nv, np);
-        for (int k = threadIdx.x; k <= P; k += kBG) s_acc[k] = 0.0;
+        for (int k = threadIdx.x; k <= P; k += kBG) H::initialize(&s_acc[k]);
         __syncthreads();
         const int g0 = un.y / kG, g1 = un.z / kG;
         const int nseg = (g1 - g0 + kSeg - 1) / kSeg;
@@ -475,7 +482,9 @@ __global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restr
             si = min(si, nseg - 1);
             load_hdr(h, seg_lo(si), seg_hi(si), b_dst, gbase);
         };
-        double hub = 0.0, hub1 = 0.0;
+        double hub, hub1;
+        H::initialize(&hub);
+        H::initialize(&hub1);
         auto process = [&](const Seg &sg) {
             wave_sync();  // the previous segment's staged messages are consumed
 #pragma unroll
@@ -493,9 +502,9 @@ __global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restr
                 // takes ~10% of the edges, and lanes of one instruction adding to one LDS
                 // address serialise, so their sums stay in registers (0.4 % faster with
                 // id 1 too, 2.5 % slower with ids 0..3); padding (0x7FFF) adds nothing
-                if (dst == 0) hub += m;
-                else if (dst == 1) hub1 += m;
-                else if (dst != 0x7FFFu) atomicAdd(&s_acc[dst], m);
+                if (dst == 0) H::accumulate_nonAtomics(&hub, m);
+                else if (dst == 1) H::accumulate_nonAtomics(&hub1, m);
+                else if (dst != 0x7FFFu) H::accumulate(&s_acc[dst], m);
             }
         };
         int sgi = wid;
@@ -527,15 +536,20 @@ __global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restr
                 if (sgi >= nseg) break;
             }
         }
-        hub = wave_sum(hub);
-        hub1 = wave_sum(hub1);
-        if (lane() == 0 && hub != 0.0) atomicAdd(&s_acc[0], hub);
-        if (lane() == 0 && hub1 != 0.0) atomicAdd(&s_acc[1], hub1);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {  // the lanes' hub sums merged
+            H::accumulate_nonAtomics(&hub, __shfl_xor(hub, o));
+            H::accumulate_nonAtomics(&hub1, __shfl_xor(hub1, o));
+        }
+        double zero;
+        H::initialize(&zero);
+        if (lane() == 0 && hub != zero) H::accumulate(&s_acc[0], hub);
+        if (lane() == 0 && hub1 != zero) H::accumulate(&s_acc[1], hub1);
         __syncthreads();
         if (un.w) {  // partition split over several units: add partials, applied later
             for (int k = threadIdx.x; k < pv; k += kBG) {
                 const double a = s_acc[k];
-                if (a != 0.0) atomicAdd(&psum[v0 + k], a);
+                if (a != zero) H::accumulate(&psum[v0 + k], a);
             }
         } else {
             // apply, then write the partition's messages for the next iteration: the
@@ -546,8 +560,8 @@ __global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restr
                 const int k = threadIdx.x + j * kBG;
                 cl[j] = 0.f;
                 if (k < pv) {
-                    ch |= pr_apply(v0 + k, s_acc[k], variant, damping, tol, deg, rank, active, &res);
-                    PRHooks::get_partial_result(&cl[j], rank[v0 + k], deg[v0 + k]);
+                    ch |= pr_apply<H>(v0 + k, s_acc[k], variant, damping, tol, deg, rank, active, &res);
+                    H::get_partial_result(&cl[j], rank[v0 + k], deg[v0 + k]);
                 }
             }
             __syncthreads();
@@ -571,6 +585,7 @@ __global__ __launch_bounds__(kBG) void k_pb_gather(Ctl *ctl, const int4 *__restr
 }
 
 // vertices of split partitions: apply from the FP64 partial sums and clear them
+template <class H>
 __global__ __launch_bounds__(kBT) void k_pb_apply_split(Ctl *ctl, const int *__restrict__ parts, int nparts, int P, int nv, int np,
                                                         int variant, float damping, float tol, const int *__restrict__ deg,
                                                         float *__restrict__ rank, int *__restrict__ active,
@@ -584,8 +599,8 @@ __global__ __launch_bounds__(kBT) void k_pb_apply_split(Ctl *ctl, const int *__r
         const int p = parts[i / P], k = (int)(i % P);
         if (k >= part_count(p, nv, np)) continue;
         const int v = p * P + k;
-        ch |= pr_apply(v, psum[v], variant, damping, tol, deg, rank, active, &res);
-        psum[v] = 0.0;
+        ch |= pr_apply<H>(v, psum[v], variant, damping, tol, deg, rank, active, &res);
+        H::initialize(&psum[v]);
     }
     res = block_sum<kBT / MG_WAVE>(res, s_tmp);
     const int any = __syncthreads_or(ch);
@@ -678,7 +693,12 @@ __device__ __forceinline__ void load_msg_w(SegW &sg, const SegHdr &h, const uint
 
 // WT: uint8_t weights when every weight fits a byte (a slot's 8 weights are one 8-byte
 // load), else uint16_t (one 16-byte load); MT: the message width (see k_ps_scatter)
-template <class WT, class MT>
+// Hooks: LDS minima start at initialize's identity; an edge's candidate is
+// get_partial_result(message, weight), combined by accumulate into the LDS slot (LDS
+// atomicMin) or by accumulate_nonAtomics into the hub's register; a vertex applies
+// with update_condition, or fused_update_condition when its partition is split over
+// several workgroups.
+template <class H, class WT, class MT>
 __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restrict__ units, int nunits, int P, int nv, int np,
                                                    const uint16_t *__restrict__ b_dst, const WT *__restrict__ b_w,
                                                    const int *__restrict__ gbase, MT *__restrict__ msgbuf, long long npairs,
@@ -701,7 +721,7 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
         const int4 un = units[u];
         const int v0 = un.x * P;
         const int pv = part_count(un.x, nv, np);
-        for (int k = threadIdx.x; k <= P; k += kBG) s_min[k] = MG_INF;
+        for (int k = threadIdx.x; k <= P; k += kBG) H::initialize(&s_min[k]);
         __syncthreads();
         const int g0 = un.y / kG, g1 = un.z / kG;
         const int nseg = (g1 - g0 + kSeg - 1) / kSeg;
@@ -718,7 +738,8 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
                 wt = ld_stream_u4(&b_w[wg]);
             }
         };
-        int hub = MG_INF;
+        int hub;
+        H::initialize(&hub);
         auto process = [&](const SegW &sg) {
             wave_sync();
 #pragma unroll
@@ -739,8 +760,10 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
                 // instruction updating one LDS address serialise, so its minimum stays in
                 // a register; padding (0x7FFF) and unreached sources relax nothing
                 if (dst == 0x7FFFu || m == (int)MG_INF) continue;
-                if (dst == 0) hub = min(hub, m + wt);
-                else atomicMin(&s_min[dst], m + wt);
+                int cand;
+                H::get_partial_result(&cand, m, wt);
+                if (dst == 0) H::accumulate_nonAtomics(&hub, cand, iter);
+                else H::accumulate(&s_min[dst], cand, iter);
             }
         };
         int sgi = wid;
@@ -771,16 +794,19 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
                 if (sgi >= nseg) break;
             }
         }
+        Acc<H> ident;
+        acc_init(ident);
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) hub = min(hub, __shfl_xor(hub, o));
-        if (lane() == 0 && hub != (int)MG_INF) atomicMin(&s_min[0], hub);
+        for (int o = 32; o > 0; o >>= 1) H::accumulate_nonAtomics(&hub, __shfl_xor(hub, o), iter);
+        if (lane() == 0 && hub != ident.v) H::accumulate(&s_min[0], hub, iter);
         __syncthreads();
         if (un.w) {  // partition split over several units: each lowers what it found
             for (int k = threadIdx.x; k < pv; k += kBG) {
-                const int best = s_min[k];
-                if (best == (int)MG_INF) continue;
+                Acc<H> a = ident;
+                a.v = s_min[k];
+                if (a.v == ident.v) continue;
                 const int v = inv[v0 + k];
-                if (best < dist[v] && atomicMin(&dist[v], best) > best) vfc[v] = iter;
+                acc_apply_atomic<H>(&dist[v], nullptr, &vfc[v], a, iter);
             }
         } else {
             // apply, then write the partition's messages for the next sweep: its
@@ -792,13 +818,10 @@ __global__ __launch_bounds__(kBG) void k_ps_gather(Ctl *ctl, const int4 *__restr
                 cl[j] = MG_INF;
                 if (k < pv) {
                     const int v = inv[v0 + k];
-                    const int best = s_min[k];
+                    Acc<H> a = ident;
+                    a.v = s_min[k];
                     int d = dist[v];
-                    if (best < d) {
-                        dist[v] = best;
-                        vfc[v] = iter;
-                        d = best;
-                    }
+                    if (acc_apply<H>(d, &dist[v], nullptr, &vfc[v], a, iter)) d = a.v;
                     if (sizeof(MT) == 2 && d != (int)MG_INF && d >= 0xFFFF) {  // cannot be sent: see k_ps_scatter
                         vfc[v] = iter;
                         sat = true;
@@ -1007,8 +1030,8 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     // dynamic LDS above the 64 KiB default needs an explicit per-kernel limit
     MG_HIP(hipFuncSetAttribute((const void *)k_pb_count, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds_count, 1024)));
     MG_HIP(hipFuncSetAttribute((const void *)k_pb_place, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds_place, 1024)));
-    MG_HIP(hipFuncSetAttribute((const void *)k_pb_scatter_msg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * P)));
-    MG_HIP(hipFuncSetAttribute((const void *)k_pb_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gather));
+    MG_HIP(hipFuncSetAttribute((const void *)k_pb_scatter_msg<MG_PR_HOOKS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * P)));
+    MG_HIP(hipFuncSetAttribute((const void *)k_pb_gather<MG_PR_HOOKS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_gather));
     // edges per tile -> padded edge slots -> tile offsets of the destination-major stream
     hipLaunchKernelGGL(k_pb_count, dim3(grid_for(nitems, 1, 4096)), dim3(kBT), lds_count, s, icol, irow, d_items, nitems,
                        L.P, np, count);
@@ -1123,10 +1146,10 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
         MG_HIP(hipFuncSetAttribute((const void *)k_ps_scatter<int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_sc));
         MG_HIP(hipFuncSetAttribute((const void *)k_ps_scatter<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_sc));
         const int lds_ps = (int)(sizeof(int) * (P + 1) + sizeof(int) * kStage);
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint8_t, int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint16_t, int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint8_t, uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
-        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<uint16_t, uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<MG_SSSP_HOOKS, uint8_t, int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<MG_SSSP_HOOKS, uint16_t, int>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<MG_SSSP_HOOKS, uint8_t, uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
+        MG_HIP(hipFuncSetAttribute((const void *)k_ps_gather<MG_SSSP_HOOKS, uint16_t, uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_ps));
     }
     MG_HIP(hipMalloc(&L.unitsA, sizeof(int4) * std::max<size_t>(1, ua.size())));
     MG_HIP(hipMalloc(&L.unitsB, sizeof(int4) * std::max<size_t>(1, ub.size())));
@@ -1153,15 +1176,15 @@ int mg_pb_iteration(mg_engine *e) {
     // algorithmic bytes of one iteration (pull formulation): 8E + 16V
     const double alg = 8.0 * (double)e->ne + 16.0 * e->nv;
     if (e->prof) mg_prof_begin(e, "k_pb_gather", alg, &slot);
-    hipLaunchKernelGGL(k_pb_gather, dim3(grid_for(L.nB, 1, G)), dim3(kBG), sizeof(double) * (P + 1) + sizeof(float) * kStage, s,
+    hipLaunchKernelGGL(k_pb_gather<MG_PR_HOOKS>, dim3(grid_for(L.nB, 1, G)), dim3(kBG), sizeof(double) * (P + 1) + sizeof(float) * kStage, s,
                        e->ctl, L.unitsB, L.nB, L.P, e->nv, L.np, variant, e->opt.pr_damping, e->opt.pr_tolerance, L.b_dst,
                        L.gbase, L.msg, L.pairs, L.apart, L.a_src, L.amap, e->deg, e->rank, e->vfc, e->psum);
     if (e->prof) mg_prof_end(e, slot);
     if (L.nApply > 0) {
-        hipLaunchKernelGGL(k_pb_apply_split, dim3(grid_for((long long)L.nApply * P, kBT, G)), dim3(kBT), 0, s, e->ctl,
+        hipLaunchKernelGGL(k_pb_apply_split<MG_PR_HOOKS>, dim3(grid_for((long long)L.nApply * P, kBT, G)), dim3(kBT), 0, s, e->ctl,
                            L.split_parts, L.nApply, L.P, e->nv, L.np, variant, e->opt.pr_damping, e->opt.pr_tolerance, e->deg,
                            e->rank, e->vfc, e->psum);
-        hipLaunchKernelGGL(k_pb_scatter_msg, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(float) * P, s, e->ctl, L.unitsA,
+        hipLaunchKernelGGL(k_pb_scatter_msg<MG_PR_HOOKS>, dim3(grid_for(L.nA, 1, G)), dim3(kBT), sizeof(float) * P, s, e->ctl, L.unitsA,
                            L.nA, L.P, e->nv, L.np, e->rank, e->deg, L.a_src, L.amap, L.msg, L.pairs);
     }
     MG_HIP(hipGetLastError());
@@ -1190,11 +1213,11 @@ int mg_ps_sweep(mg_engine *e) {
                                L.nS, P, e->nv, L.np, L.inv, e->level, e->vfc, L.a_src, L.amap, msg, L.pairs);
         if (e->prof) mg_prof_begin(e, "k_ps_gather", alg, &slot);
         if (L.w8)
-            hipLaunchKernelGGL((k_ps_gather<uint8_t, MT>), dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P,
+            hipLaunchKernelGGL((k_ps_gather<MG_SSSP_HOOKS, uint8_t, MT>), dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P,
                                e->nv, L.np, L.b_dst, (const uint8_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv,
                                e->level, e->vfc);
         else
-            hipLaunchKernelGGL((k_ps_gather<uint16_t, MT>), dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P,
+            hipLaunchKernelGGL((k_ps_gather<MG_SSSP_HOOKS, uint16_t, MT>), dim3(grid_for(L.nB, 1, G)), dim3(kBG), lds, s, e->ctl, L.unitsB, L.nB, P,
                                e->nv, L.np, L.b_dst, (const uint16_t *)L.b_w, L.gbase, msg, L.pairs, L.apart, L.a_src, L.amap, L.inv,
                                e->level, e->vfc);
         if (e->prof) mg_prof_end(e, slot);

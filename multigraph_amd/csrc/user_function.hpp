@@ -10,8 +10,23 @@
 //   fused_update_condition  push-side apply + enqueue predicate   user_function.h:386-447
 // `attribute` is the iteration number and `frontier` the per-vertex stamp (vfc),
 // exactly as in the reference.  The engine's HIP kernels are templates over these
-// structs, so the hooks are inlined into the gfx950 code the way the reference's
-// macros are inlined into its CUDA kernels.
+// structs and call the hooks at the reference's points, so editing a hook changes
+// every path that runs the algorithm:
+//   push levels (k_push, k_push_lb)        get_partial_result + fused_update_condition
+//   pull levels (k_pull, k_pull_huge,      initialize (register / slot identity),
+//     k_pull_sssp, k_ps_gather,            get_partial_result, accumulate_nonAtomics
+//     k_pb_gather, k_pr_gather)            (register), accumulate (LDS / global slot),
+//                                          update_condition (apply), fused_update_
+//                                          condition where two workgroups share a vertex
+//   batched SSSP / BC forward (k_mb_*)     the same hooks per lane (lane = source)
+// Where several lanes' register accumulators are merged, the merge is
+// accumulate_nonAtomics of one accumulator into another, so the hooks must form a
+// commutative, associative combine (all of the reference's do).
+//
+// Which struct serves which algorithm is chosen below by MG_BFS_HOOKS, MG_SSSP_HOOKS,
+// MG_BC_HOOKS, MG_PR_HOOKS and MG_CC_HOOKS; a maintainer's own hooks file, passed at build time as
+// -DMG_USER_FUNCTION='"file.hpp"' (make USER_FUNCTION=file.hpp), is included after the
+// defaults and may derive from them and re-point those names.
 #pragma once
 #include "mg_device.hpp"
 
@@ -21,6 +36,9 @@ namespace mg {
 struct BFSHooks {
     using V1T = int;
     static constexpr bool kV2 = false, kE1 = false;
+    // every message of one level carries the same value, so the first one a pull level
+    // finds for a vertex is final and the rest of its parents need not be scanned
+    static constexpr bool kFirstWins = true;
     __device__ static void initialize(V1T *loc) { *loc = MG_INF; }
     __device__ static void get_partial_result(V1T *loc, V1T data) { *loc = data + 1; }
     __device__ static void accumulate(V1T *loc, V1T data, int attribute) {
@@ -47,6 +65,7 @@ struct SSSPHooks {
     using V1T = int;
     using E1T = int;
     static constexpr bool kV2 = false, kE1 = true;
+    static constexpr bool kFirstWins = false;
     __device__ static void initialize(V1T *loc) { *loc = MG_INF; }
     __device__ static void get_partial_result(V1T *loc, V1T data, E1T weight) { *loc = data + weight; }
     __device__ static void accumulate(V1T *loc, V1T data, int) { atomicMin(loc, data); }
@@ -74,16 +93,23 @@ struct BCHooks {
     using V2T = double;
     using V3T = float;
     static constexpr bool kV2 = true, kE1 = false;
+    static constexpr bool kFirstWins = false;  // every parent's path count is summed
     __device__ static void initialize(V1T *loc, V2T *loc2) { *loc = MG_INF; *loc2 = 0.0; }
     __device__ static void get_partial_result(V1T *loc, V1T data, V2T *loc2, V2T data2) {
         *loc = data + 1; *loc2 = data2;
+    }
+    __device__ static void accumulate(V1T *loc, V1T data, int attribute, V2T *loc2, V2T data2) {
+        if (data <= attribute && data < *loc) atomicMin(loc, data);
+        if (data == attribute && (*loc == MG_INF || *loc == attribute)) atomicAdd(loc2, data2);
     }
     __device__ static void accumulate_nonAtomics(V1T *loc, V1T data, int attribute, V2T *loc2, V2T data2) {
         if (data <= attribute && data < *loc) *loc = data;
         if (data == attribute && (*loc == MG_INF || *loc == attribute)) *loc2 += data2;
     }
-    __device__ static int update_condition(V1T *old_loc, V1T *, V1T data, int, int *, V2T *old_loc2, V2T *, V2T data2) {
-        if (data < *old_loc) { *old_loc = data; *old_loc2 = data2; return 1; }
+    // user_function.h:365-373 stores into old_loc with old_loc == new_loc at its call
+    // site; storing into new_loc lets the engine pass the old value in a register
+    __device__ static int update_condition(V1T *old_loc, V1T *new_loc, V1T data, int, int *, V2T *, V2T *new_loc2, V2T data2) {
+        if (data < *old_loc) { *new_loc = data; *new_loc2 = data2; return 1; }
         return 0;
     }
     // user_function.h:409-417
@@ -95,6 +121,24 @@ struct BCHooks {
             if (atomicCAS(new_loc, (int)MG_INF, attribute) == (int)MG_INF) return 1;
         }
         return 0;
+    }
+    // Backward (dependency) pass.  The reference has no hooks here: process2.cu:135-259
+    // and validate.cu:170-185 inline delta[v] = sum over successors w of
+    // sigma[v] / sigma[w] * (1 + delta[w]).  The same rule as hooks: a finished vertex w
+    // publishes c[w] = (1 + delta[w]) / sigma[w] (back_message), an edge v -> w adds
+    // sigma[v] * c[w] (back_term) into a compensated FP32 sum (back_accumulate: the
+    // term split into two floats, Neumaier's sum), and the vertex applies it
+    // (back_update: delta, then its own c for the level above).
+    __device__ static void back_message(double *c, V3T delta, V2T sigma) { *c = (1.0 + (double)delta) / sigma; }
+    __device__ static void back_term(double *t, V2T sigma_v, double c_w) { *t = sigma_v * c_w; }
+    __device__ static void back_accumulate(KSum *acc, double t) {
+        const float hi = (float)t;
+        acc->add(hi);
+        acc->add((float)(t - (double)hi));
+    }
+    __device__ static void back_update(V3T *delta, double *c, V3T dv, V2T sigma) {
+        *delta = dv;
+        back_message(c, dv, sigma);
     }
 };
 
@@ -158,8 +202,8 @@ struct PRHooks {
     using V2T = int;
     __device__ static void initialize(double *loc) { *loc = 0.0; }
     __device__ static void get_partial_result(float *loc, float rank, int deg) { *loc = deg > 0 ? rank / (float)deg : 0.f; }
-    __device__ static void accumulate(double *loc, float data) { atomicAdd(loc, (double)data); }
-    __device__ static void accumulate_nonAtomics(double *loc, float data) { *loc += (double)data; }
+    __device__ static void accumulate(double *loc, double data) { atomicAdd(loc, data); }
+    __device__ static void accumulate_nonAtomics(double *loc, double data) { *loc += data; }
     // PR_T / PR_D rules of processi.cu:636-654 (variant 0 / 1); returns 1 when the
     // vertex counts as changed.  f1 is computed from the FP64 sum.
     __device__ static int update_condition(float *rank, int *active, double sum, int variant, float damping, float tol,
@@ -179,3 +223,22 @@ struct PRHooks {
 };
 
 }  // namespace mg
+
+#ifdef MG_USER_FUNCTION
+#include MG_USER_FUNCTION
+#endif
+#ifndef MG_BFS_HOOKS
+#define MG_BFS_HOOKS mg::BFSHooks
+#endif
+#ifndef MG_SSSP_HOOKS
+#define MG_SSSP_HOOKS mg::SSSPHooks
+#endif
+#ifndef MG_BC_HOOKS
+#define MG_BC_HOOKS mg::BCHooks
+#endif
+#ifndef MG_PR_HOOKS
+#define MG_PR_HOOKS mg::PRHooks
+#endif
+#ifndef MG_CC_HOOKS
+#define MG_CC_HOOKS mg::CCHooks
+#endif
