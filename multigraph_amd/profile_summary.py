@@ -11,6 +11,12 @@
     python3 multigraph_amd/profile_summary.py traffic --fetch DIR --write DIR \\
         --copy-fetch DIR --copy-write DIR --copy-mib 1024 [--json OUT] [--kernels a,b] [--grid G]
 
+    # HBM bytes per launch from the L2-to-memory request counts, by request size:
+    #   pass A: --pmc TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_BUBBLE_sum TCC_EA0_WRREQ_sum
+    #   pass B: --pmc TCC_EA0_WRREQ_64B_sum TCC_EA0_RDREQ_DRAM_sum TCC_EA0_WRREQ_DRAM_sum
+    python3 multigraph_amd/profile_summary.py requests --a DIR --b DIR [--copy-a DIR --copy-mib 1024]
+        [--json OUT] [--kernels a,b]
+
 Output is markdown on stdout (and JSON with --json).  Only the Python standard
 library is used.
 """
@@ -138,6 +144,67 @@ def traffic(a):
     return res
 
 
+def counters_by_name(d):
+    """{kernel: {counter: [value per dispatch]}} of a multi-counter run dir d"""
+    out = {}
+    for path in _find(d, "counter_collection.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                k = _short(r["Kernel_Name"])
+                c = out.setdefault(k, {}).setdefault(r["Counter_Name"], {})
+                c[r["Dispatch_Id"]] = c.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    return {k: {c: list(v.values()) for c, v in cs.items()} for k, cs in out.items()}
+
+
+def _mean(v):
+    return sum(v) / len(v) if v else 0.0
+
+
+def requests(a):
+    """Bytes from request counts.  Writes: 64 B per TCC_EA0_WRREQ_64B, 32 B per other
+    write request (the WRITE_SIZE rule).  Reads: 32 B per TCC_EA0_RDREQ_32B; the other
+    read requests are 64 B or 128 B, and on gfx950 TCC_BUBBLE (documented as the
+    128-byte requests) reads 0, so the size of each cannot be told apart: both bounds are
+    reported, and `bytes` takes 128 B (an L2 line; a streaming copy of known size
+    issues exactly one such request per 128 B, which the --copy-a run checks)."""
+    A, B = counters_by_name(a.a), counters_by_name(a.b)
+    res = {"kernels": {}}
+    if a.copy_a:
+        C = counters_by_name(a.copy_a)["k_copy"]
+        rd = sorted(C["TCC_EA0_RDREQ_sum"])[len(C["TCC_EA0_RDREQ_sum"]) // 2]
+        r32 = sorted(C["TCC_EA0_RDREQ_32B_sum"])[len(C["TCC_EA0_RDREQ_32B_sum"]) // 2]
+        known = a.copy_mib * 1024 * 1024
+        res["copy_check"] = {"copy_bytes": known, "read_requests": rd, "bytes_per_request": known / max(rd - r32, 1)}
+        print(f"copy check: {a.copy_mib} MiB read by {rd:.0f} read requests ({r32:.0f} of 32 B): "
+              f"{known / max(rd - r32, 1):.1f} B per request; TCC_BUBBLE reads {_mean(C['TCC_BUBBLE_sum']):.0f}\n")
+    want = split_names(a.kernels) if a.kernels else sorted(set(A) | set(B))
+    print("| kernel | dispatches | read req / launch | of 32 B | read MB (64 B .. 128 B) | from DRAM | write MB | "
+          "HBM MB / launch (128 B reads) |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|")
+    for k in want:
+        ca, cb = A.get(k, {}), B.get(k, {})
+        if not ca:
+            continue
+        rd = _mean(ca.get("TCC_EA0_RDREQ_sum", []))
+        r32 = _mean(ca.get("TCC_EA0_RDREQ_32B_sum", []))
+        wr = _mean(ca.get("TCC_EA0_WRREQ_sum", []))
+        w64 = _mean(cb.get("TCC_EA0_WRREQ_64B_sum", [])) if cb else wr
+        rdram = _mean(cb.get("TCC_EA0_RDREQ_DRAM_sum", [])) if cb else rd
+        lo = 32.0 * r32 + 64.0 * (rd - r32)
+        hi = 32.0 * r32 + 128.0 * (rd - r32)
+        wb = 64.0 * w64 + 32.0 * max(wr - w64, 0.0)
+        n = len(ca.get("TCC_EA0_RDREQ_sum", []))
+        res["kernels"][k] = {"dispatches": n, "read_requests": rd, "read_requests_32b": r32, "read_bytes_lo": lo,
+                             "read_bytes_hi": hi, "write_bytes": wb, "dram_read_share": rdram / rd if rd else 0.0,
+                             "bytes": hi + wb}
+        print(f"| {k} | {n} | {rd:.0f} | {r32:.0f} | {lo / 1e6:.1f} .. {hi / 1e6:.1f} | "
+              f"{100 * rdram / rd if rd else 0:.0f} % | {wb / 1e6:.1f} | {(hi + wb) / 1e6:.1f} |")
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(res, f, indent=1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -153,9 +220,20 @@ def main():
     t.add_argument("--kernels", default="")
     t.add_argument("--json", default="")
     t.add_argument("--grid", type=int, default=0, help="only dispatches of this grid size (workload's)")
+    q = sub.add_parser("requests")
+    q.add_argument("--a", required=True, help="pass A run dir")
+    q.add_argument("--b", default="", help="pass B run dir")
+    q.add_argument("--copy-a", dest="copy_a", default="", help="pass A of hbm_copy (check of the 128 B rule)")
+    q.add_argument("--copy-mib", type=int, default=1024)
+    q.add_argument("--kernels", default="")
+    q.add_argument("--json", default="")
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.dir, a.top, a.kernel, a.grid)
+    elif a.cmd == "requests":
+        if not a.b:
+            a.b = a.a
+        requests(a)
     else:
         traffic(a)
 
