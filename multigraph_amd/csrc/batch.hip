@@ -1041,9 +1041,7 @@ __global__ void k_ms_cw(long long n, const int *__restrict__ lv, const uint64_t 
         const int v = lv[i];
         if (row[v + 1] - row[v] <= chunk || !((lw[i] >> lane()) & 1ull)) continue;
         const long long kv = (long long)v * 64 + lane();
-        double c;
-        H::back_message(&c, delta[kv], sigma[kv]);
-        cw[kv] = (float)c;
+        H::back_message(&cw[kv], delta[kv], sigma[kv]);
     }
 }
 
@@ -1091,9 +1089,7 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
             if (k.multi) {
                 atomicAdd(&delta[kv], acc.value());
             } else {  // final: also its c for the level above
-                double c;
-                H::back_update(&delta[kv], &c, acc.value(), sv);
-                cw[kv] = (float)c;
+                H::back_update(&delta[kv], &cw[kv], acc.value(), sv);
             }
         }
     });

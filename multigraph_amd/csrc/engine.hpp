@@ -171,7 +171,7 @@ struct mg_engine {
     float *delta = nullptr;    // BC dependencies
     double *dacc = nullptr;    // BC split-vertex partial sums
     uint8_t *lev8 = nullptr;   // BC backward: levels as bytes
-    double *bc_cw = nullptr;   // BC backward: (1 + delta[w]) / sigma[w] of finished vertices
+    float *bc_cw = nullptr;    // BC backward: (1 + delta[w]) / sigma[w] of finished vertices (FP32)
     int *q = nullptr;          // frontier queue / level log
     long long qcap = 0;
     uint64_t *bm[2] = {nullptr, nullptr};

@@ -301,6 +301,8 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // the in-edges; orow, the out-edge rows of a directed graph (null when symmetric),
 // gives the degree sum.  The new frontier is the bitmap itself; its size and degree sum
 // leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
+constexpr int kPullStep = 8;  // BC pull: edges of a light row loaded together per step
+
 template <class H>
 __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, const int *__restrict__ row,
                                                  const int *__restrict__ col, const int *__restrict__ orow,
@@ -345,23 +347,23 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
             int steps = alone ? e - b : 0;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
-            // four edges per step, their column, bitmap and path-count loads issued
+            // kPullStep edges per step, their column, bitmap and path-count loads issued
             // together (three dependent loads per edge otherwise serialise)
-            for (int k = 0; k < steps; k += 4) {
-                int u[4];
-                bool f[4];
-                double sv[4];
+            for (int k = 0; k < steps; k += kPullStep) {
+                int u[kPullStep];
+                bool f[kPullStep];
+                double sv[kPullStep];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < kPullStep; ++j) {
                     f[j] = alone && b + k + j < e;
                     u[j] = f[j] ? ld_stream(&col[b + k + j]) : 0;
                 }
 #pragma unroll
-                for (int j = 0; j < 4; ++j) f[j] = f[j] && bm_test(fr, u[j]);
+                for (int j = 0; j < kPullStep; ++j) f[j] = f[j] && bm_test(fr, u[j]);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) sv[j] = f[j] ? sigma[u[j]] : 0.0;
+                for (int j = 0; j < kPullStep; ++j) sv[j] = f[j] ? sigma[u[j]] : 0.0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < kPullStep; ++j)
                     if (f[j]) acc_edge<H>(a, plev, sv[j], 0, iter);
             }
             uint64_t heavy = ballot(!alone);
@@ -371,9 +373,23 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
                 const int q0 = __shfl(b, l), q1 = __shfl(e, l);
                 Acc<H> h;
                 acc_init(h);
-                for (int p = q0 + lane(); p < q1; p += MG_WAVE) {
-                    const int u = ld_stream(&col[p]);
-                    if (bm_test(fr, u)) acc_edge<H>(h, plev, sigma[u], 0, iter);
+                for (int p0 = q0; p0 < q1; p0 += 4 * MG_WAVE) {  // four rounds of loads in flight
+                    int u4[4];
+                    bool f4[4];
+                    double s4[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int p = p0 + j * MG_WAVE + lane();
+                        f4[j] = p < q1;
+                        u4[j] = f4[j] ? ld_stream(&col[p]) : 0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) f4[j] = f4[j] && bm_test(fr, u4[j]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) s4[j] = f4[j] ? sigma[u4[j]] : 0.0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (f4[j]) acc_edge<H>(h, plev, s4[j], 0, iter);
                 }
                 h = acc_wave<H>(h, iter);
                 if (lane() == l) a = h;
@@ -704,62 +720,101 @@ __device__ __forceinline__ bool bc_next(int l, int w, const uint8_t *__restrict_
 // stored in FP64 once per vertex when its delta is final, so a successor edge costs one
 // gather and a multiply, no division.
 template <class H>
-__device__ __forceinline__ void bc_term(KSum &acc, double sv, double cw) {
+__device__ __forceinline__ void bc_term(KSum &acc, double sv, float cw) {
     double t;
-    H::back_term(&t, sv, cw);
+    H::back_term(&t, sv, (double)cw);
     H::back_accumulate(&acc, t);
 }
 
+// Tiers: a lane per vertex below 32 edges (here), a wave per vertex below 2048
+// (k_bc_back_wave, over a list this kernel appends to), kHugeChunk-edge chunks above
+// (k_bc_back_huge).  The wave tier runs as its own kernel so that every listed vertex
+// has a wave of its own: walking them from a per-workgroup list left a few thousand
+// waves on the whole GPU for a level of hub neighbours.
 template <class H>
 __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
                                                     const int *__restrict__ row, const int *__restrict__ col,
                                                     const int *__restrict__ level, const uint8_t *__restrict__ lev8,
                                                     const double *__restrict__ sigma,
-                                                    float *__restrict__ delta, double *__restrict__ cw,
-                                                    HugeChunk *__restrict__ chunks, int *__restrict__ nchunks) {
+                                                    float *__restrict__ delta, float *__restrict__ cw,
+                                                    HugeChunk *__restrict__ chunks, int *__restrict__ nchunks,
+                                                    int *__restrict__ wl, int *__restrict__ wn) {
     const int b = lvl_off[l], n = lvl_off[l + 1] - b;
-    __shared__ int s_wv[kBlock];
-    __shared__ int s_wn;
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        if (threadIdx.x == 0) s_wn = 0;
-        __syncthreads();
         const int i = base + threadIdx.x;
+        int v = 0, rb = 0, re = 0;
         if (i < n) {
-            const int v = q[b + i];
-            const int rb = row[v], re = row[v + 1], deg = re - rb;
-            if (deg < 32) {
-                const double sv = sigma[v];
-                KSum acc;
-                for (int p = rb; p < re; ++p) {
-                    const int wv = col[p];
-                    if (bc_next(l, wv, lev8, level)) bc_term<H>(acc, sv, cw[wv]);
-                }
-                H::back_update(&delta[v], &cw[v], acc.value(), sv);
-            } else if (deg < 2048) {
-                s_wv[atomicAdd(&s_wn, 1)] = v;
-            } else {
-                const int nch = (deg + kHugeChunk - 1) / kHugeChunk;
-                const int c0 = atomicAdd(nchunks, nch);
-                for (int j = 0; j < nch; ++j) chunks[c0 + j] = HugeChunk{v, j};
-            }
+            v = q[b + i];
+            rb = row[v];
+            re = row[v + 1];
         }
-        __syncthreads();
-        const int wn = s_wn;
-        for (int k = threadIdx.x >> 6; k < wn; k += kBlock / MG_WAVE) {
-            const int v = s_wv[k];
-            const int rb = row[v], re = row[v + 1];
+        const int deg = re - rb;
+        const bool mid = i < n && deg >= 32 && deg < 2048;
+        const int pos = wave_append(wn, mid);
+        if (mid) wl[pos] = v;
+        if (i < n && deg < 32) {
             const double sv = sigma[v];
             KSum acc;
-            for (int p = rb + lane(); p < re; p += MG_WAVE) {
-                const int wv = ld_stream(&col[p]);
-                if (bc_next(l, wv, lev8, level)) bc_term<H>(acc, sv, cw[wv]);
+            // four edges per step: their column, level and c loads are in flight together
+            for (int p = rb; p < re; p += 4) {
+                int w4[4];
+                bool f[4];
+                float c4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) { f[j] = p + j < re; w4[j] = f[j] ? col[p + j] : 0; }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (f[j]) bc_term<H>(acc, sv, c4[j]);
             }
-            acc = wave_ksum(acc);
-            if (lane() == 0) {
-                H::back_update(&delta[v], &cw[v], acc.value(), sv);
-            }
+            H::back_update(&delta[v], &cw[v], acc.value(), sv);
+        } else if (i < n && deg >= 2048) {
+            const int nch = (deg + kHugeChunk - 1) / kHugeChunk;
+            const int c0 = atomicAdd(nchunks, nch);
+            for (int j = 0; j < nch; ++j) chunks[c0 + j] = HugeChunk{v, j};
         }
-        __syncthreads();
+    }
+}
+
+// a wave per listed vertex (32 to 2047 edges): lanes over edges, four rounds of loads
+// in flight, the lanes' compensated sums merged
+template <class H>
+__global__ __launch_bounds__(kBlock) void k_bc_back_wave(int l, const int *__restrict__ wl, const int *__restrict__ wn,
+                                                         const int *__restrict__ row, const int *__restrict__ col,
+                                                         const int *__restrict__ level, const uint8_t *__restrict__ lev8,
+                                                         const double *__restrict__ sigma, float *__restrict__ delta,
+                                                         float *__restrict__ cw) {
+    const int n = *wn;
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * kBlock) >> 6;
+    for (int k = wave; k < n; k += nwaves) {
+        const int v = wl[k];
+        const int rb = row[v], re = row[v + 1];
+        const double sv = sigma[v];
+        KSum acc;
+        for (int p0 = rb; p0 < re; p0 += 4 * MG_WAVE) {
+            int w4[4];
+            bool f[4];
+            float c4[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int p = p0 + j * MG_WAVE + lane();
+                f[j] = p < re;
+                w4[j] = f[j] ? ld_stream(&col[p]) : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (f[j]) bc_term<H>(acc, sv, c4[j]);
+        }
+        acc = wave_ksum(acc);
+        if (lane() == 0) H::back_update(&delta[v], &cw[v], acc.value(), sv);
     }
 }
 
@@ -767,7 +822,7 @@ template <class H>
 __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__restrict__ row, const int *__restrict__ col,
                                                          const int *__restrict__ level, const uint8_t *__restrict__ lev8,
                                                          const double *__restrict__ sigma,
-                                                         const double *__restrict__ cw, const HugeChunk *__restrict__ chunks,
+                                                         const float *__restrict__ cw, const HugeChunk *__restrict__ chunks,
                                                          const int *__restrict__ nchunks, double *__restrict__ dacc) {
     const int nch = *nchunks;
     __shared__ float s_s[kBlock / MG_WAVE], s_c[kBlock / MG_WAVE];
@@ -777,9 +832,23 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
         const int b = row[v] + hc.j * kHugeChunk, e = min(row[v + 1], b + kHugeChunk);
         const double sv = sigma[v];
         KSum acc;
-        for (int p = b + threadIdx.x; p < e; p += kBlock) {
-            const int wv = ld_stream(&col[p]);
-            if (bc_next(l, wv, lev8, level)) bc_term<H>(acc, sv, cw[wv]);
+        for (int p0 = b; p0 < e; p0 += 4 * kBlock) {
+            int w4[4];
+            bool f[4];
+            float c4[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int p = p0 + j * kBlock + threadIdx.x;
+                f[j] = p < e;
+                w4[j] = f[j] ? ld_stream(&col[p]) : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (f[j]) bc_term<H>(acc, sv, c4[j]);
         }
         acc = wave_ksum(acc);
         if (lane() == 0) { s_s[threadIdx.x >> 6] = acc.s; s_c[threadIdx.x >> 6] = acc.c; }
@@ -795,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
 
 template <class H>
 __global__ void k_bc_back_fin(const HugeChunk *__restrict__ chunks, int *__restrict__ nchunks, double *__restrict__ dacc,
-                              const double *__restrict__ sigma, float *__restrict__ delta, double *__restrict__ cw) {
+                              const double *__restrict__ sigma, float *__restrict__ delta, float *__restrict__ cw) {
     const int nch = *nchunks;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nch; c += gridDim.x * blockDim.x) {
         if (chunks[c].j != 0) continue;  // the first chunk of each vertex finalizes it
@@ -815,7 +884,7 @@ template <class H>
 __global__ __launch_bounds__(kBlock) void k_bc_push(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
                                                     const int *__restrict__ row, const int *__restrict__ col,
                                                     const int *__restrict__ level, const uint8_t *__restrict__ lev8,
-                                                    const double *__restrict__ cw, double *__restrict__ dacc,
+                                                    const float *__restrict__ cw, double *__restrict__ dacc,
                                                     HugeChunk *__restrict__ chunks, int *__restrict__ nchunks) {
     const int b = lvl_off[l + 1], n = lvl_off[l + 2] - b;
     __shared__ int s_wv[kBlock];
@@ -859,7 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_push(int l, const int *__restrict
 
 __global__ __launch_bounds__(kBlock) void k_bc_push_huge(int l, const int *__restrict__ row, const int *__restrict__ col,
                                                          const int *__restrict__ level, const uint8_t *__restrict__ lev8,
-                                                         const double *__restrict__ cw, const HugeChunk *__restrict__ chunks,
+                                                         const float *__restrict__ cw, const HugeChunk *__restrict__ chunks,
                                                          const int *__restrict__ nchunks, double *__restrict__ dacc) {
     const int nch = *nchunks;
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
@@ -877,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_push_huge(int l, const int *__res
 // level l after a push: delta = sigma * sum, its c for the level above, sums cleared
 template <class H>
 __global__ void k_bc_push_fin(int l, const int *__restrict__ lvl_off, const int *__restrict__ q, const double *__restrict__ sigma,
-                              double *__restrict__ dacc, float *__restrict__ delta, double *__restrict__ cw) {
+                              double *__restrict__ dacc, float *__restrict__ delta, float *__restrict__ cw) {
     const int b = lvl_off[l], n = lvl_off[l + 1] - b;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int v = q[b + i];
@@ -1008,7 +1077,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         MG_HIP(hipMalloc(&e->delta, sizeof(float) * padv));
         MG_HIP(hipMalloc(&e->dacc, sizeof(double) * padv));
         MG_HIP(hipMalloc(&e->lev8, padv));
-        MG_HIP(hipMalloc(&e->bc_cw, sizeof(double) * padv));
+        MG_HIP(hipMalloc(&e->bc_cw, sizeof(float) * padv));
         MG_HIP(hipMalloc(&e->lvl_off, sizeof(int) * ((long long)nv + 4)));
         MG_HIP(hipMalloc(&e->lvl_mf, sizeof(long long) * ((long long)nv + 4)));
         hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(padv, 256, 4096)), dim3(256), 0, e->stream, e->dacc, padv, 0.0);
@@ -1297,9 +1366,13 @@ int mg_bc_backward_impl(mg_engine *e) {
             if (e->prof) mg_prof_end(e, slot);
             continue;
         }
+        int *wn = e->aux + 1;  // wave-tier list length (the list reuses the degree-scan buffer)
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, wn);
         if (e->prof) mg_prof_begin(e, "k_bc_back", 0.0, &slot);
         hipLaunchKernelGGL(k_bc_back<MG_BC_HOOKS>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
-                           e->level, e->lev8, e->sigma, e->delta, e->bc_cw, e->chunks, nch);
+                           e->level, e->lev8, e->sigma, e->delta, e->bc_cw, e->chunks, nch, e->eoff, wn);
+        hipLaunchKernelGGL(k_bc_back_wave<MG_BC_HOOKS>, dim3(grid_for(n, kBlock / MG_WAVE, 16384)), dim3(kBlock), 0, s, l, e->eoff, wn,
+                           e->row, e->col, e->level, e->lev8, e->sigma, e->delta, e->bc_cw);
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_bc_back_huge<MG_BC_HOOKS>, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->lev8, e->sigma, e->bc_cw,
                            e->chunks, nch, e->dacc);

@@ -129,14 +129,17 @@ struct BCHooks {
     // sigma[v] * c[w] (back_term) into a compensated FP32 sum (back_accumulate: the
     // term split into two floats, Neumaier's sum), and the vertex applies it
     // (back_update: delta, then its own c for the level above).
-    __device__ static void back_message(double *c, V3T delta, V2T sigma) { *c = (1.0 + (double)delta) / sigma; }
+    // c is stored in FP32 (CT): one gather per successor edge of half the bytes; its
+    // relative rounding (6e-8) bounds each term's, far inside BC's 1e-6 tolerance
+    using CT = float;
+    __device__ static void back_message(CT *c, V3T delta, V2T sigma) { *c = (CT)((1.0 + (double)delta) / sigma); }
     __device__ static void back_term(double *t, V2T sigma_v, double c_w) { *t = sigma_v * c_w; }
     __device__ static void back_accumulate(KSum *acc, double t) {
         const float hi = (float)t;
         acc->add(hi);
         acc->add((float)(t - (double)hi));
     }
-    __device__ static void back_update(V3T *delta, double *c, V3T dv, V2T sigma) {
+    __device__ static void back_update(V3T *delta, CT *c, V3T dv, V2T sigma) {
         *delta = dv;
         back_message(c, dv, sigma);
     }
