@@ -301,7 +301,8 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // the in-edges; orow, the out-edge rows of a directed graph (null when symmetric),
 // gives the degree sum.  The new frontier is the bitmap itself; its size and degree sum
 // leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
-constexpr int kPullStep = 8;  // BC pull: edges of a light row loaded together per step
+constexpr int kPullStep = 8;      // BC pull: edges of a light row loaded together per step
+constexpr int kBackChunk = 1024;  // BC backward: edges per chunk of a vertex above 2047 edges
 
 template <class H>
 __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, const int *__restrict__ row,
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
             if (e - b > cut) e = b;  // left to k_pull_huge
             if (!kSum) {
                 for (int p = b; p < e; ++p)
-                    if (bm_test(fr, ld_stream(&col[p]))) { acc_edge<H>(a, plev, 0.0, 0, iter); break; }
+                    if (bm_test(fr, col[p])) { acc_edge<H>(a, plev, 0.0, 0, iter); break; }
             }
         }
         if (kSum) {
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
 #pragma unroll
                 for (int j = 0; j < kPullStep; ++j) {
                     f[j] = alone && b + k + j < e;
-                    u[j] = f[j] ? ld_stream(&col[b + k + j]) : 0;
+                    u[j] = f[j] ? col[b + k + j] : 0;  // caching load: the row's next step reads the same line
                 }
 #pragma unroll
                 for (int j = 0; j < kPullStep; ++j) f[j] = f[j] && bm_test(fr, u[j]);
@@ -727,7 +728,7 @@ __device__ __forceinline__ void bc_term(KSum &acc, double sv, float cw) {
 }
 
 // Tiers: a lane per vertex below 32 edges (here), a wave per vertex below 2048
-// (k_bc_back_wave, over a list this kernel appends to), kHugeChunk-edge chunks above
+// (k_bc_back_wave, over a list this kernel appends to), kBackChunk-edge chunks above
 // (k_bc_back_huge).  The wave tier runs as its own kernel so that every listed vertex
 // has a wave of its own: walking them from a per-workgroup list left a few thousand
 // waves on the whole GPU for a level of hub neighbours.
@@ -772,7 +773,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
             }
             H::back_update(&delta[v], &cw[v], acc.value(), sv);
         } else if (i < n && deg >= 2048) {
-            const int nch = (deg + kHugeChunk - 1) / kHugeChunk;
+            const int nch = (deg + kBackChunk - 1) / kBackChunk;
             const int c0 = atomicAdd(nchunks, nch);
             for (int j = 0; j < nch; ++j) chunks[c0 + j] = HugeChunk{v, j};
         }
@@ -829,7 +830,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const HugeChunk hc = chunks[c];
         const int v = hc.v;
-        const int b = row[v] + hc.j * kHugeChunk, e = min(row[v + 1], b + kHugeChunk);
+        const int b = row[v] + hc.j * kBackChunk, e = min(row[v + 1], b + kBackChunk);
         const double sv = sigma[v];
         KSum acc;
         for (int p0 = b; p0 < e; p0 += 4 * kBlock) {
@@ -980,7 +981,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMalloc(&e->level, sizeof(int) * padv));
     e->qcap = (e->algo == MG_SSSP) ? 2 * (long long)nv + 2 : (long long)nv + 2;
     MG_HIP(hipMalloc(&e->q, sizeof(int) * e->qcap));
-    e->chunk_cap = 2 * (ne / kHugeChunk) + nv / 1024 + 64;
+    e->chunk_cap = 2 * (ne / kBackChunk) + nv / 1024 + 64;
     MG_HIP(hipMalloc(&e->chunks, sizeof(HugeChunk) * e->chunk_cap));
     MG_HIP(hipMalloc(&e->eoff, sizeof(int) * e->qcap));
     MG_HIP(hipMalloc(&e->tsum, sizeof(int) * (e->qcap / kTile + 2)));
