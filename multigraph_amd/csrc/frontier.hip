@@ -714,12 +714,12 @@ __device__ __forceinline__ bool bc_next(int l, int w, const uint8_t *__restrict_
 // delta[v] = sum over neighbours w one level deeper of sigma[v]/sigma[w] * (1 + delta[w])
 // (process2.cu:5-123, validate.cu:182-189), for the vertices of one level.  Degree
 // tiers: a lane per vertex below 32 edges, a wave per vertex below 2048, and the
-// whole grid (4096-edge chunks, FP64 partials) above.  Sums are Neumaier-compensated
-// FP32 over terms split into two floats, so each term keeps ~48 bits.
+// whole grid (kBackChunk-edge chunks, FP64 partials) above.  Sums are
+// Neumaier-compensated FP32 over terms split into two floats, so each term keeps ~48 bits.
 // The per-edge rule, its compensated sum and the vertex update are BCHooks' backward
 // hooks (back_term, back_accumulate, back_update): c[w] = (1 + delta[w]) / sigma[w] is
-// stored in FP64 once per vertex when its delta is final, so a successor edge costs one
-// gather and a multiply, no division.
+// stored (in the hooks' CT, FP32 by default) once per vertex when its delta is final,
+// so a successor edge costs one 4-byte gather and a multiply, no division.
 template <class H>
 __device__ __forceinline__ void bc_term(KSum &acc, double sv, float cw) {
     double t;
