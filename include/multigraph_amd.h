@@ -95,6 +95,11 @@ typedef struct mg_options {
                                pairs below a distance threshold that rises by this much
                                per level, the others wait (0: a quarter of the mean edge
                                weight; -1: every pair, plain Bellman-Ford)              */
+    int    batch_lanes;     /* batched runs without a slot list: 0, the default, puts the
+                               sources in lanes ordered by their 2-hop reach (sources whose
+                               levels run alike share cache lines of the per-vertex rows;
+                               results come back in the caller's order); 1 keeps source i
+                               in lane i                                                */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */

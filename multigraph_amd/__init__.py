@@ -54,6 +54,7 @@ class Options(C.Structure):
         ("persist_levels", C.c_int),
         ("pull_layout", C.c_int),
         ("sssp_delta", C.c_int),
+        ("batch_lanes", C.c_int),
     ]
 
 

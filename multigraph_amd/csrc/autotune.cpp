@@ -124,5 +124,7 @@ extern "C" int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32
     }
     if (kind == MG_BATCH_SSSP)  // near-far band (0: a quarter of the mean weight; -1: off)
         coords.push_back({[](mg_options &o, double v) { o.sssp_delta = (int)v; }, {-1, 0, 2, 4, 16}});
+    if (!slots)  // lane order of the sources (a slot list fixes the lanes itself)
+        coords.push_back({[](mg_options &o, double v) { o.batch_lanes = (int)v; }, {0, 1}});
     return descend(coords, cost, best, ms_default, ms_tuned);
 }

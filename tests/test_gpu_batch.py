@@ -288,3 +288,30 @@ def test_batch_sssp_near_far(mg, orc, delta, gname, persist):
     for i, s in enumerate(src):
         np.testing.assert_array_equal(d[i], orc.sssp_bf(row, col, w, int(s))[0], err_msg=f"delta {delta} source {i}={s}")
     b.close()
+
+
+@pytest.mark.parametrize("kind", ["bfs", "sssp", "bc"])
+def test_batch_lane_order(mg, orc, kind):
+    """Sources placed in lanes by 2-hop reach (the default) give the rows of lanes in
+    the caller's order: levels, distances and path counts equal, BC sums within FP32
+    rounding, and every row is the oracle's for its source."""
+    g = mg.Graph.rmat(13, 16, seed=11)
+    row, col, w = g.csr()
+    src = _sources(g, 64, 17)
+    src[0] = g.max_degree_vertex()  # the widest reach lands in lane 0, so the order moves it
+    k = {"bfs": mg.BATCH_BFS, "sssp": mg.BATCH_SSSP, "bc": mg.BATCH_BC}[kind]
+    out = []
+    for lanes in (0, 1):
+        b = mg.Batch(g, k, batch_lanes=lanes)
+        b.run(src)
+        out.append((b.levels(), b.sigma() if kind == "bc" else None, b.bc() if kind == "bc" else None))
+        b.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for i in (0, 5, 63):
+        s = int(src[i])
+        ref = orc.bfs(row, col, s) if kind != "sssp" else orc.sssp_dijkstra(row, col, w, s)
+        np.testing.assert_array_equal(out[0][0][i], ref, err_msg=f"source {i}={s}")
+    if kind == "bc":
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+        a, b2 = out[0][2].astype(np.float64), out[1][2].astype(np.float64)
+        assert np.max(np.abs(a - b2) / np.maximum(1.0, np.abs(b2))) < 1e-5
