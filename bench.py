@@ -240,7 +240,8 @@ def uneven(mg, np, orc):
                      "results_equal": same, "slots_match_oracle": ok,
                      "tuned_options": {"alpha": opt.alpha, "beta": opt.beta, "batch_chunk": opt.batch_chunk,
                                        "grid_blocks": opt.grid_blocks, "graph_levels": opt.graph_levels,
-                                       "persist_levels": opt.persist_levels, "sssp_delta": opt.sssp_delta},
+                                       "persist_levels": opt.persist_levels, "sssp_delta": opt.sssp_delta,
+                                       "batch_light": opt.batch_light},
                      "untuned_stats": st_def}
         log(f"uneven {name}: {t_def:.2f} ms untuned, {t_tun:.2f} ms tuned ({tune_s:.1f}s of tuning)")
     return out

@@ -100,6 +100,18 @@ typedef struct mg_options {
                                levels run alike share cache lines of the per-vertex rows;
                                results come back in the caller's order); 1 keeps source i
                                in lane i                                                */
+    /* Bin edges and block shape of the frontier kernels (0: the default in brackets):   */
+    int    lane_edges;      /* BC pull and backward: rows of at most this many edges are
+                               walked by one lane, longer ones by a wave [32]            */
+    int    pull_split;      /* BFS / BC pull: rows above this many in-edges are cut into
+                               2048-edge chunks, each a wave's [2048]                    */
+    int    back_split;      /* BC backward: rows of at least this many edges are cut into
+                               back_chunk-edge chunks, each a workgroup's [2048]         */
+    int    back_chunk;      /* BC backward: edges per chunk [1024]                       */
+    int    block_threads;   /* workgroup size of the pull, edge-balanced push and BC
+                               backward kernels: 256, 512 or 1024 [256]                 */
+    int    batch_light;     /* batched BFS pull: vertices of at most this many in-edges
+                               are walked by one lane, heavier ones by a wave [32]      */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */

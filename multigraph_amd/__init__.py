@@ -55,6 +55,12 @@ class Options(C.Structure):
         ("pull_layout", C.c_int),
         ("sssp_delta", C.c_int),
         ("batch_lanes", C.c_int),
+        ("lane_edges", C.c_int),
+        ("pull_split", C.c_int),
+        ("back_split", C.c_int),
+        ("back_chunk", C.c_int),
+        ("block_threads", C.c_int),
+        ("batch_light", C.c_int),
     ]
 
 

@@ -1,9 +1,12 @@
 // Autotuner: picks mg_options for one graph and algorithm by timing candidates on the
 // graph itself.  The reference re-decides push vs pull every level from wall-clock
-// samples of both (process.cu:1161-1255, pseudo_* kernels); here the decision rule is
-// deterministic and its thresholds, with the grid shape, the levels per captured
-// graph, the PageRank partition size, the batched chunk size and whether batched
-// sparse levels run in a persistent launch, are searched once.
+// samples of both (process.cu:1161-1255, pseudo_* kernels) and fixes its degree tiers
+// and block sizes at compile time (common.h:10-61, process.cu:522-537); here the
+// decision rule is deterministic and its thresholds, the bin edges of the degree tiers
+// (push split, pull split, BC lane / wave / chunk tiers, batched lane / wave split), the
+// workgroup size of the level kernels, the grid shape, the levels per captured graph,
+// the PageRank partition size, the batched chunk size and lane order, and whether
+// batched sparse levels run in a persistent launch, are searched once.
 //
 // Search: coordinate descent from the starting options, one pass over the
 // coordinates in the order listed; each candidate is built, run once to warm up, then
@@ -86,6 +89,18 @@ extern "C" int mg_autotune(const mg_graph *g, int algo, int nsrc, const int32_t 
         coords.push_back({[](mg_options &o, double v) { o.beta = (float)v; }, {6, 12, 18, 36, 72}});
         coords.push_back(grid);
         coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {0, 4, 16}});
+        // bin edges and block shape: push split of high-degree vertices, the workgroup
+        // size of the level kernels, the pull's chunk split and (BC) the lane / wave /
+        // chunk tiers of the pull and the backward pass
+        coords.push_back({[](mg_options &o, double v) { o.huge_degree = (int)v; }, {4096, 8192, 16384, 32768}});
+        coords.push_back({[](mg_options &o, double v) { o.block_threads = (int)v; }, {256, 512, 1024}});
+        if (algo != MG_SSSP)
+            coords.push_back({[](mg_options &o, double v) { o.pull_split = (int)v; }, {1024, 2048, 4096, 8192}});
+        if (algo == MG_BC) {
+            coords.push_back({[](mg_options &o, double v) { o.lane_edges = (int)v; }, {16, 32, 64}});
+            coords.push_back({[](mg_options &o, double v) { o.back_split = (int)v; }, {1024, 2048, 4096}});
+            coords.push_back({[](mg_options &o, double v) { o.back_chunk = (int)v; }, {512, 1024, 2048}});
+        }
     } else if (algo == MG_CC) {
         coords.push_back(grid);
     } else {
@@ -118,6 +133,8 @@ extern "C" int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32
         coords.push_back({[](mg_options &o, double v) { o.beta = (float)v; }, {4, 9, 18, 36, 72}});
     coords.push_back({[](mg_options &o, double v) { o.batch_chunk = (int)v; }, {256, 512, 1024, 4096}});
     coords.push_back({[](mg_options &o, double v) { o.grid_blocks = (int)v; }, {1024, 2048, 4096}});
+    if (kind == MG_BATCH_BFS)  // lane / wave split of the pull's in-edge rows
+        coords.push_back({[](mg_options &o, double v) { o.batch_light = (int)v; }, {16, 32, 64}});
     if (kind != MG_BATCH_BC) {
         coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {2, 8, 32}});
         coords.push_back({[](mg_options &o, double v) { o.persist_levels = (int)v; }, {0, -1}});

@@ -291,13 +291,13 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bfs(const BCtl *ctl, long long 
     });
 }
 
-// pull, BFS, vertices of at most kLightPull in-edges: a lane per vertex walks its row
+// pull, BFS, vertices of at most `light` in-edges (mg_options.batch_light, 32 by
+// default): a lane per vertex walks its row
 // alone (64 consecutive ids per wave, so the row loads of a step are close together)
 // and stops once its missing bits are covered; k_mb_pull_bfs takes the heavier ones'
 // chunks.  A wave per light vertex would leave most lanes idle and pay a cross-lane
 // reduction per 64 edges for a handful of edges.
-constexpr int kLightPull = 32;
-__global__ __launch_bounds__(kBB) void k_mb_pull_bfs_light(const BCtl *ctl, int nv, const int *__restrict__ row,
+__global__ __launch_bounds__(kBB) void k_mb_pull_bfs_light(const BCtl *ctl, int nv, int light, const int *__restrict__ row,
                                                            const int *__restrict__ col, uint64_t all, uint64_t *FA,
                                                            const uint64_t *__restrict__ vis, uint64_t *FB) {
     if (!level_runs(ctl, 0)) return;
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bfs_light(const BCtl *ctl, int 
     if (ctl) all = ctl->all;
     for (int v = blockIdx.x * kBB + threadIdx.x; v < nv; v += gridDim.x * kBB) {
         const int p0 = row[v], p1 = row[v + 1];
-        if (p1 == p0 || p1 - p0 > kLightPull) continue;
+        if (p1 == p0 || p1 - p0 > light) continue;
         const uint64_t need = all & ~vis[v];
         if (!need) continue;
         uint64_t acc = 0;
@@ -1276,10 +1276,11 @@ struct mg_batch {
     const int *in_row() const { return tchunks ? trow : row; }
     const int *in_col() const { return tchunks ? tcol : col; }
     const int *in_w() const { return tchunks ? tw : w; }
-    // BFS pulls: in-edge chunks of the vertices above kLightPull in-edges only (the
+    // BFS pulls: in-edge chunks of the vertices above `light` in-edges only (the
     // lighter ones are walked a lane per vertex)
     Chunk *hchunks = nullptr;
     long long nhch = 0;
+    int light = 32;               // BFS pulls: the lane / wave split (mg_options.batch_light)
     // BFS: new-bit words of the first kDenseLog dense levels ([level][nv + 1]) and their
     // level numbers, expanded into the level rows after the run (k_mb_expand)
     uint64_t *dlog = nullptr;
@@ -1330,6 +1331,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     b->ne = g->ne;
     b->max_sources = max_sources;
     if (opt) b->opt = *opt; else mg_default_options(&b->opt);
+    if (b->opt.batch_light > 0) b->light = std::min(b->opt.batch_light, 1024);
     if (b->opt.grid_blocks > 4096) b->opt.grid_blocks = 4096;  // per-workgroup partial arrays hold 4096
     auto fail = [&](int rc) { batch_free(b); return rc; };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "stream"));
@@ -1370,7 +1372,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     if (kind == MG_BATCH_BFS)
         for (const Chunk &c : (trow.empty() ? ch : tch)) {
             const int32_t *r = trow.empty() ? g->row.data() : trow.data();
-            if (r[c.v + 1] - r[c.v] > kLightPull) hch.push_back(c);
+            if (r[c.v + 1] - r[c.v] > b->light) hch.push_back(c);
         }
     b->nhch = (long long)hch.size();
     if (b->opt.batch_lanes == 0) {  // 2-hop reach of every vertex, for the lane order
@@ -1545,7 +1547,7 @@ static int run_bfs_bc(mg_batch *b) {
         } else {
             if (bc) hipLaunchKernelGGL(k_mb_pull_bc<MG_BC_HOOKS>, dim3(gci), dim3(kBB), 0, s, it, b->in_nch(), b->in_chunks(), all, b->in_col(), F, b->vis, Fn, b->sigma);
             else {
-                hipLaunchKernelGGL(k_mb_pull_bfs_light, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->in_row(), b->in_col(),
+                hipLaunchKernelGGL(k_mb_pull_bfs_light, dim3(gv), dim3(kBB), 0, s, (const BCtl *)nullptr, nv, b->light, b->in_row(), b->in_col(),
                                    all, F, b->vis, Fn);
                 hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gch), dim3(kBB), 0, s, (const BCtl *)nullptr, b->nhch, b->hchunks, all,
                                    b->in_col(), F, b->vis, Fn);
@@ -1638,7 +1640,7 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
         } else {
             hipLaunchKernelGGL((k_mb_push<false>), dim3(gc), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nch, b->chunks, b->col,
                                b->F[0], b->vis, b->F[1], (double *)nullptr);
-            hipLaunchKernelGGL(k_mb_pull_bfs_light, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->in_row(), b->in_col(),
+            hipLaunchKernelGGL(k_mb_pull_bfs_light, dim3(gv), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->light, b->in_row(), b->in_col(),
                                0ull, b->F[0], b->vis, b->F[1]);
             hipLaunchKernelGGL(k_mb_pull_bfs, dim3(gch), dim3(kBB), 0, s, (const BCtl *)b->ctl, b->nhch, b->hchunks, 0ull,
                                b->in_col(), b->F[0], b->vis, b->F[1]);

@@ -1,6 +1,7 @@
 // C ABI of the device engine (include/multigraph_amd.h): build, reset, step, run,
 // results, release, profiling.  Dispatches to the per-algorithm code in
 // frontier.hip (BFS/SSSP/BC) and cc_pr.hip (CC, PageRank).
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include "engine.hpp"
@@ -36,6 +37,16 @@ static void fill_defaults(mg_options *o) {
     if (o->grid_blocks <= 0) o->grid_blocks = d.grid_blocks;
     if (o->pr_damping <= 0) o->pr_damping = d.pr_damping;
     if (o->pr_tolerance <= 0) o->pr_tolerance = d.pr_tolerance;
+    if (o->lane_edges <= 0) o->lane_edges = 32;
+    o->lane_edges = std::min(o->lane_edges, 1024);
+    if (o->pull_split <= 0) o->pull_split = kPullHuge;
+    o->pull_split = std::max(o->pull_split, 64);
+    if (o->back_chunk <= 0) o->back_chunk = 1024;
+    o->back_chunk = std::max(o->back_chunk, 64);
+    if (o->back_split <= 0) o->back_split = 2048;
+    o->back_split = std::max(o->back_split, o->lane_edges + 1);
+    if (o->block_threads <= 0) o->block_threads = kBlock;
+    if (o->batch_light <= 0) o->batch_light = 32;
 }
 
 extern "C" int mg_device_count(int *n) {
@@ -67,6 +78,8 @@ extern "C" int mg_build(const mg_graph *g, int algo, const mg_options *opt, mg_e
     if (!g || !out) return mg_fail(MG_ERR_ARG, "null argument");
     if (algo < MG_BFS || algo > MG_PR_D) return mg_fail(MG_ERR_ARG, "unknown algorithm %d", algo);
     *out = nullptr;
+    if (opt && opt->block_threads != 0 && opt->block_threads != 256 && opt->block_threads != 512 && opt->block_threads != 1024)
+        return mg_fail(MG_ERR_ARG, "block_threads %d: 256, 512 or 1024", opt->block_threads);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return mg_fail(MG_ERR_DEVICE, "no HIP device available");
     mg_engine *e = new mg_engine();

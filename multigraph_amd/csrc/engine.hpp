@@ -77,8 +77,7 @@ struct Params {
 struct HugeChunk { int v, j; };
 // pull chunk of a high-degree vertex: edges [p0, p1) of v, k = v's slot in the huge list
 struct PullChunk { int v, p0, p1, k; };
-constexpr int kPullHuge = 2048;   // pull levels split vertices above this degree
-constexpr int kPullLane = 32;     // BC pull: rows up to this many edges are walked by one lane
+constexpr int kPullHuge = 2048;   // pull levels split vertices above this degree (default of pull_split)
 constexpr int kPullChunk = 2048;  // edges per pull chunk
 
 struct KStat {

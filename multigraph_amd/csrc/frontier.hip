@@ -219,8 +219,8 @@ __global__ void k_lb_part(Ctl *ctl, const int *__restrict__ eoff, const int *__r
 }
 
 // 4. the push: each workgroup takes kEPB consecutive edges of the frontier's edge list
-template <class H>
-__global__ __launch_bounds__(kBlock) void k_push_lb(Ctl *ctl, const int *__restrict__ row, const int *__restrict__ col,
+template <class H, int B>
+__global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const int *__restrict__ row, const int *__restrict__ col,
                                                     const int *__restrict__ wgt, int *__restrict__ state,
                                                     int *__restrict__ vfc, double *__restrict__ sigma,
                                                     int *__restrict__ q, const int *__restrict__ eoff,
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_push_lb(Ctl *ctl, const int *__restr
         const int cnt = i1 - i0 + 1;
         const bool staged = cnt <= kCap;
         if (staged) {
-            for (int k = threadIdx.x; k < cnt; k += kBlock) {
+            for (int k = threadIdx.x; k < cnt; k += B) {
                 const int i = i0 + k;
                 const int v = q[in_begin + i];
                 s_off[k] = lb_off(eoff, toff, i);
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_push_lb(Ctl *ctl, const int *__restr
         }
         __syncthreads();
         const int e0 = c * kEPB, e1 = min(total, e0 + kEPB);
-        for (int eb = e0; eb < e1; eb += kBlock) {
+        for (int eb = e0; eb < e1; eb += B) {
             const int e = eb + threadIdx.x;
             int win = 0, u = 0;
             if (e < e1) {
@@ -302,10 +302,9 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // gives the degree sum.  The new frontier is the bitmap itself; its size and degree sum
 // leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
 constexpr int kPullStep = 8;      // BC pull: edges of a light row loaded together per step
-constexpr int kBackChunk = 1024;  // BC backward: edges per chunk of a vertex above 2047 edges
 
-template <class H>
-__global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, const int *__restrict__ row,
+template <class H, int B>
+__global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_max, const int *__restrict__ row,
                                                  const int *__restrict__ col, const int *__restrict__ orow,
                                                  int *__restrict__ level,
                                                  double *__restrict__ sigma, uint64_t *__restrict__ vis,
@@ -318,8 +317,8 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
     const uint64_t *fr = ctl->bm_cur ? bm1 : bm0;
     uint64_t *nx = ctl->bm_cur ? bm0 : bm1;
     const int words = (nv + 63) >> 6;
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * kBlock) >> 6;
+    const int wave = (blockIdx.x * B + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * B) >> 6;
     int cnt = 0;
     long long mf = 0;
     for (int w = wave; w < words; w += nwaves) {
@@ -340,11 +339,11 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
             }
         }
         if (kSum) {
-            // lanes walk rows of at most kPullLane edges alone, stepping together for the
+            // lanes walk rows of at most lane_max edges alone, stepping together for the
             // longest of them; longer rows are walked by the whole wave, one at a time
             // (lanes over edges, then the lanes' accumulators merged).  Path counts are
             // integers, exact in FP64 whatever the order of the additions.
-            const bool alone = e - b <= kPullLane;
+            const bool alone = e - b <= lane_max;
             int steps = alone ? e - b : 0;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
@@ -407,13 +406,13 @@ __global__ __launch_bounds__(kBlock) void k_pull(Ctl *ctl, int nv, int cut, cons
             mf += orow ? orow[v + 1] - orow[v] : deg;  // out-degree: the next level's push edges
         }
     }
-    __shared__ long long s_tmp[kBlock / MG_WAVE];
-    const long long tn = block_sum<kBlock / MG_WAVE>((long long)cnt, s_tmp);
-    const long long tm = block_sum<kBlock / MG_WAVE>(mf, s_tmp);
+    __shared__ long long s_tmp[B / MG_WAVE];
+    const long long tn = block_sum<B / MG_WAVE>((long long)cnt, s_tmp);
+    const long long tm = block_sum<B / MG_WAVE>(mf, s_tmp);
     if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_mf[blockIdx.x] = tm; }
 }
 
-// Unvisited vertices above kPullHuge edges, cut into static kPullChunk-edge chunks
+// Unvisited vertices above pull_split edges, cut into static kPullChunk-edge chunks
 // (PullChunk {v, p0, p1, k}, k = the vertex's slot): a wave per chunk scans it and
 // merges its accumulator into the vertex's slot (hlev, hsum) with accumulate (FP64
 // path-count sums: exact in any order); k_pull_huge_fin then applies the slots to
@@ -713,8 +712,9 @@ __device__ __forceinline__ bool bc_next(int l, int w, const uint8_t *__restrict_
 
 // delta[v] = sum over neighbours w one level deeper of sigma[v]/sigma[w] * (1 + delta[w])
 // (process2.cu:5-123, validate.cu:182-189), for the vertices of one level.  Degree
-// tiers: a lane per vertex below 32 edges, a wave per vertex below 2048, and the
-// whole grid (kBackChunk-edge chunks, FP64 partials) above.  Sums are
+// tiers (the engine's bin edges, 32 / 2048 / 1024 by default): a lane per vertex of at
+// most lane_max edges, a wave per vertex below split, and the whole grid (chunks of
+// `chunk` edges, FP64 partials) from split up.  Sums are
 // Neumaier-compensated FP32 over terms split into two floats, so each term keeps ~48 bits.
 // The per-edge rule, its compensated sum and the vertex update are BCHooks' backward
 // hooks (back_term, back_accumulate, back_update): c[w] = (1 + delta[w]) / sigma[w] is
@@ -727,21 +727,22 @@ __device__ __forceinline__ void bc_term(KSum &acc, double sv, float cw) {
     H::back_accumulate(&acc, t);
 }
 
-// Tiers: a lane per vertex below 32 edges (here), a wave per vertex below 2048
-// (k_bc_back_wave, over a list this kernel appends to), kBackChunk-edge chunks above
-// (k_bc_back_huge).  The wave tier runs as its own kernel so that every listed vertex
+// Tiers: a lane per vertex of at most lane_max edges (here), a wave per vertex below
+// split (k_bc_back_wave, over a list this kernel appends to), chunks of `chunk` edges
+// from split up (k_bc_back_huge).  The wave tier runs as its own kernel so that every listed vertex
 // has a wave of its own: walking them from a per-workgroup list left a few thousand
 // waves on the whole GPU for a level of hub neighbours.
-template <class H>
-__global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
+template <class H, int B>
+__global__ __launch_bounds__(B) void k_bc_back(int l, const int *__restrict__ lvl_off, const int *__restrict__ q,
                                                     const int *__restrict__ row, const int *__restrict__ col,
                                                     const int *__restrict__ level, const uint8_t *__restrict__ lev8,
                                                     const double *__restrict__ sigma,
                                                     float *__restrict__ delta, float *__restrict__ cw,
                                                     HugeChunk *__restrict__ chunks, int *__restrict__ nchunks,
-                                                    int *__restrict__ wl, int *__restrict__ wn) {
+                                                    int *__restrict__ wl, int *__restrict__ wn, int lane_max, int split,
+                                                    int chunk) {
     const int b = lvl_off[l], n = lvl_off[l + 1] - b;
-    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    for (int base = blockIdx.x * B; base < n; base += gridDim.x * B) {
         const int i = base + threadIdx.x;
         int v = 0, rb = 0, re = 0;
         if (i < n) {
@@ -750,10 +751,10 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
             re = row[v + 1];
         }
         const int deg = re - rb;
-        const bool mid = i < n && deg >= 32 && deg < 2048;
+        const bool mid = i < n && deg > lane_max && deg < split;
         const int pos = wave_append(wn, mid);
         if (mid) wl[pos] = v;
-        if (i < n && deg < 32) {
+        if (i < n && deg <= lane_max) {
             const double sv = sigma[v];
             KSum acc;
             // four edges per step: their column, level and c loads are in flight together
@@ -772,8 +773,8 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
                     if (f[j]) bc_term<H>(acc, sv, c4[j]);
             }
             H::back_update(&delta[v], &cw[v], acc.value(), sv);
-        } else if (i < n && deg >= 2048) {
-            const int nch = (deg + kBackChunk - 1) / kBackChunk;
+        } else if (i < n && deg >= split) {
+            const int nch = (deg + chunk - 1) / chunk;
             const int c0 = atomicAdd(nchunks, nch);
             for (int j = 0; j < nch; ++j) chunks[c0 + j] = HugeChunk{v, j};
         }
@@ -782,15 +783,15 @@ __global__ __launch_bounds__(kBlock) void k_bc_back(int l, const int *__restrict
 
 // a wave per listed vertex (32 to 2047 edges): lanes over edges, four rounds of loads
 // in flight, the lanes' compensated sums merged
-template <class H>
-__global__ __launch_bounds__(kBlock) void k_bc_back_wave(int l, const int *__restrict__ wl, const int *__restrict__ wn,
+template <class H, int B>
+__global__ __launch_bounds__(B) void k_bc_back_wave(int l, const int *__restrict__ wl, const int *__restrict__ wn,
                                                          const int *__restrict__ row, const int *__restrict__ col,
                                                          const int *__restrict__ level, const uint8_t *__restrict__ lev8,
                                                          const double *__restrict__ sigma, float *__restrict__ delta,
                                                          float *__restrict__ cw) {
     const int n = *wn;
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * kBlock) >> 6;
+    const int wave = (blockIdx.x * B + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * B) >> 6;
     for (int k = wave; k < n; k += nwaves) {
         const int v = wl[k];
         const int rb = row[v], re = row[v + 1];
@@ -819,27 +820,27 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_wave(int l, const int *__res
     }
 }
 
-template <class H>
-__global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__restrict__ row, const int *__restrict__ col,
+template <class H, int B>
+__global__ __launch_bounds__(B) void k_bc_back_huge(int l, const int *__restrict__ row, const int *__restrict__ col,
                                                          const int *__restrict__ level, const uint8_t *__restrict__ lev8,
                                                          const double *__restrict__ sigma,
                                                          const float *__restrict__ cw, const HugeChunk *__restrict__ chunks,
-                                                         const int *__restrict__ nchunks, double *__restrict__ dacc) {
+                                                         const int *__restrict__ nchunks, double *__restrict__ dacc, int chunk) {
     const int nch = *nchunks;
-    __shared__ float s_s[kBlock / MG_WAVE], s_c[kBlock / MG_WAVE];
+    __shared__ float s_s[B / MG_WAVE], s_c[B / MG_WAVE];
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const HugeChunk hc = chunks[c];
         const int v = hc.v;
-        const int b = row[v] + hc.j * kBackChunk, e = min(row[v + 1], b + kBackChunk);
+        const int b = row[v] + hc.j * chunk, e = min(row[v + 1], b + chunk);
         const double sv = sigma[v];
         KSum acc;
-        for (int p0 = b; p0 < e; p0 += 4 * kBlock) {
+        for (int p0 = b; p0 < e; p0 += 4 * B) {
             int w4[4];
             bool f[4];
             float c4[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int p = p0 + j * kBlock + threadIdx.x;
+                const int p = p0 + j * B + threadIdx.x;
                 f[j] = p < e;
                 w4[j] = f[j] ? ld_stream(&col[p]) : 0;
             }
@@ -856,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_back_huge(int l, const int *__res
         __syncthreads();
         if (threadIdx.x == 0) {
             KSum t;
-            for (int k = 0; k < kBlock / MG_WAVE; ++k) { t.add(s_s[k]); t.add(s_c[k]); }
+            for (int k = 0; k < B / MG_WAVE; ++k) { t.add(s_s[k]); t.add(s_c[k]); }
             atomicAdd(&dacc[v], (double)t.s + (double)t.c);
         }
         __syncthreads();
@@ -974,6 +975,17 @@ int max_grid(const mg_engine *e) { return e->opt.grid_blocks > 0 ? e->opt.grid_b
 
 // ---------------------------------------------------------------------------- host side
 
+// the launch statements with kB bound to the engine's workgroup size for the pull,
+// edge-balanced push and BC backward kernels (mg_options.block_threads)
+#define MG_WITH_BLOCK(e, ...)                                                   \
+    do {                                                                        \
+        switch ((e)->opt.block_threads) {                                       \
+        case 512: { constexpr int kB = 512; __VA_ARGS__; } break;               \
+        case 1024: { constexpr int kB = 1024; __VA_ARGS__; } break;             \
+        default: { constexpr int kB = kBlock; __VA_ARGS__; } break;             \
+        }                                                                       \
+    } while (0)
+
 int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     const int nv = e->nv;
     const long long ne = e->ne;
@@ -981,7 +993,9 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMalloc(&e->level, sizeof(int) * padv));
     e->qcap = (e->algo == MG_SSSP) ? 2 * (long long)nv + 2 : (long long)nv + 2;
     MG_HIP(hipMalloc(&e->q, sizeof(int) * e->qcap));
-    e->chunk_cap = 2 * (ne / kBackChunk) + nv / 1024 + 64;
+    // chunks of one level: the backward's (ne / back_chunk + ne / back_split at most) or
+    // the backward push's (kHugeChunk-edge chunks of vertices of 2048+ edges)
+    e->chunk_cap = ne / e->opt.back_chunk + ne / e->opt.back_split + 2 * (ne / 2048) + nv / 1024 + 64;
     MG_HIP(hipMalloc(&e->chunks, sizeof(HugeChunk) * e->chunk_cap));
     MG_HIP(hipMalloc(&e->eoff, sizeof(int) * e->qcap));
     MG_HIP(hipMalloc(&e->tsum, sizeof(int) * (e->qcap / kTile + 2)));
@@ -1047,12 +1061,12 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         MG_HIP(hipMalloc(&e->bm[0], sizeof(uint64_t) * words));
         MG_HIP(hipMalloc(&e->bm[1], sizeof(uint64_t) * words));
         MG_HIP(hipMalloc(&e->vis, sizeof(uint64_t) * words));
-        // pull chunks of the vertices above kPullHuge edges (static)
+        // pull chunks of the vertices above pull_split edges (static)
         std::vector<PullChunk> pc;
         std::vector<int> hv;
         for (int v = 0; v < nv; ++v) {
             const int b = irow[v], d = irow[v + 1] - b;
-            if (d <= kPullHuge) continue;
+            if (d <= e->opt.pull_split) continue;
             for (int p = b; p < b + d; p += kPullChunk) pc.push_back(PullChunk{v, p, std::min(b + d, p + kPullChunk), (int)hv.size()});
             hv.push_back(v);
         }
@@ -1179,17 +1193,17 @@ static int launch_level(mg_engine *e, bool known) {
             if (e->prof) mg_prof_end(e, slot);
             const int gp = known ? grid_for(maxch, 1, G) : G;
             if (e->prof) mg_prof_begin(e, "k_push_lb", bytes, &slot);
-            switch (e->algo) {
-            case MG_BFS:
-                hipLaunchKernelGGL((k_push_lb<MG_BFS_HOOKS>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
-                break;
-            case MG_SSSP:
-                hipLaunchKernelGGL((k_push_lb<MG_SSSP_HOOKS>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
-                break;
-            default:
-                hipLaunchKernelGGL((k_push_lb<MG_BC_HOOKS>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
-                break;
-            }
+            MG_WITH_BLOCK(e, switch (e->algo) {
+                case MG_BFS:
+                    hipLaunchKernelGGL((k_push_lb<MG_BFS_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                    break;
+                case MG_SSSP:
+                    hipLaunchKernelGGL((k_push_lb<MG_SSSP_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_push_lb<MG_BC_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                    break;
+            });
             if (e->prof) mg_prof_end(e, slot);
         }
     }
@@ -1216,13 +1230,16 @@ static int launch_level(mg_engine *e, bool known) {
         const double bytes = known ? (8.0 * nv / 64.0 * 3 + 4.0 * (double)c.mu + 8.0 * nv) : 0.0;
         if (e->prof) mg_prof_begin(e, "k_pull", bytes, &slot);
         const int *orow = e->trow ? e->row : nullptr;
-        // rows above kPullHuge edges go to k_pull_huge's chunks; an engine without
+        // rows above pull_split edges go to k_pull_huge's chunks; an engine without
         // chunks (none was that long, or no table was built) walks every row here
-        const int cut = e->pull_ch ? kPullHuge : 0x7FFFFFFF;
-        if (e->algo == MG_BC)
-            hipLaunchKernelGGL((k_pull<MG_BC_HOOKS>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
-        else
-            hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, cut, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+        const int cut = e->pull_ch ? e->opt.pull_split : 0x7FFFFFFF;
+        const int lanes = e->opt.lane_edges;
+        int gpl = gsc;  // k_pull's workgroups = its partials
+        MG_WITH_BLOCK(e, gpl = grid_for(words, kB / MG_WAVE, G);
+            if (e->algo == MG_BC)
+                hipLaunchKernelGGL((k_pull<MG_BC_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            else
+                hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf));
         if (e->pull_nch > 0) {
             const int gh = grid_for(e->pull_nch, kBlock / MG_WAVE, G);
             const int gf = grid_for(e->pull_nh, kBlock, 64);
@@ -1239,7 +1256,7 @@ static int launch_level(mg_engine *e, bool known) {
             }
         }
         if (e->prof) mg_prof_end(e, slot);
-        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gpl);
     }
     const bool mirror = known && e->mirror;
     hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf, mirror ? e->h_ctl : (Ctl *)nullptr,
@@ -1370,13 +1387,16 @@ int mg_bc_backward_impl(mg_engine *e) {
         int *wn = e->aux + 1;  // wave-tier list length (the list reuses the degree-scan buffer)
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, wn);
         if (e->prof) mg_prof_begin(e, "k_bc_back", 0.0, &slot);
-        hipLaunchKernelGGL(k_bc_back<MG_BC_HOOKS>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, s, l, e->lvl_off, e->q, e->row, e->col,
-                           e->level, e->lev8, e->sigma, e->delta, e->bc_cw, e->chunks, nch, e->eoff, wn);
-        hipLaunchKernelGGL(k_bc_back_wave<MG_BC_HOOKS>, dim3(grid_for(n, kBlock / MG_WAVE, 16384)), dim3(kBlock), 0, s, l, e->eoff, wn,
-                           e->row, e->col, e->level, e->lev8, e->sigma, e->delta, e->bc_cw);
+        const int lanes = e->opt.lane_edges, split = e->opt.back_split, chunk = e->opt.back_chunk;
+        MG_WITH_BLOCK(e,
+            hipLaunchKernelGGL((k_bc_back<MG_BC_HOOKS, kB>), dim3(grid_for(n, kB, G)), dim3(kB), 0, s, l, e->lvl_off, e->q, e->row, e->col,
+                               e->level, e->lev8, e->sigma, e->delta, e->bc_cw, e->chunks, nch, e->eoff, wn, lanes, split, chunk);
+            hipLaunchKernelGGL((k_bc_back_wave<MG_BC_HOOKS, kB>), dim3(grid_for(n, kB / MG_WAVE, 16384)), dim3(kB), 0, s, l, e->eoff, wn,
+                               e->row, e->col, e->level, e->lev8, e->sigma, e->delta, e->bc_cw));
         if (e->prof) mg_prof_end(e, slot);
-        hipLaunchKernelGGL(k_bc_back_huge<MG_BC_HOOKS>, dim3(G), dim3(kBlock), 0, s, l, e->row, e->col, e->level, e->lev8, e->sigma, e->bc_cw,
-                           e->chunks, nch, e->dacc);
+        MG_WITH_BLOCK(e,
+            hipLaunchKernelGGL((k_bc_back_huge<MG_BC_HOOKS, kB>), dim3(G), dim3(kB), 0, s, l, e->row, e->col, e->level, e->lev8, e->sigma, e->bc_cw,
+                               e->chunks, nch, e->dacc, chunk));
         hipLaunchKernelGGL(k_bc_back_fin<MG_BC_HOOKS>, dim3(64), dim3(256), 0, s, e->chunks, nch, e->dacc, e->sigma, e->delta, e->bc_cw);
     }
     MG_HIP(hipGetLastError());
@@ -1471,6 +1491,9 @@ extern "C" int mg_bfs_noalloc(int nv, int64_t ne, const int32_t *row, const int3
     e->opt.alpha = 15.0f;
     e->opt.beta = 18.0f;
     e->opt.grid_blocks = kNaGrid;
+    e->opt.lane_edges = 32;
+    e->opt.pull_split = kPullHuge;
+    e->opt.block_threads = kBlock;
     Params &P = e->P;
     P.nv = nv;
     P.ne = ne;

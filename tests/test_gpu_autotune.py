@@ -11,14 +11,28 @@ def test_autotune_engine(mg, orc):
     g = mg.Graph.rmat(13, 16, seed=21)
     row, col, w = g.csr()
     src = [g.max_degree_vertex(), 5, 700]
-    for algo in (mg.BFS, mg.SSSP):
+    for algo in (mg.BFS, mg.SSSP, mg.BC):
         opt, t0, t1 = mg.autotune(g, algo, src, reps=1)
         assert t1 <= t0 + 1e-6
         assert opt.alpha in (4, 8, 15, 30, 60) and opt.beta in (6, 12, 18, 36, 72)
         assert opt.grid_blocks in (1024, 2048, 4096) and opt.graph_levels in (0, 4, 16)
+        # bin edges and block shape (0: the default, kept when no candidate was faster)
+        assert opt.huge_degree in (4096, 8192, 16384, 32768) and opt.block_threads in (0, 256, 512, 1024)
+        if algo != mg.SSSP:
+            assert opt.pull_split in (0, 1024, 2048, 4096, 8192)
+        if algo == mg.BC:
+            assert opt.lane_edges in (0, 16, 32, 64) and opt.back_split in (0, 1024, 2048, 4096)
+            assert opt.back_chunk in (0, 512, 1024, 2048)
+        print(f"algo {algo}: {t0:.3f} -> {t1:.3f} ms, block {opt.block_threads}, huge {opt.huge_degree}, "
+              f"pull_split {opt.pull_split}, lane {opt.lane_edges}, back {opt.back_split}/{opt.back_chunk}", flush=True)
         e = mg.Engine(g, algo, options=opt)
         for s in src:
             e.run(s)
+            if algo == mg.BC:
+                lv, sg, dl = orc.bc_fp64(row, col, s)
+                np.testing.assert_array_equal(e.result(mg.RES_SIGMA), sg)
+                np.testing.assert_allclose(e.result(mg.RES_DELTA).astype(np.float64), dl, rtol=1e-6, atol=1e-6)
+                continue
             ref = orc.bfs(row, col, s) if algo == mg.BFS else orc.sssp_bf(row, col, w, s)[0]
             np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref)
         e.close()
@@ -50,6 +64,8 @@ def test_autotune_batch_union(mg, orc):
     for kind in (mg.BATCH_BFS, mg.BATCH_SSSP):
         opt, t0, t1 = mg.batch_autotune(u, kind, src, slots, reps=1)
         assert t1 <= t0 + 1e-6 and opt.batch_chunk in (0, 256, 512, 1024, 4096)
+        if kind == mg.BATCH_BFS:
+            assert opt.batch_light in (0, 16, 32, 64)
         b = mg.Batch(u, kind, options=opt)
         b.run(src, slots)
         lv = b.levels()

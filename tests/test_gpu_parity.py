@@ -265,3 +265,44 @@ def test_from_csr_engines(mg, orc):
     e.run(0)
     np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.cc_unionfind(row, col)[0])
     e.close()
+
+
+SHAPES = [
+    dict(block_threads=512, lane_edges=4, pull_split=64, back_split=16, back_chunk=64),
+    dict(block_threads=1024, lane_edges=200, pull_split=100000, back_split=100000, back_chunk=4096),
+    dict(block_threads=256, lane_edges=1, pull_split=64, back_split=2, back_chunk=64, huge_degree=4096),
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["b512-fine", "b1024-coarse", "b256-finest"])
+@pytest.mark.parametrize("gname", ["rmat16", "star", "grid64"])
+def test_bin_edges_and_blocks(mg, orc, gname, shape):
+    """Bin edges and workgroup sizes move work between the lane / wave / chunk tiers
+    and change the grids, never the results: BFS and SSSP bit-exact, BC levels and
+    path counts bit-exact, dependencies within the FP32 tolerance."""
+    g = graphs(mg)[gname]
+    row, col, w = g.csr()
+    src = sources(g)[:2]
+    for algo in (mg.BFS, mg.SSSP, mg.BC):
+        for mode in (0, 2):  # hybrid, and every level a pull
+            e = mg.Engine(g, algo, mode=mode, **shape)
+            for s in src:
+                print(f"{gname} algo={algo} mode={mode} {shape} src={s}", flush=True)
+                e.run(s)
+                got = e.result(mg.RES_LEVEL)
+                if algo == mg.BFS:
+                    np.testing.assert_array_equal(got, orc.bfs(row, col, s))
+                elif algo == mg.SSSP:
+                    np.testing.assert_array_equal(got, orc.sssp_bf(row, col, w, s)[0])
+                else:
+                    lv, sg, dl = orc.bc_fp64(row, col, s)
+                    np.testing.assert_array_equal(got, lv)
+                    np.testing.assert_array_equal(e.result(mg.RES_SIGMA), sg)
+                    np.testing.assert_allclose(e.result(mg.RES_DELTA).astype(np.float64), dl, rtol=1e-6, atol=1e-6)
+            e.close()
+
+
+def test_block_threads_rejected(mg):
+    g = graphs(mg)["edge"]
+    with pytest.raises(mg.MGError):
+        mg.Engine(g, mg.BFS, block_threads=300)
