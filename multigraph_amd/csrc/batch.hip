@@ -550,23 +550,6 @@ __global__ void k_mb_ctl(BCtl *ctl, const int *__restrict__ pn, const long long 
     *ctl = c;
 }
 
-// the control kernel of the sparse-only graph: it runs the next level only when that
-// level is sparse and needs neither a compaction nor a full clear; otherwise it flags
-// need_full (state untouched) and the host replays the full graph, whose control
-// kernel decides again from the same partials
-__global__ void k_mb_ctl_sparse(BCtl *ctl, const int *__restrict__ pn, const long long *__restrict__ pe,
-                                const int *__restrict__ px, int nparts, LevelRule r) {
-    if (ctl->done || ctl->need_full) return;
-    long long a, b, x;
-    sum_partials(pn, pe, px, nparts, &a, &b, &x);
-    if (threadIdx.x != 0) return;
-    BCtl c = *ctl;
-    decide(c, a, b, x, r);
-    if (!c.done && (!c.sparse || c.compact || c.fill)) { ctl->need_full = 1; return; }
-    if (!c.done) ++c.nsparse;
-    *ctl = c;
-}
-
 // the host's pinned copy of the control block after a replay: the block, then (after a
 // system-scope fence) the check's sequence number, on which the host spins
 __global__ void k_mb_mirror(const BCtl *ctl, BCtl *mirror, int seq) {
@@ -672,6 +655,35 @@ __device__ __forceinline__ void queue_add(bool add, int w, const int *__restrict
     }
 }
 
+// U appends per lane at once (lane l's j-th entry when add[j], of degree dg[j], loaded
+// by the caller alongside its atomics): one counter atomic for the wave
+template <int U>
+__device__ __forceinline__ void queue_add_n(const bool *add, const int *w, const int *dgs, int *Qn, int *qn_next, int &cnt,
+                                            long long &mf, int &mx) {
+    uint64_t bm[U];
+    int tot = 0;
+#pragma unroll
+    for (int j = 0; j < U; ++j) { bm[j] = ballot(add[j]); tot += __builtin_popcountll(bm[j]); }
+    if (tot == 0) return;
+    int base = 0;
+    if (lane() == 0) base = atomicAdd(qn_next, tot);
+    base = readlane(base, 0);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        if (add[j]) {
+            Qn[base + mbcnt(bm[j])] = w[j];
+            const int dg = dgs[j];
+            ++cnt;
+            mf += dg;
+            mx = max(mx, dg);
+        }
+        base += __builtin_popcountll(bm[j]);
+    }
+}
+
+constexpr int kSparseLevels = 32;  // levels per replay of the sparse-only graph (at least)
+constexpr int kVisit = 4;  // edges a lane of a sparse BFS level visits together
+
 __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict__ row, const int *__restrict__ col, uint64_t *F,
                                            uint64_t *Fn, uint64_t *vis, const int *Q, int *Qn, int *qn_next,
                                            int *__restrict__ lev, int &cnt, long long &mf, int &mx) {
@@ -686,11 +698,28 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
         int steps = alone ? p1 - p0 : 0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
-        for (int k = 0; k < steps; ++k) {
-            const bool mine = alone && p0 + k < p1;
-            const int w = mine ? col[p0 + k] : 0;
-            const bool add = mine && bfs_visit(it, w, fu, Fn, vis, lev);
-            queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
+        // kVisit edges per step, each stage's loads / atomics issued for all of them
+        // before any result is used (one dependent chain per step, not per edge): the
+        // claim on the visited word goes out with the neighbours' degree loads (no
+        // load of the word first: an atomicOr of bits already set claims nothing)
+        for (int k0 = 0; k0 < steps; k0 += kVisit) {
+            int w[kVisit], dg[kVisit];
+            uint64_t m[kVisit];
+            bool add[kVisit];
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) w[j] = alone && p0 + k0 + j < p1 ? col[p0 + k0 + j] : -1;
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) {
+                m[j] = w[j] >= 0 ? fu & ~(uint64_t)atomicOr((unsigned long long *)&vis[w[j]], (unsigned long long)fu) : 0ull;
+                dg[j] = w[j] >= 0 ? row[w[j] + 1] - row[w[j]] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j)
+                add[j] = m[j] && atomicOr((unsigned long long *)&Fn[w[j]], (unsigned long long)m[j]) == 0ull;
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j)
+                for (uint64_t bits = m[j]; bits; bits &= bits - 1) lev[(long long)w[j] * 64 + __builtin_ctzll(bits)] = it;
+            queue_add_n<kVisit>(add, w, dg, Qn, qn_next, cnt, mf, mx);
         }
         uint64_t heavy = ballot(u >= 0 && !alone);
         while (heavy) {  // the whole wave over one heavier vertex's edges
@@ -743,31 +772,48 @@ __device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *_
             }
             rem &= ~far;
         }
-        int steps = __builtin_popcountll(rem) * (p1 - p0);
+        // one source (bit) at a time, its edges kVisit at a time: the column, weight and
+        // distance loads, the relaxations and the changed-word atomics of a step are
+        // issued together (the neighbours' degrees load with the distances)
+        const int nchunk = (p1 - p0 + kVisit - 1) / kVisit;
+        int steps = __builtin_popcountll(rem) * nchunk;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
         int p = p0, du = 0;
         if (rem) du = dist[(long long)u * 64 + __builtin_ctzll(rem)];
         for (int k = 0; k < steps; ++k) {
-            bool add = false;
-            int w = 0;
+            const int sl = rem ? __builtin_ctzll(rem) : 0;
+            int w[kVisit], wt[kVisit], cur[kVisit], nd[kVisit], dg[kVisit];
+            bool imp[kVisit], add[kVisit];
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) {
+                const bool ok = rem && p + j < p1;
+                w[j] = ok ? col[p + j] : -1;
+                wt[j] = ok ? wgt[p + j] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) {
+                cur[j] = w[j] >= 0 ? dist[(long long)w[j] * 64 + sl] : 0;
+                dg[j] = w[j] >= 0 ? row[w[j] + 1] - row[w[j]] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) {
+                H::get_partial_result(&nd[j], du, wt[j]);
+                imp[j] = w[j] >= 0 && nd[j] < cur[j];
+                if (imp[j]) H::accumulate(&dist[(long long)w[j] * 64 + sl], nd[j], it);
+            }
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j)
+                add[j] = imp[j] && atomicOr((unsigned long long *)&Cn[w[j]], 1ull << sl) == 0ull;
+            queue_add_n<kVisit>(add, w, dg, Qn, qn_next, cnt, mf, mx);
             if (rem) {
-                const int sl = __builtin_ctzll(rem);
-                w = col[p];
-                int nd;
-                H::get_partial_result(&nd, du, wgt[p]);
-                int *dw = &dist[(long long)w * 64 + sl];
-                if (nd < *dw) {
-                    H::accumulate(dw, nd, it);
-                    add = atomicOr((unsigned long long *)&Cn[w], 1ull << sl) == 0ull;
-                }
-                if (++p == p1) {
+                p += kVisit;
+                if (p >= p1) {
                     p = p0;
                     rem &= rem - 1;
                     if (rem) du = dist[(long long)u * 64 + __builtin_ctzll(rem)];
                 }
             }
-            queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
         }
         {  // far pairs wait in the next changed word
             const bool add = far && atomicOr((unsigned long long *)&Cn[u], (unsigned long long)far) == 0ull;
@@ -848,6 +894,83 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__
                 dist, cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
     zero_rest(nparts, part_n, part_e, part_x);
+}
+
+// Sparse levels one graph node each (the sparse-only graph).  A level kernel takes the
+// totals of the frontier the previous level produced (count = its queue length, degree
+// sum, largest degree) from a triple-buffered FStat, which the previous kernel's
+// workgroups filled with atomics, and applies the level rule itself, every workgroup
+// alike; the control block is double-buffered (node k reads copy k & 1, workgroup 0
+// writes the decided block to copy (k + 1) & 1 and clears the stats slot node k + 1
+// will fill).  So no control kernel runs between levels: the kernel boundary alone
+// orders one level's atomics before the next level's reads.  A level that cannot run
+// sparse (or an empty frontier) passes block and stats on unchanged; the exit kernel
+// then hands both back to the full graph, whose control kernel decides again.
+struct FStat { int cnt; int mx; long long mf; };
+
+__global__ void k_mbs_fused_enter(const BCtl *ctl, BCtl *c2, FStat *st, const int *__restrict__ pn,
+                                  const long long *__restrict__ pe, const int *__restrict__ px, int nparts) {
+    long long a, b, x;
+    sum_partials(pn, pe, px, nparts, &a, &b, &x);
+    if (threadIdx.x != 0) return;
+    c2[0] = *ctl;
+    st[0] = FStat{(int)a, (int)x, b};
+    st[1] = FStat{0, 0, 0};
+    st[2] = FStat{0, 0, 0};
+}
+
+template <bool kSSSP>
+__global__ __launch_bounds__(kBB) void k_mbs_fused(int k, BCtl *c2, FStat *st, LevelRule r, const int *__restrict__ row,
+                                                   const int *__restrict__ col, const int *__restrict__ wgt, uint64_t *FA,
+                                                   uint64_t *FB, uint64_t *vis, int *QA, int *QB, int *__restrict__ out) {
+    const BCtl c0 = c2[k & 1];
+    const FStat s0 = st[k % 3];
+    BCtl c = c0;
+    bool run = !c.done && !c.need_full;
+    if (run) {
+        decide(c, s0.cnt, s0.mf, s0.mx, r);
+        if (c.done) run = false;
+        else if (!c.sparse || c.compact || c.fill) { c = c0; c.need_full = 1; run = false; }
+        else ++c.nsparse;
+    }
+    if (!run) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { c2[(k + 1) & 1] = c; st[(k + 1) % 3] = s0; }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { c2[(k + 1) & 1] = c; st[(k + 2) % 3] = FStat{0, 0, 0}; }
+    FStat *nx = &st[(k + 1) % 3];
+    const int it = c.it, cur = (it - 1) & 1;
+    int cnt = 0, mx = 0;
+    long long mf = 0;
+    if (kSSSP)
+        sparse_sssp<MG_SSSP_HOOKS>(it, s0.cnt, c.thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB,
+                                   &nx->cnt, out, cnt, mf, mx);
+    else
+        sparse_bfs(it, s0.cnt, row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, cur ? QA : QB, &nx->cnt, out, cnt, mf, mx);
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    const long long tm = block_sum<kBB / MG_WAVE>(mf, s_tmp);
+    const long long tx = block_max<kBB / MG_WAVE>((long long)mx, s_tmp);
+    if (threadIdx.x == 0) {
+        if (tm) atomicAdd((unsigned long long *)&nx->mf, (unsigned long long)tm);
+        if (tx) atomicMax(&nx->mx, (int)tx);
+    }
+}
+
+// after K fused nodes: the control block (with the produced frontier's queue length)
+// and the frontier's totals as partials, as a sparse level of the full graph leaves them
+__global__ void k_mbs_fused_exit(int K, BCtl *ctl, const BCtl *c2, const FStat *st, int *__restrict__ pn,
+                                 long long *__restrict__ pe, int *__restrict__ px, int nparts) {
+    const FStat t = st[K % 3];
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        pn[i] = i ? 0 : t.cnt;
+        pe[i] = i ? 0 : t.mf;
+        px[i] = i ? 0 : t.mx;
+    }
+    if (threadIdx.x == 0) {
+        BCtl c = c2[K & 1];
+        c.qn[c.it & 1] = t.cnt;
+        *ctl = c;
+    }
 }
 
 // Persistent sparse levels: one cooperative launch (every workgroup resident) runs
@@ -1292,8 +1415,11 @@ struct mg_batch {
     long long *part_e = nullptr, *tot = nullptr, *h_tot = nullptr;
     int nsrc = 0, levels = 0;     // slots of the last run, levels it swept
     BCtl *ctl = nullptr, *h_ctl = nullptr;  // device level control (BFS, SSSP)
+    BCtl *ctl2 = nullptr;                   // the sparse-only graph's two control copies
+    FStat *fst = nullptr;                   // and its three frontier-total slots
     hipGraphExec_t gexec = nullptr;         // K captured levels (all level kinds)
-    hipGraphExec_t gsparse = nullptr;       // K captured sparse levels (control + fused kernel)
+    hipGraphExec_t gsparse = nullptr;       // K captured sparse levels (one node each)
+    hipGraphExec_t gsparse_long = nullptr;  // the same with kSparseLevels levels, for long tails
     int gexec_levels = 0;
     PCtl *pc = nullptr;           // counters of the persistent sparse levels
     int persist_grid = 0;         // workgroups of its cooperative launch (0: not available)
@@ -1308,9 +1434,10 @@ static void batch_free(mg_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->gsparse) (void)hipGraphExecDestroy(b->gsparse);
+    if (b->gsparse_long) (void)hipGraphExecDestroy(b->gsparse_long);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
                     b->delta, b->bc, b->chunks, b->logv, b->logw, b->cw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc,
-                    b->tchunks, b->trow, b->tcol, b->tw, b->hchunks, b->dlog, b->dlev, b->perm, b->mv};
+                    b->tchunks, b->trow, b->tcol, b->tw, b->hchunks, b->dlog, b->dlev, b->perm, b->mv, b->ctl2, b->fst};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1395,6 +1522,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
               hipMalloc(&b->tot, sizeof(long long) * 4) == hipSuccess && hipMalloc(&b->perm, sizeof(int) * 64) == hipSuccess &&
               hipHostMalloc(&b->h_tot, sizeof(long long) * 4, hipHostMallocDefault) == hipSuccess &&
               hipMalloc(&b->ctl, sizeof(BCtl)) == hipSuccess && hipMalloc(&b->pc, sizeof(PCtl)) == hipSuccess &&
+              hipMalloc(&b->ctl2, 2 * sizeof(BCtl)) == hipSuccess && hipMalloc(&b->fst, 3 * sizeof(FStat)) == hipSuccess &&
               hipMalloc(&b->Q[0], sizeof(int) * (nv + 1)) == hipSuccess && hipMalloc(&b->Q[1], sizeof(int) * (nv + 1)) == hipSuccess &&
               hipHostMalloc(&b->h_ctl, sizeof(BCtl), hipHostMallocCoherent) == hipSuccess;  // written by k_mb_mirror
     if (ok && kind != MG_BATCH_SSSP) ok = hipMalloc(&b->lev, sizeof(int) * slots) == hipSuccess;
@@ -1617,12 +1745,21 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
             hipLaunchKernelGGL(k_mbs_level_bfs, dim3(gsp), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->F[0], b->F[1], b->vis, b->Q[0],
                                b->Q[1], b->lev, b->part_n, b->part_e, b->part_x, gv);
     };
-    for (int k = 0; k < K; ++k) {
-        if (sparse_only) {  // control + one fused kernel per level
-            hipLaunchKernelGGL(k_mb_ctl_sparse, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, b->part_x, gv, rule);
-            sparse_level();
-            continue;
+    if (sparse_only) {  // one node per level, between an entry and an exit node
+        hipLaunchKernelGGL(k_mbs_fused_enter, dim3(1), dim3(256), 0, s, (const BCtl *)b->ctl, b->ctl2, b->fst, b->part_n, b->part_e,
+                           b->part_x, gv);
+        for (int k = 0; k < K; ++k) {
+            if (sssp)
+                hipLaunchKernelGGL(k_mbs_fused<true>, dim3(gsp), dim3(kBB), 0, s, k, b->ctl2, b->fst, rule, b->row, b->col, b->w,
+                                   b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1], b->dist);
+            else
+                hipLaunchKernelGGL(k_mbs_fused<false>, dim3(gsp), dim3(kBB), 0, s, k, b->ctl2, b->fst, rule, b->row, b->col,
+                                   (const int *)nullptr, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1], b->lev);
         }
+        hipLaunchKernelGGL(k_mbs_fused_exit, dim3(1), dim3(256), 0, s, K, b->ctl, (const BCtl *)b->ctl2, (const FStat *)b->fst,
+                           b->part_n, b->part_e, b->part_x, gv);
+    }
+    for (int k = 0; k < K && !sparse_only; ++k) {
         hipLaunchKernelGGL(k_mb_ctl, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, b->part_x, gv, rule);
         hipLaunchKernelGGL(k_mbs_qcount, dim3(gq), dim3(kBB), 0, s, (const BCtl *)b->ctl, nv, b->F[0], b->F[1], b->part_n);
         hipLaunchKernelGGL(k_mbs_qscan, dim3(1), dim3(1024), 0, s, b->ctl, b->part_n, gq);
@@ -1668,9 +1805,11 @@ static int run_levels(mg_batch *b) {
     if (!b->gexec || b->gexec_levels != K) {  // captured once per batch, on an idle stream
         if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
         if (b->gsparse) { hipGraphExecDestroy(b->gsparse); b->gsparse = nullptr; }
+        if (b->gsparse_long) { hipGraphExecDestroy(b->gsparse_long); b->gsparse_long = nullptr; }
         MG_HIP(hipStreamSynchronize(s));
         MG_TRY(capture_levels(b, K, false, &b->gexec));
         MG_TRY(capture_levels(b, K, true, &b->gsparse));
+        if (K < kSparseLevels) MG_TRY(capture_levels(b, kSparseLevels, true, &b->gsparse_long));
         b->gexec_levels = K;
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, b->vis, (long long)nv + 1, 0ull);
@@ -1688,6 +1827,7 @@ static int run_levels(mg_batch *b) {
     memset(b->h_ctl, 0, sizeof(BCtl));  // the first replay is always the full graph
     const int max_levels = b->opt.persist_levels > 0 ? b->opt.persist_levels : (1 << 30);  // -1: unbounded
     long long trips = 0;
+    bool was_sparse = false;  // the last replay was the sparse-only graph and ran on
     for (;;) {
         // after a sparse level the persistent kernel (or the sparse-only graph) runs on
         // until a level needs more
@@ -1700,8 +1840,12 @@ static int run_levels(mg_batch *b) {
             const void *fn = sssp ? (const void *)k_mbs_persist<true> : (const void *)k_mbs_persist<false>;
             MG_HIP(hipLaunchCooperativeKernel(fn, dim3(b->persist_grid), dim3(kBB), args, 0, s));
         } else {
-            MG_HIP(hipGraphLaunch(sparse_next ? b->gsparse : b->gexec, s));
+            // a tail that outlasted one short sparse replay gets the long one: fewer
+            // host checks; a short tail does not pay the long graph's idle nodes
+            hipGraphExec_t ge = !sparse_next ? b->gexec : (was_sparse && b->gsparse_long) ? b->gsparse_long : b->gsparse;
+            MG_HIP(hipGraphLaunch(ge, s));
         }
+        was_sparse = sparse_next;
         // the control block through the pinned mirror: spin on its sequence number
         // (after ~2 ms, block on the stream, which also reports a failed kernel)
         const int seq = ++b->mirror_seq;
