@@ -112,6 +112,9 @@ typedef struct mg_options {
                                backward kernels: 256, 512 or 1024 [256]                 */
     int    batch_light;     /* batched BFS pull: vertices of at most this many in-edges
                                are walked by one lane, heavier ones by a wave [32]      */
+    int    wide_offsets;    /* 1: the engine keeps 64-bit row offsets on the device even
+                               when the graph's fit int32 (a wide graph always has them;
+                               BFS and SSSP run on such engines, symmetric graphs)      */
 } mg_options;
 
 /* ---- errors ---------------------------------------------------------------- */
@@ -151,8 +154,14 @@ int mg_graph_info(const mg_graph *g, int32_t *nv, int64_t *ne, int32_t *start_p)
  * pull levels.  No reference counterpart (its -DNSYM path is unfinished,
  * read_input.cu:42-44). */
 int mg_graph_symmetric(const mg_graph *g, int *symmetric);
+/* The CSR's arrays (the reference's csc_v / csc_e / csc_ev).  Row offsets are int32
+ * as csc_v is while the graph has at most 2^31 - 1 edges; past that (a "wide" graph,
+ * scale 26-27 RMAT) the row request fails with MG_ERR_LIMIT and mg_graph_csr64 copies
+ * the 64-bit offsets (nv + 1 of them) into `row`.  The reference has no such graphs
+ * (int csc_v, read_input.cu:135-171). */
 int mg_graph_csr(const mg_graph *g, const int32_t **row, const int32_t **col,
                  const int32_t **weight);
+int mg_graph_csr64(const mg_graph *g, int64_t *row);
 /* Original (pre-compaction) id of every compacted vertex, length nv. */
 int mg_graph_orig_ids(const mg_graph *g, const int32_t **orig);
 /* Map an original vertex id to the compacted id exactly as read_input.cu:130 does. */

@@ -61,6 +61,7 @@ class Options(C.Structure):
         ("back_chunk", C.c_int),
         ("block_threads", C.c_int),
         ("batch_light", C.c_int),
+        ("wide_offsets", C.c_int),
     ]
 
 
@@ -82,6 +83,7 @@ _SIGS = {
     "mg_graph_info": (C.c_int, [_P, _I32P, C.POINTER(C.c_int64), _I32P]),
     "mg_graph_symmetric": (C.c_int, [_P, _I32P]),
     "mg_graph_csr": (C.c_int, [_P, C.POINTER(_I32P), C.POINTER(_I32P), C.POINTER(_I32P)]),
+    "mg_graph_csr64": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "mg_graph_orig_ids": (C.c_int, [_P, C.POINTER(_I32P)]),
     "mg_graph_remap": (C.c_int, [_P, C.c_int, _I32P]),
     "mg_graph_max_degree_vertex": (C.c_int, [_P, _I32P]),
@@ -246,14 +248,23 @@ class Graph:
     def start_p(self) -> int:
         return self.info()[2]
 
-    def csr(self):
-        """(row, col, weight) as numpy copies."""
+    def csr(self, copy: bool = True):
+        """(row, col, weight) as numpy arrays: copies, or (copy=False) views of the graph's
+        own col / weight arrays, valid while the graph lives.  Row offsets are int32, or
+        int64 on a graph past 2^31 - 1 edges."""
         nv, ne, _ = self.info()
         r, c, w = _I32P(), _I32P(), _I32P()
-        _check(lib().mg_graph_csr(self._h, C.byref(r), C.byref(c), C.byref(w)))
-        row = np.ctypeslib.as_array(r, shape=(nv + 1,)).copy()
-        col = np.ctypeslib.as_array(c, shape=(ne,)).copy() if ne else np.zeros(0, np.int32)
-        wt = np.ctypeslib.as_array(w, shape=(ne,)).copy() if ne else np.zeros(0, np.int32)
+        if ne > 2**31 - 1:
+            row = np.empty(nv + 1, np.int64)
+            _check(lib().mg_graph_csr64(self._h, row.ctypes.data_as(C.POINTER(C.c_int64))))
+            _check(lib().mg_graph_csr(self._h, None, C.byref(c), C.byref(w)))
+        else:
+            _check(lib().mg_graph_csr(self._h, C.byref(r), C.byref(c), C.byref(w)))
+            row = np.ctypeslib.as_array(r, shape=(nv + 1,)).copy()
+        col = np.ctypeslib.as_array(c, shape=(ne,)) if ne else np.zeros(0, np.int32)
+        wt = np.ctypeslib.as_array(w, shape=(ne,)) if ne else np.zeros(0, np.int32)
+        if copy:
+            col, wt = col.copy(), wt.copy()
         return row, col, wt
 
     def orig_ids(self) -> np.ndarray:

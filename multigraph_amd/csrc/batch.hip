@@ -1450,6 +1450,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     *out = nullptr;
     if (kind < MG_BATCH_BFS || kind > MG_BATCH_SSSP) return mg_fail(MG_ERR_ARG, "unknown batch kind %d", kind);
     if (max_sources < 1 || max_sources > 64) return mg_fail(MG_ERR_LIMIT, "a batch holds 1..64 sources (asked %d)", max_sources);
+    if (g->wide()) return mg_fail(MG_ERR_LIMIT, "batched runs need int32 row offsets (%lld edges)", (long long)g->ne);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return mg_fail(MG_ERR_DEVICE, "no HIP device available");
     mg_batch *b = new mg_batch();

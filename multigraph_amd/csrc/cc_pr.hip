@@ -423,7 +423,7 @@ int mg_cc_build(mg_engine *e, const mg_graph *g) {
     const int nch = (int)((ne_u + kEPB - 1) / kEPB);
     e->chunk_cap = nch;
     MG_HIP(hipMalloc(&e->cstart, sizeof(int) * (nch + 2)));
-    if (nch > 0) hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(nch + 1, 256, 4096)), dim3(256), 0, s, nv, ne_u, e->urow, nch, e->cstart);
+    if (nch > 0) hipLaunchKernelGGL(k_chunk_part<int>, dim3(grid_for(nch + 1, 256, 4096)), dim3(256), 0, s, nv, ne_u, e->urow, nch, e->cstart);
     MG_HIP(hipStreamSynchronize(s));
     MG_HIP(hipGetLastError());
     hipFree(cnt);

@@ -62,7 +62,7 @@ static void release_all(mg_engine *e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) hipGraphExecDestroy(e->gexec);
-    void *ptrs[] = {e->pull_ch, e->pull_hv, e->pull_found, e->pull_sum, e->part_n, e->part_mf, e->eoff, e->tsum, e->cstart, e->pcstart, e->row, e->col, e->w, e->trow, e->tcol, e->tw, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->lev8, e->bc_cw, e->q, e->bm[0], e->bm[1],
+    void *ptrs[] = {e->pull_ch, e->pull_hv, e->pull_found, e->pull_sum, e->part_n, e->part_mf, e->eoff, e->tsum, e->cstart, e->pcstart, e->row, e->row64, e->col, e->w, e->trow, e->tcol, e->tw, e->level, e->vfc, e->sigma, e->delta, e->dacc, e->lev8, e->bc_cw, e->q, e->bm[0], e->bm[1],
                     e->vis, e->chunks, e->lvl_off, e->lvl_mf, e->aux, e->ctl, e->urow, e->ucol, e->rank, e->cval, e->deg, e->psum,
                     e->pb.a_src, e->pb.b_dst, e->pb.amap, e->pb.msg, e->pb.gbase, e->pb.unitsA,
                     e->pb.unitsB, e->pb.split_parts, e->pb.apart, e->pb.b_w, e->pb.inv, e->pb.unitsS};
@@ -92,11 +92,31 @@ extern "C" int mg_build(const mg_graph *g, int algo, const mg_options *opt, mg_e
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "hipStreamCreate failed"));
     const long long nv1 = (long long)g->nv + 1;
     const long long ne = g->ne > 0 ? g->ne : 1;
-    if (hipMalloc(&e->row, sizeof(int) * nv1) != hipSuccess || hipMalloc(&e->col, sizeof(int) * ne) != hipSuccess)
-        return fail(mg_fail(MG_ERR_NOMEM, "device allocation of the CSR failed"));
-    if (hipMemcpy(e->row, g->row.data(), sizeof(int) * nv1, hipMemcpyHostToDevice) != hipSuccess ||
-        (g->ne > 0 && hipMemcpy(e->col, g->col.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess))
-        return fail(mg_fail(MG_ERR_DEVICE, "CSR upload failed"));
+    e->wide = g->wide() || e->opt.wide_offsets == 1;
+    if (e->wide) {
+        // 64-bit row offsets: the BFS / SSSP frontier kernels have them (symmetric
+        // graphs: pull levels read a row as in-edges); every push level's edges must
+        // stay below 2^31 (the edge-balanced scan counts them in int), so pushes stop
+        // at E / alpha with alpha >= 4 (E <= 2^32) and a push-only mode is refused
+        if (algo != MG_BFS && algo != MG_SSSP)
+            return fail(mg_fail(MG_ERR_LIMIT, "64-bit row offsets (%lld edges): BFS and SSSP only", (long long)g->ne));
+        if (!g->symmetric) return fail(mg_fail(MG_ERR_LIMIT, "64-bit row offsets need a symmetric graph"));
+        if (e->opt.mode == MG_MODE_SPARSE || e->opt.alpha < 4.0f)
+            return fail(mg_fail(MG_ERR_LIMIT, "64-bit row offsets: push-only levels, or alpha below 4, are not supported"));
+        std::vector<long long> r((size_t)nv1);
+        for (long long v = 0; v < nv1; ++v) r[(size_t)v] = g->off(v);
+        if (hipMalloc(&e->row64, sizeof(long long) * nv1) != hipSuccess || hipMalloc(&e->col, sizeof(int) * ne) != hipSuccess)
+            return fail(mg_fail(MG_ERR_NOMEM, "device allocation of the CSR failed"));
+        if (hipMemcpy(e->row64, r.data(), sizeof(long long) * nv1, hipMemcpyHostToDevice) != hipSuccess ||
+            (g->ne > 0 && hipMemcpy(e->col, g->col.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(mg_fail(MG_ERR_DEVICE, "CSR upload failed"));
+    } else {
+        if (hipMalloc(&e->row, sizeof(int) * nv1) != hipSuccess || hipMalloc(&e->col, sizeof(int) * ne) != hipSuccess)
+            return fail(mg_fail(MG_ERR_NOMEM, "device allocation of the CSR failed"));
+        if (hipMemcpy(e->row, g->row.data(), sizeof(int) * nv1, hipMemcpyHostToDevice) != hipSuccess ||
+            (g->ne > 0 && hipMemcpy(e->col, g->col.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(mg_fail(MG_ERR_DEVICE, "CSR upload failed"));
+    }
     if (algo == MG_SSSP) {
         if (hipMalloc(&e->w, sizeof(int) * ne) != hipSuccess) return fail(mg_fail(MG_ERR_NOMEM, "weight allocation failed"));
         if (g->ne > 0 && hipMemcpy(e->w, g->w.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess)

@@ -76,7 +76,10 @@ struct Params {
 
 struct HugeChunk { int v, j; };
 // pull chunk of a high-degree vertex: edges [p0, p1) of v, k = v's slot in the huge list
-struct PullChunk { int v, p0, p1, k; };
+// (OT: the type of edge offsets, int, or long long for graphs past 2^31 - 1 edges)
+template <class OT>
+struct PullChunkT { int v; OT p0, p1; int k; };
+using PullChunk = PullChunkT<int>;
 constexpr int kPullHuge = 2048;   // pull levels split vertices above this degree (default of pull_split)
 constexpr int kPullChunk = 2048;  // edges per pull chunk
 
@@ -157,6 +160,10 @@ struct mg_engine {
     hipStream_t stream = nullptr;
     // CSR on the device (the renumbered CSR of HYB: _ncsc_v/_ncsc_e/_ncsc_ev)
     int *row = nullptr, *col = nullptr, *w = nullptr;
+    // 64-bit row offsets instead of row on a graph past 2^31 - 1 edges (or with
+    // mg_options.wide_offsets): BFS and SSSP on symmetric graphs
+    long long *row64 = nullptr;
+    bool wide = false;
     // in-edges of a directed graph (the transposed CSR, rows sorted by source, weights
     // carried along) for pull levels; null on a symmetric graph, whose rows are both
     int *trow = nullptr, *tcol = nullptr, *tw = nullptr;
@@ -174,7 +181,7 @@ struct mg_engine {
     int *q = nullptr;          // frontier queue / level log
     long long qcap = 0;
     uint64_t *bm[2] = {nullptr, nullptr};
-    mg::PullChunk *pull_ch = nullptr;   // pull chunks of high-degree vertices (BFS, BC)
+    void *pull_ch = nullptr;   // pull chunks of high-degree vertices (BFS, BC): PullChunkT<int> or <long long>
     int *pull_hv = nullptr, *pull_found = nullptr;
     double *pull_sum = nullptr;
     int pull_nch = 0, pull_nh = 0;

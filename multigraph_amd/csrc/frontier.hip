@@ -74,8 +74,8 @@ __global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf, Ctl *
     }
 }
 
-template <bool kTrackDeg>
-__device__ __forceinline__ void append(Ctl *ctl, int *q, int out_begin, const int *row, bool win, int u) {
+template <bool kTrackDeg, class OT>
+__device__ __forceinline__ void append(Ctl *ctl, int *q, int out_begin, const OT *row, bool win, int u) {
     int pos = wave_append(&ctl->out_n, win);
     if (win) q[out_begin + pos] = u;
     if (kTrackDeg) {
@@ -88,8 +88,8 @@ __device__ __forceinline__ void append(Ctl *ctl, int *q, int out_begin, const in
 }
 
 // Push level: expand the frontier's edges, contract winners into the next queue.
-template <class H, bool kTrackDeg>
-__global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const int *__restrict__ row,
+template <class H, bool kTrackDeg, class OT = int>
+__global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const OT *__restrict__ row,
                                                  const int *__restrict__ col, const int *__restrict__ wgt,
                                                  int *__restrict__ state, int *__restrict__ vfc,
                                                  double *__restrict__ sigma, int *__restrict__ q,
@@ -97,18 +97,19 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
     if (ctl->mode != MODE_TD || ctl->lb) return;
     const int iter = ctl->iter, in_begin = ctl->in_begin, n = ctl->in_n, out_begin = ctl->out_begin;
     __shared__ int s_off[kBlock];
-    __shared__ int s_rb[kBlock];
+    __shared__ OT s_rb[kBlock];
     __shared__ int s_val[kBlock];
     __shared__ double s_val2[kBlock];
     __shared__ int s_tmp[kBlock / MG_WAVE + 1];
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const int i = base + threadIdx.x;
-        int v = -1, rb = 0, deg = 0, val = 0;
+        int v = -1, deg = 0, val = 0;
+        OT rb = 0;
         double val2 = 0.0;
         if (i < n) {
             v = q[in_begin + i];
             rb = row[v];
-            deg = row[v + 1] - rb;
+            deg = (int)(row[v + 1] - rb);
             val = state[v];
             if (sigma) val2 = sigma[v];
         }
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
                     int mid = (lo + hi + 1) >> 1;
                     if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
                 }
-                const int p = s_rb[lo] + (e - s_off[lo]);
+                const OT p = s_rb[lo] + (e - s_off[lo]);
                 u = ld_stream(&col[p]);
                 win = relax<H>(s_val[lo], s_val2[lo], u, p, wgt, state, vfc, sigma, iter);
             }
@@ -140,7 +141,8 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
 
 // ---- edge-balanced push (frontiers above kSmallMF edges) --------------------------
 // 1. degrees of the frontier, exclusive scan inside tiles of kTile entries
-__global__ __launch_bounds__(kBlock) void k_lb_deg(Ctl *ctl, const int *__restrict__ row, const int *__restrict__ q,
+template <class OT = int>
+__global__ __launch_bounds__(kBlock) void k_lb_deg(Ctl *ctl, const OT *__restrict__ row, const int *__restrict__ q,
                                                    int *__restrict__ eoff, int *__restrict__ tsum) {
     if (ctl->mode != MODE_TD || !ctl->lb) return;
     const int n = ctl->in_n, in_begin = ctl->in_begin;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_lb_deg(Ctl *ctl, const int *__restri
         for (int k = 0; k < kPer; ++k) {
             const int i = i0 + k;
             d[k] = 0;
-            if (i < n) { const int v = q[in_begin + i]; d[k] = row[v + 1] - row[v]; }
+            if (i < n) { const int v = q[in_begin + i]; d[k] = (int)(row[v + 1] - row[v]); }
             local += d[k];
         }
         int total;
@@ -219,8 +221,8 @@ __global__ void k_lb_part(Ctl *ctl, const int *__restrict__ eoff, const int *__r
 }
 
 // 4. the push: each workgroup takes kEPB consecutive edges of the frontier's edge list
-template <class H, int B>
-__global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const int *__restrict__ row, const int *__restrict__ col,
+template <class H, int B, class OT = int>
+__global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const OT *__restrict__ row, const int *__restrict__ col,
                                                     const int *__restrict__ wgt, int *__restrict__ state,
                                                     int *__restrict__ vfc, double *__restrict__ sigma,
                                                     int *__restrict__ q, const int *__restrict__ eoff,
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const int *__restrict__
     const int nch = ctl->lb_chunks, total = ctl->lb_total;
     constexpr int kCap = kEPB + 1;
     __shared__ int s_off[kCap];
-    __shared__ int s_rb[kCap];
+    __shared__ OT s_rb[kCap];
     __shared__ int s_v[kCap];
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const int i0 = cstart[c];
@@ -252,7 +254,8 @@ __global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const int *__restrict__
             const int e = eb + threadIdx.x;
             int win = 0, u = 0;
             if (e < e1) {
-                int k, off, rb, v;
+                int k, off, v;
+                OT rb;
                 if (staged) {
                     int lo = 0, hi = cnt - 1;
                     while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_off[mid] <= e) lo = mid; else hi = mid - 1; }
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const int *__restrict__
                     const int i = lb_owner(eoff, toff, n, e);
                     v = q[in_begin + i]; off = lb_off(eoff, toff, i); rb = row[v];
                 }
-                const int p = rb + (e - off);
+                const OT p = rb + (e - off);
                 u = ld_stream(&col[p]);
                 win = relax<H>(state[v], sigma ? sigma[v] : 0.0, u, p, wgt, state, vfc, sigma, iter);
             }
@@ -303,9 +306,9 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
 constexpr int kPullStep = 8;      // BC pull: edges of a light row loaded together per step
 
-template <class H, int B>
-__global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_max, const int *__restrict__ row,
-                                                 const int *__restrict__ col, const int *__restrict__ orow,
+template <class H, int B, class OT = int>
+__global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_max, const OT *__restrict__ row,
+                                                 const int *__restrict__ col, const OT *__restrict__ orow,
                                                  int *__restrict__ level,
                                                  double *__restrict__ sigma, uint64_t *__restrict__ vis,
                                                  uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
@@ -326,15 +329,16 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
         const int v = (w << 6) + lane();
         Acc<H> a;
         acc_init(a);
-        int b = 0, e = 0, deg = 0;
+        OT b = 0, e = 0;
+        int deg = 0;
         const bool open = vw != ~0ull && !((vw >> lane()) & 1ull);
         if (open) {
             b = row[v];
             e = row[v + 1];
-            deg = e - b;
+            deg = (int)(e - b);
             if (e - b > cut) e = b;  // left to k_pull_huge
             if (!kSum) {
-                for (int p = b; p < e; ++p)
+                for (OT p = b; p < e; ++p)
                     if (bm_test(fr, col[p])) { acc_edge<H>(a, plev, 0.0, 0, iter); break; }
             }
         }
@@ -344,7 +348,7 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
             // (lanes over edges, then the lanes' accumulators merged).  Path counts are
             // integers, exact in FP64 whatever the order of the additions.
             const bool alone = e - b <= lane_max;
-            int steps = alone ? e - b : 0;
+            int steps = alone ? (int)(e - b) : 0;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
             // kPullStep edges per step, their column, bitmap and path-count loads issued
@@ -370,16 +374,16 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
             while (heavy) {
                 const int l = __builtin_ctzll(heavy);
                 heavy &= heavy - 1;
-                const int q0 = __shfl(b, l), q1 = __shfl(e, l);
+                const OT q0 = __shfl(b, l), q1 = __shfl(e, l);
                 Acc<H> h;
                 acc_init(h);
-                for (int p0 = q0; p0 < q1; p0 += 4 * MG_WAVE) {  // four rounds of loads in flight
+                for (OT p0 = q0; p0 < q1; p0 += 4 * MG_WAVE) {  // four rounds of loads in flight
                     int u4[4];
                     bool f4[4];
                     double s4[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const int p = p0 + j * MG_WAVE + lane();
+                        const OT p = p0 + j * MG_WAVE + lane();
                         f4[j] = p < q1;
                         u4[j] = f4[j] ? ld_stream(&col[p]) : 0;
                     }
@@ -418,8 +422,8 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
 // path-count sums: exact in any order); k_pull_huge_fin then applies the slots to
 // level, sigma, bitmaps and counts and resets them to the identity.  Without this
 // split, one lane walks a 160 K-edge hub alone (BC scans every parent).
-template <class H>
-__global__ __launch_bounds__(kBlock) void k_pull_huge(Ctl *ctl, int nch, const PullChunk *__restrict__ ch,
+template <class H, class OT = int>
+__global__ __launch_bounds__(kBlock) void k_pull_huge(Ctl *ctl, int nch, const PullChunkT<OT> *__restrict__ ch,
                                                       const int *__restrict__ col, const double *__restrict__ sigma,
                                                       const uint64_t *__restrict__ vis, const uint64_t *__restrict__ bm0,
                                                       const uint64_t *__restrict__ bm1, int *__restrict__ hlev,
@@ -430,13 +434,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_huge(Ctl *ctl, int nch, const P
     const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * kBlock) >> 6;
     for (int c = wave; c < nch; c += nwaves) {
-        const PullChunk k = ch[c];
+        const PullChunkT<OT> k = ch[c];
         if (bm_test(vis, k.v)) continue;
         Acc<H> a;
         acc_init(a);
         bool found = false;
-        for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
-            const int p = p0 + lane();
+        for (OT p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+            const OT p = p0 + lane();
             bool f = false;
             if (p < k.p1) {
                 const int u = ld_stream(&col[p]);
@@ -453,9 +457,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_huge(Ctl *ctl, int nch, const P
     }
 }
 
-template <class H>
+template <class H, class OT = int>
 __global__ __launch_bounds__(kBlock) void k_pull_huge_fin(Ctl *ctl, int nh, const int *__restrict__ hv,
-                                                          const int *__restrict__ row, int *__restrict__ level,
+                                                          const OT *__restrict__ row, int *__restrict__ level,
                                                           double *__restrict__ sigma, uint64_t *__restrict__ vis,
                                                           uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
                                                           int *__restrict__ hlev, double *__restrict__ hsum,
@@ -588,23 +592,23 @@ constexpr int kPE = kEPB / kBlock;  // consecutive edges per lane (8)
 // chunk is applied with update_condition, one that straddles a chunk boundary (another
 // workgroup may update it too) with fused_update_condition.  Unreached sources
 // (distance MG_INF) send nothing.
-template <class H>
-__global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long long ne, const int *__restrict__ row,
+template <class H, class OT = int>
+__global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long long ne, const OT *__restrict__ row,
                                                       const int *__restrict__ col, const int *__restrict__ wgt,
                                                       const int *__restrict__ cstart, int *__restrict__ dist,
                                                       int *__restrict__ vfc) {
     if (ctl->mode != MODE_BU) return;
     const int iter = ctl->iter;
-    __shared__ int s_row[kEPB + 2];
+    __shared__ OT s_row[kEPB + 2];
     __shared__ int s_best[kEPB + 1];
     Acc<H> ident;
     acc_init(ident);
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
-        const int e0 = c * kEPB;
-        const int e1 = (int)min((long long)e0 + kEPB, ne);
+        const OT e0 = (OT)c * kEPB;
+        const OT e1 = (OT)min((long long)e0 + kEPB, ne);
         const int u0 = cstart[c], u1 = cstart[c + 1];
         const int nown = u1 - u0 + 1;
-        const int eb = e0 + threadIdx.x * kPE;
+        const OT eb = e0 + threadIdx.x * kPE;
         int cu[kPE], wv[kPE], du[kPE];
         if (eb + kPE <= e1) {
             const uint4 c0 = ld_stream_u4(&col[eb]), c1 = ld_stream_u4(&col[eb + 4]);
@@ -622,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
         for (int j = 0; j < kPE; ++j) du[j] = eb + j < e1 ? dist[cu[j]] : (int)MG_INF;
         if (nown > kEPB + 1) {  // empty rows inside the chunk (not after compaction): owners by search in global row
             for (int j = 0; j < kPE; ++j) {
-                const int e = eb + j;
+                const OT e = eb + j;
                 if (e >= e1 || du[j] == (int)MG_INF) continue;
                 int lo = u0, hi = u1;
                 while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (row[mid] <= e) lo = mid; else hi = mid - 1; }
@@ -638,11 +642,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
         if (eb < e1) {
             int lo = 0, hi = nown - 1;  // owner of edge eb: the last k with s_row[k] <= eb
             while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_row[mid] <= eb) lo = mid; else hi = mid - 1; }
-            int k = lo, next = s_row[lo + 1];
+            int k = lo;
+            OT next = s_row[lo + 1];
             Acc<H> a = ident;
 #pragma unroll
             for (int j = 0; j < kPE; ++j) {
-                const int e = eb + j;
+                const OT e = eb + j;
                 if (e < e1) {
                     while (e >= next) {
                         if (a.v != ident.v) acc_atomic<H>(&s_best[k], nullptr, a, iter);
@@ -671,8 +676,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
 // Frontier of stamp `iter + offset` from the stamps: a bitmap word per wave
 // (ballot) and per-workgroup partial counts (size, degree sum) for k_reduce_parts.
 // gate 0: always; 1: only on a pull level; 2: only right after a switch to pull
+template <class OT = int>
 __global__ __launch_bounds__(kBlock) void k_sssp_scan(Ctl *ctl, int nv, int offset, int gate, const int *__restrict__ vfc,
-                                                      const int *__restrict__ row, uint64_t *__restrict__ bm,
+                                                      const OT *__restrict__ row, uint64_t *__restrict__ bm,
                                                       int *__restrict__ part_n, long long *__restrict__ part_mf) {
     if (gate >= 1 && ctl->mode != MODE_BU) return;
     if (gate == 2 && !ctl->convert) return;
@@ -986,6 +992,24 @@ int max_grid(const mg_engine *e) { return e->opt.grid_blocks > 0 ? e->opt.grid_b
         }                                                                       \
     } while (0)
 
+// the launch statements with OT bound to the engine's row-offset type and rowp / irowp
+// to its out- / in-edge rows: int, or long long on a graph past 2^31 - 1 edges (or
+// with mg_options.wide_offsets), which is symmetric, so its rows serve both ways
+#define MG_WITH_OFFSETS(e, ...)                                                               \
+    do {                                                                                      \
+        if ((e)->wide) {                                                                      \
+            using OT = long long;                                                             \
+            const OT *rowp = (e)->row64, *irowp = (e)->row64;                                 \
+            (void)rowp; (void)irowp;                                                          \
+            __VA_ARGS__;                                                                      \
+        } else {                                                                              \
+            using OT = int;                                                                   \
+            const OT *rowp = (e)->row, *irowp = (e)->in_row();                                \
+            (void)rowp; (void)irowp;                                                          \
+            __VA_ARGS__;                                                                      \
+        }                                                                                     \
+    } while (0)
+
 int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     const int nv = e->nv;
     const long long ne = e->ne;
@@ -1009,7 +1033,9 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     // a directed graph's pull levels read in-edges: transpose the CSR once (counting
     // sort by destination; scanning sources in order leaves every row sorted by source)
     std::vector<int32_t> hrow_t, hcol_t;
-    const int32_t *irow = g->row.data();
+    // host row offsets of the pull's in-edges, as 64-bit values (a wide graph is
+    // symmetric: its rows serve both ways)
+    auto irow = [&](int v) -> long long { return hrow_t.empty() ? g->off(v) : (long long)hrow_t[(size_t)v]; };
     if (!g->symmetric && nv > 0) {
         hrow_t.assign((size_t)nv + 1, 0);
         hcol_t.resize((size_t)ne);
@@ -1031,7 +1057,6 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
             MG_HIP(hipMalloc(&e->tw, sizeof(int) * ne));
             MG_HIP(hipMemcpy(e->tw, hw_t.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
         }
-        irow = hrow_t.data();
     }
     if (e->algo == MG_SSSP) {
         MG_HIP(hipMalloc(&e->vfc, sizeof(int) * padv));
@@ -1040,11 +1065,11 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         e->pnch = (int)((ne + kEPB - 1) / kEPB);
         MG_HIP(hipMalloc(&e->pcstart, sizeof(int) * (e->pnch + 2)));
         if (e->pnch > 0)
-            hipLaunchKernelGGL(k_chunk_part, dim3(grid_for(e->pnch + 1, 256, 4096)), dim3(256), 0, e->stream, nv, ne, e->in_row(),
-                               e->pnch, e->pcstart);
+            MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_chunk_part<OT>), dim3(grid_for(e->pnch + 1, 256, 4096)), dim3(256), 0, e->stream,
+                                                  nv, ne, irowp, e->pnch, e->pcstart));
         // dense sweeps over the partitioned layout (symmetric graphs, weights in 16 bits),
-        // else over the CSR
-        bool w16 = g->symmetric && e->opt.pull_layout != 1 && nv >= 2;
+        // else over the CSR (always on an engine with 64-bit offsets)
+        bool w16 = g->symmetric && e->opt.pull_layout != 1 && nv >= 2 && !e->wide;
         int wmax = 0;
         for (long long p = 0; w16 && p < ne; ++p) {
             w16 = g->w[p] >= 0 && g->w[p] <= 0xFFFF;
@@ -1052,32 +1077,48 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
         }
         if (w16) {
             e->pb.w8 = wmax <= 0xFF;
-            MG_TRY(mg_pb_build(e, g));
-            e->sssp_pb = true;
-            e->pb16_ok = e->opt.pull_layout == 0;  // 2: 32-bit messages throughout
+            // a layout past 2^31 - 1 edge slots (padded tiles of a graph near int32
+            // offsets, e.g. RMAT-26) is refused by its builder: the sweeps then run over
+            // the CSR, as for weights above 16 bits
+            const int rc = mg_pb_build(e, g);
+            if (rc == MG_OK) {
+                e->sssp_pb = true;
+                e->pb16_ok = e->opt.pull_layout == 0;  // 2: 32-bit messages throughout
+            } else if (rc != MG_ERR_LIMIT) {
+                return rc;
+            }
         }
     } else {
         const long long words = (nv + 63) / 64 + 1;
         MG_HIP(hipMalloc(&e->bm[0], sizeof(uint64_t) * words));
         MG_HIP(hipMalloc(&e->bm[1], sizeof(uint64_t) * words));
         MG_HIP(hipMalloc(&e->vis, sizeof(uint64_t) * words));
-        // pull chunks of the vertices above pull_split edges (static)
-        std::vector<PullChunk> pc;
+        // pull chunks of the vertices above pull_split edges (static), with the engine's
+        // offset type
+        std::vector<PullChunkT<long long>> pc;
         std::vector<int> hv;
         for (int v = 0; v < nv; ++v) {
-            const int b = irow[v], d = irow[v + 1] - b;
+            const long long b = irow(v), d = irow(v + 1) - b;
             if (d <= e->opt.pull_split) continue;
-            for (int p = b; p < b + d; p += kPullChunk) pc.push_back(PullChunk{v, p, std::min(b + d, p + kPullChunk), (int)hv.size()});
+            for (long long p = b; p < b + d; p += kPullChunk)
+                pc.push_back(PullChunkT<long long>{v, p, std::min(b + d, p + kPullChunk), (int)hv.size()});
             hv.push_back(v);
         }
         e->pull_nch = (int)pc.size();
         e->pull_nh = (int)hv.size();
         if (!pc.empty()) {
-            MG_HIP(hipMalloc(&e->pull_ch, sizeof(PullChunk) * pc.size()));
             MG_HIP(hipMalloc(&e->pull_hv, sizeof(int) * hv.size()));
             MG_HIP(hipMalloc(&e->pull_found, sizeof(int) * hv.size()));
             MG_HIP(hipMalloc(&e->pull_sum, sizeof(double) * hv.size()));
-            MG_HIP(hipMemcpy(e->pull_ch, pc.data(), sizeof(PullChunk) * pc.size(), hipMemcpyHostToDevice));
+            if (e->wide) {
+                MG_HIP(hipMalloc(&e->pull_ch, sizeof(PullChunkT<long long>) * pc.size()));
+                MG_HIP(hipMemcpy(e->pull_ch, pc.data(), sizeof(PullChunkT<long long>) * pc.size(), hipMemcpyHostToDevice));
+            } else {
+                std::vector<PullChunk> pc32(pc.size());
+                for (size_t k = 0; k < pc.size(); ++k) pc32[k] = PullChunk{pc[k].v, (int)pc[k].p0, (int)pc[k].p1, pc[k].k};
+                MG_HIP(hipMalloc(&e->pull_ch, sizeof(PullChunk) * pc.size()));
+                MG_HIP(hipMemcpy(e->pull_ch, pc32.data(), sizeof(PullChunk) * pc.size(), hipMemcpyHostToDevice));
+            }
             MG_HIP(hipMemcpy(e->pull_hv, hv.data(), sizeof(int) * hv.size(), hipMemcpyHostToDevice));
             if (e->algo == MG_BC)
                 hipLaunchKernelGGL(k_pull_slots_init<MG_BC_HOOKS>, dim3(grid_for((long long)hv.size(), 256, 64)), dim3(256), 0,
@@ -1124,9 +1165,15 @@ int mg_frontier_reset(mg_engine *e, int source) {
     }
     int deg_src = 0;
     if (nv > 0) {
-        int rr[2];
-        MG_HIP(hipMemcpy(rr, e->row + source, sizeof(int) * 2, hipMemcpyDeviceToHost));
-        deg_src = rr[1] - rr[0];
+        if (e->wide) {
+            long long rr[2];
+            MG_HIP(hipMemcpy(rr, e->row64 + source, sizeof(long long) * 2, hipMemcpyDeviceToHost));
+            deg_src = (int)(rr[1] - rr[0]);
+        } else {
+            int rr[2];
+            MG_HIP(hipMemcpy(rr, e->row + source, sizeof(int) * 2, hipMemcpyDeviceToHost));
+            deg_src = rr[1] - rr[0];
+        }
     }
     const int out_begin = (e->P.log_queue) ? 1 : nv;
     if (nv > 0)
@@ -1170,40 +1217,47 @@ static int launch_level(mg_engine *e, bool known) {
         if (!known || !c.lb) {
             const int gt = known ? grid_for(c.in_n, kBlock, G) : G;
             if (e->prof) mg_prof_begin(e, "k_push", bytes, &slot);
-            switch (e->algo) {
-            case MG_BFS:
-                hipLaunchKernelGGL((k_push<MG_BFS_HOOKS, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
-                break;
-            case MG_SSSP:
-                hipLaunchKernelGGL((k_push<MG_SSSP_HOOKS, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
-                break;
-            default:
-                hipLaunchKernelGGL((k_push<MG_BC_HOOKS, true>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
-                break;
-            }
+            MG_WITH_OFFSETS(e, switch (e->algo) {
+                case MG_BFS:
+                    hipLaunchKernelGGL((k_push<MG_BFS_HOOKS, true, OT>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, rowp, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                    break;
+                case MG_SSSP:
+                    hipLaunchKernelGGL((k_push<MG_SSSP_HOOKS, true, OT>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, rowp, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_push<MG_BC_HOOKS, true, OT>), dim3(gt), dim3(kBlock), 0, s, e->ctl, e->P, rowp, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->chunks);
+                    break;
+            });
             if (e->prof) mg_prof_end(e, slot);
         }
         if (!known || c.lb) {
             const long long maxch = known ? (c.mf + kEPB - 1) / kEPB : (e->ne + kEPB - 1) / kEPB;
             const int gd = known ? grid_for(c.in_n, kTile, G) : G;
             if (e->prof) mg_prof_begin(e, "k_lb_scan", 12.0 * c.in_n, &slot);
-            hipLaunchKernelGGL(k_lb_deg, dim3(gd), dim3(kBlock), 0, s, e->ctl, e->row, e->q, e->eoff, e->tsum);
+            MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_lb_deg<OT>), dim3(gd), dim3(kBlock), 0, s, e->ctl, rowp, e->q, e->eoff, e->tsum));
             hipLaunchKernelGGL(k_lb_tscan, dim3(1), dim3(1024), 0, s, e->ctl, e->tsum);
             hipLaunchKernelGGL(k_lb_part, dim3(grid_for(maxch + 1, 256, G)), dim3(256), 0, s, e->ctl, e->eoff, e->tsum, e->cstart);
             if (e->prof) mg_prof_end(e, slot);
             const int gp = known ? grid_for(maxch, 1, G) : G;
             if (e->prof) mg_prof_begin(e, "k_push_lb", bytes, &slot);
-            MG_WITH_BLOCK(e, switch (e->algo) {
-                case MG_BFS:
-                    hipLaunchKernelGGL((k_push_lb<MG_BFS_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
-                    break;
-                case MG_SSSP:
-                    hipLaunchKernelGGL((k_push_lb<MG_SSSP_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
-                    break;
-                default:
-                    hipLaunchKernelGGL((k_push_lb<MG_BC_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
-                    break;
-            });
+            if (e->wide) {  // BFS / SSSP only
+                if (e->algo == MG_BFS)
+                    hipLaunchKernelGGL((k_push_lb<MG_BFS_HOOKS, kBlock, long long>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row64, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                else
+                    hipLaunchKernelGGL((k_push_lb<MG_SSSP_HOOKS, kBlock, long long>), dim3(gp), dim3(kBlock), 0, s, e->ctl, e->row64, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+            } else {
+                MG_WITH_BLOCK(e, switch (e->algo) {
+                    case MG_BFS:
+                        hipLaunchKernelGGL((k_push_lb<MG_BFS_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                        break;
+                    case MG_SSSP:
+                        hipLaunchKernelGGL((k_push_lb<MG_SSSP_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                        break;
+                    default:
+                        hipLaunchKernelGGL((k_push_lb<MG_BC_HOOKS, kB>), dim3(gp), dim3(kB), 0, s, e->ctl, e->row, e->col, e->w, e->level, e->vfc, e->sigma, e->q, e->eoff, e->tsum, e->cstart);
+                        break;
+                });
+            }
             if (e->prof) mg_prof_end(e, slot);
         }
     }
@@ -1217,12 +1271,12 @@ static int launch_level(mg_engine *e, bool known) {
             MG_TRY(mg_ps_sweep(e));
         } else {
         if (e->prof) mg_prof_begin(e, "k_pull_sssp", bytes, &slot);
-        hipLaunchKernelGGL(k_pull_sssp<MG_SSSP_HOOKS>, dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl, e->pnch, e->ne, e->in_row(),
-                           e->in_col(), e->in_w(), e->pcstart, e->level, e->vfc);
+        MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_pull_sssp<MG_SSSP_HOOKS, OT>), dim3(grid_for(e->pnch, 1, G)), dim3(kBlock), 0, s, e->ctl,
+                                              e->pnch, e->ne, irowp, e->in_col(), e->in_w(), e->pcstart, e->level, e->vfc));
         if (e->prof) mg_prof_end(e, slot);
         }
-        hipLaunchKernelGGL(k_sssp_scan, dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, e->row, e->bm[0], e->part_n,
-                           e->part_mf);
+        MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_sssp_scan<OT>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, rowp, e->bm[0],
+                                              e->part_n, e->part_mf));
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
         bu = false;
     }
@@ -1235,25 +1289,31 @@ static int launch_level(mg_engine *e, bool known) {
         const int cut = e->pull_ch ? e->opt.pull_split : 0x7FFFFFFF;
         const int lanes = e->opt.lane_edges;
         int gpl = gsc;  // k_pull's workgroups = its partials
-        MG_WITH_BLOCK(e, gpl = grid_for(words, kB / MG_WAVE, G);
-            if (e->algo == MG_BC)
-                hipLaunchKernelGGL((k_pull<MG_BC_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
-            else
-                hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf));
+        if (e->wide) {  // BFS only: a wide graph is symmetric (in-edge rows = out-edge rows)
+            hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kBlock, long long>), dim3(gpl), dim3(kBlock), 0, s, e->ctl, nv, cut, lanes, e->row64,
+                               e->col, (const long long *)nullptr, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+        } else {
+            MG_WITH_BLOCK(e, gpl = grid_for(words, kB / MG_WAVE, G);
+                if (e->algo == MG_BC)
+                    hipLaunchKernelGGL((k_pull<MG_BC_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+                else
+                    hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf));
+        }
         if (e->pull_nch > 0) {
             const int gh = grid_for(e->pull_nch, kBlock / MG_WAVE, G);
             const int gf = grid_for(e->pull_nh, kBlock, 64);
-            if (e->algo == MG_BC) {
-                hipLaunchKernelGGL((k_pull_huge<MG_BC_HOOKS>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
-                                   e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
-                hipLaunchKernelGGL((k_pull_huge_fin<MG_BC_HOOKS>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, e->row,
-                                   e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
-            } else {
-                hipLaunchKernelGGL((k_pull_huge<MG_BFS_HOOKS>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, e->pull_ch, e->in_col(), e->sigma,
-                                   e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
-                hipLaunchKernelGGL((k_pull_huge_fin<MG_BFS_HOOKS>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, e->row,
-                                   e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
-            }
+            MG_WITH_OFFSETS(e, const PullChunkT<OT> *pch = (const PullChunkT<OT> *)e->pull_ch;
+                if (e->algo == MG_BC) {
+                    hipLaunchKernelGGL((k_pull_huge<MG_BC_HOOKS, OT>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, pch, e->in_col(), e->sigma,
+                                       e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
+                    hipLaunchKernelGGL((k_pull_huge_fin<MG_BC_HOOKS, OT>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, rowp,
+                                       e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
+                } else {
+                    hipLaunchKernelGGL((k_pull_huge<MG_BFS_HOOKS, OT>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, pch, e->in_col(), e->sigma,
+                                       e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
+                    hipLaunchKernelGGL((k_pull_huge_fin<MG_BFS_HOOKS, OT>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, rowp,
+                                       e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
+                });
         }
         if (e->prof) mg_prof_end(e, slot);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gpl);
@@ -1278,8 +1338,8 @@ static int launch_after_ctl(mg_engine *e) {
     hipLaunchKernelGGL(k_bmq_scan, dim3(1), dim3(1024), 0, s, e->ctl, e->part_n, gq);
     hipLaunchKernelGGL(k_bmq_write, dim3(gq), dim3(kBlock), 0, s, e->ctl, words, e->bm[0], e->bm[1], e->part_n, e->q);
     if (e->algo == MG_SSSP)
-        hipLaunchKernelGGL(k_sssp_scan, dim3(grid_for(words, kBlock / MG_WAVE, G)), dim3(kBlock), 0, s, e->ctl, nv, -1, 2,
-                           e->vfc, e->row, e->bm[0], (int *)nullptr, (long long *)nullptr);
+        MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_sssp_scan<OT>), dim3(grid_for(words, kBlock / MG_WAVE, G)), dim3(kBlock), 0, s, e->ctl, nv,
+                                              -1, 2, e->vfc, rowp, e->bm[0], (int *)nullptr, (long long *)nullptr));
     MG_HIP(hipGetLastError());
     return MG_OK;
 }

@@ -110,10 +110,10 @@ __device__ __forceinline__ int acc_apply_atomic(typename H::V1T *loc, double *lo
     }
 }
 
-// push along one edge src -> u (edge index p): message from the source's state, then
-// the push-side apply; returns 1 when u joins the next frontier
-template <class H>
-__device__ __forceinline__ int relax(typename H::V1T sval, double s2, int u, int p, const int *wgt,
+// push along one edge src -> u (edge index p, int or 64-bit): message from the source's
+// state, then the push-side apply; returns 1 when u joins the next frontier
+template <class H, class PT>
+__device__ __forceinline__ int relax(typename H::V1T sval, double s2, int u, PT p, const int *wgt,
                                      typename H::V1T *state, int *vfc, double *sigma, int iter) {
     typename H::V1T msg;
     if constexpr (H::kV2) {

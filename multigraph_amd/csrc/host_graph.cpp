@@ -46,6 +46,11 @@ int mg_host_threads() {
     return n;
 }
 
+// Entries of one build (after symmetrization): int32 row offsets up to 2^31 - 1 kept
+// edges, 64-bit ones past that (a "wide" graph, row64), up to 2^32 entries (RMAT-27
+// with edge factor 16).
+constexpr int64_t kMaxEntries = (int64_t)1 << 32;
+
 // ---------------------------------------------------------------------------
 // generate_CSC core (read_input.cu:88-171), restated for large inputs:
 //  * entry i of the input becomes (src,dst,val) and, when symmetrizing, (dst,src,val)
@@ -60,7 +65,7 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
                           const int32_t *val, bool sym, bool cc, int source, mg_graph *g) {
     if (nv < 0 || nnz < 0) return mg_fail(MG_ERR_ARG, "negative size");
     const int64_t n = sym ? 2 * nnz : nnz;
-    if (n > (int64_t)INT32_MAX) return mg_fail(MG_ERR_LIMIT, "%lld entries exceed the int32 CSR limit", (long long)n);
+    if (n > kMaxEntries) return mg_fail(MG_ERR_LIMIT, "%lld entries exceed the CSR limit of %lld", (long long)n, (long long)kMaxEntries);
     for (int64_t i = 0; i < nnz; ++i) {
         if (src[i] < 0 || src[i] >= nv || dst[i] < 0 || dst[i] >= nv)
             return mg_fail(MG_ERR_RANGE, "A vertex id is out of range %d %d", src[i], dst[i]);
@@ -189,7 +194,6 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
                        first_row + 1, sd[first_nonempty] + 1);
     // CSR of kept edges in compacted ids (read_input.cu:135-171)
     g->nv = cnv;
-    g->row.assign((size_t)cnv + 1, 0);
     std::vector<int64_t> rowcnt((size_t)nv, 0);
     mg_parallel_dyn(nv, [&](int64_t b, int64_t e) {
         for (int64_t v = b; v < e; ++v) {
@@ -208,14 +212,22 @@ int mg_build_from_entries(int32_t nv, int64_t nnz, const int32_t *src, const int
     g->entries = n;
     g->col.resize((size_t)ne);
     g->w.resize((size_t)ne);
-    for (int32_t v = 0; v < nv; ++v) {
-        if (!rowcnt[v]) continue;
-        int32_t cv = g->valid[v];
-        g->row[(size_t)cv + 1] = (int32_t)rstart[v + 1];
+    {
+        std::vector<int64_t> r((size_t)cnv + 1, 0);
+        for (int32_t v = 0; v < nv; ++v) {
+            if (!rowcnt[v]) continue;
+            r[(size_t)g->valid[v] + 1] = rstart[v + 1];
+        }
+        // fill rows of compacted vertices without kept edges (read_input.cu:167-170)
+        for (int32_t v = 1; v <= cnv; ++v) if (r[v] < r[v - 1]) r[v] = r[v - 1];
+        r[cnv] = ne;
+        if (ne > (int64_t)INT32_MAX) {
+            g->row64 = std::move(r);
+        } else {
+            g->row.resize((size_t)cnv + 1);
+            for (size_t v = 0; v <= (size_t)cnv; ++v) g->row[v] = (int32_t)r[v];
+        }
     }
-    // fill rows of compacted vertices without kept edges (read_input.cu:167-170)
-    for (int32_t v = 1; v <= cnv; ++v) if (g->row[v] < g->row[v - 1]) g->row[v] = g->row[v - 1];
-    g->row[cnv] = (int32_t)ne;
     mg_parallel_dyn(nv, [&](int64_t b, int64_t e) {
         for (int64_t v = b; v < e; ++v) {
             int64_t o = rstart[v];
@@ -339,7 +351,7 @@ extern "C" int mg_graph_gen_rmat(int scale, int edge_factor, double a, double b,
     *out = nullptr;
     const int64_t nv = (int64_t)1 << scale;
     const int64_t draws = nv * (int64_t)edge_factor;
-    if (draws * 2 > (int64_t)INT32_MAX) return mg_fail(MG_ERR_LIMIT, "RMAT too large for the int32 CSR");
+    if (draws * 2 > kMaxEntries) return mg_fail(MG_ERR_LIMIT, "RMAT of %lld edge draws exceeds the CSR limit", (long long)draws);
     std::vector<int32_t> s((size_t)draws), d((size_t)draws), w((size_t)draws);
     std::vector<uint8_t> self((size_t)draws, 0);
     // quadrant choice per level from 16 random bits (4 levels per 64-bit draw), with the
@@ -491,12 +503,12 @@ extern "C" int mg_graph_write_mtx(const mg_graph *g, const char *path) {
     if (!fp) return mg_fail(MG_ERR_IO, "cannot write %s", path);
     int64_t nnz = 0;
     for (int32_t u = 0; u < g->nv; ++u)
-        for (int32_t p = g->row[u]; p < g->row[u + 1]; ++p) if (g->col[p] <= u) ++nnz;
+        for (int64_t p = g->off(u); p < g->off(u + 1); ++p) if (g->col[p] <= u) ++nnz;
     fprintf(fp, "%%%%MatrixMarket matrix coordinate integer symmetric\n%% written by multigraph-amd\n");
     fprintf(fp, "%d %d %lld\n", g->nv, g->nv, (long long)nnz);
     std::string line;
     for (int32_t u = 0; u < g->nv; ++u)
-        for (int32_t p = g->row[u]; p < g->row[u + 1]; ++p)
+        for (int64_t p = g->off(u); p < g->off(u + 1); ++p)
             if (g->col[p] <= u) fprintf(fp, "%d %d %d\n", u + 1, g->col[p] + 1, g->w[p]);
     if (fclose(fp) != 0) return mg_fail(MG_ERR_IO, "write failed on %s", path);
     return MG_OK;
@@ -518,9 +530,16 @@ extern "C" int mg_graph_symmetric(const mg_graph *g, int *symmetric) {
 
 extern "C" int mg_graph_csr(const mg_graph *g, const int32_t **row, const int32_t **col, const int32_t **weight) {
     if (!g) return mg_fail(MG_ERR_ARG, "null graph");
+    if (row && g->wide()) return mg_fail(MG_ERR_LIMIT, "%lld edges: the row offsets are 64-bit (mg_graph_csr64)", (long long)g->ne);
     if (row) *row = g->row.data();
     if (col) *col = g->col.data();
     if (weight) *weight = g->w.data();
+    return MG_OK;
+}
+
+extern "C" int mg_graph_csr64(const mg_graph *g, int64_t *row) {
+    if (!g || !row) return mg_fail(MG_ERR_ARG, "null argument");
+    for (int64_t v = 0; v <= g->nv; ++v) row[v] = g->off(v);
     return MG_OK;
 }
 
@@ -539,9 +558,10 @@ extern "C" int mg_graph_remap(const mg_graph *g, int orig_id, int32_t *compact_i
 
 extern "C" int mg_graph_max_degree_vertex(const mg_graph *g, int32_t *v) {
     if (!g || !v) return mg_fail(MG_ERR_ARG, "null argument");
-    int32_t best = 0, bd = -1;
+    int32_t best = 0;
+    int64_t bd = -1;
     for (int32_t u = 0; u < g->nv; ++u) {
-        int32_t d = g->row[u + 1] - g->row[u];
+        const int64_t d = g->off(u + 1) - g->off(u);
         if (d > bd) { bd = d; best = u; }
     }
     *v = best;

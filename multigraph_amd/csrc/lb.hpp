@@ -9,7 +9,9 @@
 namespace mg {
 
 // cstart[c] = owner of the first edge of chunk c; cstart[nch] = owner of the last edge
-static __global__ void k_chunk_part(int nv, long long ne, const int *__restrict__ off, int nch, int *__restrict__ cstart) {
+// (OT: int or 64-bit row offsets)
+template <class OT = int>
+static __global__ void k_chunk_part(int nv, long long ne, const OT *__restrict__ off, int nch, int *__restrict__ cstart) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= nch; c += gridDim.x * blockDim.x) {
         const long long e = c < nch ? (long long)c * kEPB : ne - 1;
         int lo = 0, hi = nv - 1;  // largest u with off[u] <= e

@@ -22,11 +22,14 @@ struct mg_graph {
     int flags = 0;
     bool weighted = true;
     bool symmetric = true;     // every stored edge has its reverse stored too
-    std::vector<int32_t> row;  // nv + 1
+    std::vector<int32_t> row;  // nv + 1 (empty on a wide graph)
+    std::vector<int64_t> row64;  // nv + 1 instead of row when ne > 2^31 - 1 (a wide graph)
     std::vector<int32_t> col;  // ne
     std::vector<int32_t> w;    // ne
     std::vector<int32_t> orig; // compacted id -> original id
     std::vector<int32_t> valid;// original id -> compacted prefix (length orig_nv + 1)
+    bool wide() const { return !row64.empty(); }
+    int64_t off(int64_t v) const { return row64.empty() ? (int64_t)row[(size_t)v] : row64[(size_t)v]; }
 };
 
 void mg_set_error(const char *fmt, ...);

@@ -200,8 +200,9 @@ __global__ __launch_bounds__(kBT) void k_pb_weights(long long nt, int np, int P,
         for (int j = threadIdx.x; j < n; j += kBT) {
             const uint32_t key = keys[eb + j];
             const int src = inv[s * P + (int)(key >> 16)], dst = inv[d * P + (int)(key & 0xFFFFu)];
+            // edge positions reach 2^31 - 1: the midpoint is taken without adding them
             int lo = row[dst], hi = row[dst + 1] - 1;
-            while (lo < hi) { const int mid = (lo + hi) >> 1; if (col[mid] < src) lo = mid + 1; else hi = mid; }
+            while (lo < hi) { const int mid = lo + ((hi - lo) >> 1); if (col[mid] < src) lo = mid + 1; else hi = mid; }
             b_w[eb + j] = (W)w[lo];
         }
     }
@@ -1040,7 +1041,12 @@ int mg_pb_layout_build(mg_engine *e, const std::vector<int> &hrow, const int *ir
     if (rc != MG_OK) { cleanup(); return rc; }
     long long slots = 0;
     MG_HIP(hipMemcpy(&slots, boff + nt, sizeof(long long), hipMemcpyDeviceToHost));
-    if (slots >= (1ll << 31)) { cleanup(); return mg_fail(MG_ERR_LIMIT, "partitioned layout: %lld slots exceed int32", slots); }
+    // slot indices are int32 in the units and kernels; a margin of 2^24 keeps every
+    // segment bound (slot + a few hundred) inside the type
+    if (slots > (1ll << 31) - (1ll << 24)) {
+        cleanup();
+        return mg_fail(MG_ERR_LIMIT, "partitioned layout: %lld slots exceed the int32 range", slots);
+    }
     L.slots = slots;
     const long long sl = slots > 0 ? slots : kG;
     if (hipMalloc(&keys, sizeof(uint32_t) * sl) != hipSuccess) { cleanup(); return mg_fail(MG_ERR_NOMEM, "partitioned layout: keys"); }

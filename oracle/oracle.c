@@ -23,6 +23,10 @@
  *   orc_*_multi,       OpenMP drivers of the functions above for the full-size parity
  *   orc_pr_fp64_pull   tests: bit-identical to the single-thread functions (checked on
  *                      small graphs), so they inherit those functions' pins
+ *   orc_bfs64          networkx shortest-path lengths with the rows cast to int64
+ *   orc_sssp_certificate64  holds for networkx Dijkstra distances; fails for a raised
+ *                      or lowered distance, a dropped or invented reached vertex, a
+ *                      nonzero source (planted on small graphs)
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -411,4 +415,55 @@ int orc_pr_fp64_pull(int nv, const int32_t *row, const int32_t *col, int variant
 out:
     free(trow); free(tcol); free(cur); free(nr); free(c); free(mark);
     return rc;
+}
+
+/* ---- graphs past 2^31 - 1 edges (64-bit row offsets) ---------------------------- */
+
+/* validate.cu:106-127 with 64-bit row offsets: the same FIFO queue BFS as orc_bfs. */
+int orc_bfs64(int nv, const int64_t *row, const int32_t *col, int src, int32_t *level) {
+    int32_t *q = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nv > 0 ? nv : 1));
+    if (!q) return -1;
+    for (int i = 0; i < nv; ++i) level[i] = ORC_INF;
+    int head = 0, tail = 0;
+    if (nv > 0) { level[src] = 0; q[tail++] = src; }
+    while (head < tail) {
+        int i = q[head++];
+        for (int64_t j = row[i]; j < row[i + 1]; ++j) {
+            int k = col[j];
+            if (level[k] == ORC_INF) { level[k] = level[i] + 1; q[tail++] = k; }
+        }
+    }
+    free(q);
+    return tail;
+}
+
+/* Certificate of single-source shortest distances (positive integer weights): they are
+ * exact iff dist[src] = 0, no edge can lower them (dist[v] <= dist[u] + w(u,v) for
+ * every edge with dist[u] finite), every other reached vertex has an in-edge that is
+ * tight (dist[v] = dist[u] + w, u reached), and the reached vertices are exactly those
+ * the BFS levels `level` reach.  With a symmetric CSR a row's edges are its in-edges.
+ * Returns 0 when it holds; else 1 (source), 2 (an edge lowers a distance), 3 (a
+ * reached vertex has no tight in-edge) or 4 (reached sets differ). */
+int orc_sssp_certificate64(int nv, const int64_t *row, const int32_t *col, const int32_t *w, int src,
+                           const int32_t *dist, const int32_t *level) {
+    if (nv == 0) return 0;
+    if (dist[src] != 0) return 1;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad)
+    for (int v = 0; v < nv; ++v) {
+        if ((dist[v] == ORC_INF) != (level[v] == ORC_INF)) { bad |= 8; continue; }
+        int tight = v == src;
+        for (int64_t j = row[v]; j < row[v + 1]; ++j) {
+            const int u = col[j];
+            if (dist[u] == ORC_INF) continue;
+            const int64_t cand = (int64_t)dist[u] + w[j];
+            if ((int64_t)dist[v] > cand) bad |= 2;
+            if ((int64_t)dist[v] == cand) tight = 1;
+        }
+        if (dist[v] != ORC_INF && !tight) bad |= 4;
+    }
+    if (bad & 8) return 4;
+    if (bad & 2) return 2;
+    if (bad & 4) return 3;
+    return 0;
 }

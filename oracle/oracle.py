@@ -44,6 +44,9 @@ def lib():
         L.orc_sssp_dijkstra_multi.argtypes = [C.c_int, I, I, I, C.c_int, I, I]
         L.orc_bc_fp64_multi.argtypes = [C.c_int, I, I, C.c_int, I, I, D, D]
         L.orc_pr_fp64_pull.argtypes = [C.c_int, I, I, C.c_int, C.c_double, C.c_double, C.c_int, D]
+        I64 = C.POINTER(C.c_int64)
+        L.orc_bfs64.argtypes = [C.c_int, I64, I, C.c_int, I]
+        L.orc_sssp_certificate64.argtypes = [C.c_int, I64, I, I, C.c_int, I, I]
         _lib = L
     return _lib
 
@@ -57,11 +60,29 @@ def _csr(row, col):
 
 
 def bfs(row, col, src):
-    row, col = _csr(row, col)
+    """BFS levels; 64-bit row offsets (a graph past 2^31 - 1 edges) go to orc_bfs64"""
     nv = len(row) - 1
     out = np.empty(nv, np.int32)
+    if np.asarray(row).dtype == np.int64:
+        row = np.ascontiguousarray(row, np.int64)
+        col = np.ascontiguousarray(col, np.int32)
+        lib().orc_bfs64(nv, _p(row, C.c_int64), _p(col), src, _p(out))
+        return out
+    row, col = _csr(row, col)
     lib().orc_bfs(nv, _p(row), _p(col), src, _p(out))
     return out
+
+
+def sssp_certificate(row, col, w, src, dist, level):
+    """0 when `dist` are the exact shortest distances from src of a symmetric CSR
+    (orc_sssp_certificate64: source 0, no edge lowers a distance, every reached vertex
+    tight, reached set = the BFS levels' `level`); else the failed condition 1..4"""
+    row = np.ascontiguousarray(row, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    w = np.ascontiguousarray(w, np.int32)
+    dist = np.ascontiguousarray(dist, np.int32)
+    level = np.ascontiguousarray(level, np.int32)
+    return lib().orc_sssp_certificate64(len(row) - 1, _p(row, C.c_int64), _p(col), _p(w), src, _p(dist), _p(level))
 
 
 def sssp_bf(row, col, w, src):

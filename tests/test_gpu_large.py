@@ -96,38 +96,45 @@ def test_large_batched(mg, orc, big):
         b.close()
 
 
-def test_rmat25_bfs_sssp(mg, orc):
-    """The largest RMAT whose symmetrized CSR fits int32 offsets: scale 25 (17.1 M
-    vertices, 1.05 G directed edges after compaction; hub rows over 800 K edges).
-    BFS levels bit-exact against the oracle; SSSP distances proven exact by their
-    certificate (d[source] = 0; d[v] <= d[u] + w on every edge; every other reached
-    vertex has a tight in-edge; reached set = BFS's), checked in edge blocks."""
-    g = mg.Graph.rmat(25, 16, 0.57, 0.19, 0.19, seed=1, weight_seed=7)
-    row, col, w = g.csr()
+def _bfs_sssp_at_scale(mg, orc, g, tag, offsets):
+    """BFS levels bit-exact against the oracle; SSSP distances proven exact by their
+    certificate (source 0, no edge lowers a distance, every reached vertex has a tight
+    in-edge, reached set = BFS's); for each row-offset width in `offsets`"""
+    import time
+    t0 = time.time()
+    row, col, w = g.csr(copy=False)
     src = g.max_degree_vertex()
-    print(f"rmat25 nv={g.nv} ne={g.ne} src={src}", flush=True)
-    e = mg.Engine(g, mg.BFS)
-    e.run(src)
-    np.testing.assert_array_equal(e.result(mg.RES_LEVEL), orc.bfs(row, col, src))
-    e.close()
-    print("rmat25 bfs exact", flush=True)
-    e = mg.Engine(g, mg.SSSP)
-    e.run(src)
-    d = e.result(mg.RES_LEVEL).astype(np.int64)
-    e.close()
-    inf = 0x64646464
-    reached = d != inf
-    assert d[src] == 0
-    np.testing.assert_array_equal(reached, np.asarray(orc.bfs(row, col, src)) != inf)
-    tight = np.zeros(g.nv, bool)
-    blk = 1 << 26
-    for p0 in range(0, g.ne, blk):
-        p1 = min(g.ne, p0 + blk)
-        u = np.searchsorted(row, np.arange(p0, p1), side="right") - 1
-        du, dv, wp = d[u], d[col[p0:p1]], w[p0:p1]
-        ok = du != inf
-        assert np.all(dv[ok] <= du[ok] + wp[ok])
-        tight[col[p0:p1][ok & (dv == du + wp)]] = True
-    tight[src] = True
-    assert np.all(tight[reached])
-    print("rmat25 sssp certificate holds", flush=True)
+    print(f"{tag} nv={g.nv} ne={g.ne} row {row.dtype} src={src}", flush=True)
+    ref = orc.bfs(row, col, src)
+    print(f"{tag} oracle bfs ({time.time() - t0:.0f} s)", flush=True)
+    for wide in offsets:
+        e = mg.Engine(g, mg.BFS, wide_offsets=wide)
+        ms = e.run(src)
+        np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref)
+        e.close()
+        print(f"{tag} wide_offsets={wide} bfs exact ({ms:.2f} ms)", flush=True)
+        e = mg.Engine(g, mg.SSSP, wide_offsets=wide)
+        ms = e.run(src)
+        d = e.result(mg.RES_LEVEL)
+        e.close()
+        assert orc.sssp_certificate(row, col, w, src, d, ref) == 0
+        print(f"{tag} wide_offsets={wide} sssp certificate holds ({ms:.2f} ms, {time.time() - t0:.0f} s)", flush=True)
+
+
+def test_rmat26_bfs_sssp(mg, orc):
+    """Symmetrized RMAT-26 (32.8 M vertices, 2.10 G directed edges after compaction,
+    just inside int32 offsets; 2.15 G entries before deduplication), run with int32 row
+    offsets and with 64-bit ones forced (the kernels of a graph past 2^31 - 1 edges)."""
+    g = mg.Graph.rmat(26, 16, 0.57, 0.19, 0.19, seed=1, weight_seed=7)
+    assert g.ne <= 2**31 - 1
+    _bfs_sssp_at_scale(mg, orc, g, "rmat26", (0, 1))
+
+
+@pytest.mark.skipif(not __import__("os").environ.get("MG_TEST_RMAT27"),
+                    reason="several minutes and ~120 GB of host memory: set MG_TEST_RMAT27=1")
+def test_rmat27_wide_bfs_sssp(mg, orc):
+    """Past int32 offsets: symmetrized RMAT-27 (about 4.2 G directed edges), whose row
+    offsets are 64-bit on the host (mg_graph_csr64) and on the device."""
+    g = mg.Graph.rmat(27, 16, 0.57, 0.19, 0.19, seed=1, weight_seed=7)
+    assert g.ne > 2**31 - 1
+    _bfs_sssp_at_scale(mg, orc, g, "rmat27", (0,))

@@ -306,3 +306,36 @@ def test_block_threads_rejected(mg):
     g = graphs(mg)["edge"]
     with pytest.raises(mg.MGError):
         mg.Engine(g, mg.BFS, block_threads=300)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=[m[0] for m in MODES])
+def test_wide_offsets(mg, orc, case, mode):
+    """Engines with 64-bit row offsets (what a graph past 2^31 - 1 edges gets; forced
+    here on small graphs) give the same BFS levels and SSSP distances as the oracle"""
+    name, g, row, col, w = case
+    if mode[0] == "sparse":
+        with pytest.raises(mg.MGError):  # push-only levels need int32 frontier edge counts
+            mg.Engine(g, mg.BFS, wide_offsets=1, **mode[1])
+        return
+    for algo in (mg.BFS, mg.SSSP):
+        e = mg.Engine(g, algo, wide_offsets=1, **mode[1])
+        for s in sources(g):
+            print(f"wide {name} algo={algo} mode={mode[0]} src={s}", flush=True)
+            e.run(s)
+            ref = orc.bfs(row, col, s) if algo == mg.BFS else orc.sssp_bf(row, col, w, s)[0]
+            np.testing.assert_array_equal(e.result(mg.RES_LEVEL), ref)
+        e.close()
+
+
+def test_wide_offsets_limits(mg):
+    """64-bit row offsets serve BFS and SSSP on symmetric graphs; the rest is refused"""
+    g = graphs(mg)["rmat16"]
+    for algo in (mg.BC, mg.CC, mg.PR_T):
+        with pytest.raises(mg.MGError):
+            mg.Engine(g, algo, wide_offsets=1)
+    with pytest.raises(mg.MGError):
+        mg.Engine(g, mg.BFS, wide_offsets=1, alpha=2.0)
+    d = mg.Graph.from_coo(4, [0, 1, 2], [1, 2, 3], flags=0)  # directed path
+    assert not d.symmetric
+    with pytest.raises(mg.MGError):
+        mg.Engine(d, mg.BFS, wide_offsets=1)

@@ -290,3 +290,47 @@ def test_pr_pull_equals_push(orc, name, variant):
 
 def test_omp_threads_reported(orc):
     assert orc.threads() >= 1
+
+
+# ---- graphs past 2^31 - 1 edges: 64-bit row offsets ---------------------------------
+@pytest.mark.parametrize("name", NAMES)
+def test_bfs64_vs_networkx(orc, name):
+    """orc_bfs64 (int64 row offsets) gives networkx's shortest-path lengths"""
+    G = GRAPHS[name]
+    row, col = csr_of(G)
+    for s in _sources(G.number_of_nodes()):
+        lv = orc.bfs(row.astype(np.int64), col, s)
+        ref = np.full(G.number_of_nodes(), INF, np.int64)
+        for v, d in nx.single_source_shortest_path_length(G, s).items():
+            ref[v] = d
+        np.testing.assert_array_equal(lv, ref)
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if n != "directed"])
+def test_sssp_certificate(orc, name):
+    """The certificate holds for networkx's Dijkstra distances and fails for each kind
+    of wrong answer: a raised distance, a lowered one, a reached vertex marked
+    unreached, an unreached one given a distance, a nonzero source."""
+    G = GRAPHS[name]
+    row, col = csr_of(G)
+    w = weights_of(G, row, col)
+    n = G.number_of_nodes()
+    for s in _sources(n):
+        d = np.full(n, INF, np.int32)
+        for v, x in nx.single_source_dijkstra_path_length(G, s, weight="w").items():
+            d[v] = x
+        lv = orc.bfs(row, col, s)
+        assert orc.sssp_certificate(row, col, w, s, d, lv) == 0
+        far = int(np.argmax(np.where(d != INF, d, -1)))        # a reached vertex other than s
+        bad = d.copy(); bad[far] += 1
+        assert orc.sssp_certificate(row, col, w, s, bad, lv) != 0
+        bad = d.copy(); bad[far] -= 1
+        assert orc.sssp_certificate(row, col, w, s, bad, lv) != 0
+        bad = d.copy(); bad[far] = INF
+        assert orc.sssp_certificate(row, col, w, s, bad, lv) != 0
+        bad = d.copy(); bad[s] = 1
+        assert orc.sssp_certificate(row, col, w, s, bad, lv) == 1
+        un = np.flatnonzero(d == INF)
+        if len(un):
+            bad = d.copy(); bad[un[0]] = 5
+            assert orc.sssp_certificate(row, col, w, s, bad, lv) != 0
