@@ -84,7 +84,7 @@ def gteps(edges, ms):
 
 def measured_traffic(kernel, scale):
     """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None"""
+    counter summary (L2-to-memory request counts by size), or None"""
     path = os.path.join(ROOT, "profiles", "traffic_rmat22_sssp_cc.json")
     if scale != 22 or not os.path.exists(path):
         return None
@@ -415,7 +415,9 @@ def main():
                    "scale": args.scale, "edge_factor": 16, "nv": nv, "ne": ne, "ne_u": ne_u, "source": int(src)},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_name, args.scale),
-                     "traffic_source": "profiles/traffic_rmat22_sssp_cc.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+                     "traffic_source": "profiles/traffic_rmat22_sssp_cc.json (rocprofv3 L2-to-memory requests: 128 B per read "
+                                       "request, the size a copy of known bytes issues; 64 B per 64-byte write request, "
+                                       "32 B per other)",
                      "avg_launch_ms": avg_ms,
                      "bytes_per_launch": avg_bytes, "launches": dom["launches"],
                      "dominant_in_profiled_step": dominant_ok},
