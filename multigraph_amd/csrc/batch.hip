@@ -34,13 +34,6 @@ namespace {
 
 constexpr int kBB = 256;
 
-// order this wave's LDS stores before its loads of other lanes' slots (and the loads
-// of one chunk before the stores of the next)
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 int grid_for(long long items, int per_block, int cap) {
     long long g = (items + per_block - 1) / per_block;

@@ -304,7 +304,10 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // the in-edges; orow, the out-edge rows of a directed graph (null when symmetric),
 // gives the degree sum.  The new frontier is the bitmap itself; its size and degree sum
 // leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
-constexpr int kPullStep = 8;      // BC pull: edges of a light row loaded together per step
+constexpr int kPullStep = 8;      // BC pull: edges a lane loads together per step
+#ifndef MG_PULL_FLAT
+#define MG_PULL_FLAT 1            // BC pull: a word's rows as one flat edge range (0: lane / wave tiers)
+#endif
 
 template <class H, int B, class OT = int>
 __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_max, const OT *__restrict__ row,
@@ -322,6 +325,13 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
     const int words = (nv + 63) >> 6;
     const int wave = (blockIdx.x * B + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * B) >> 6;
+#if MG_PULL_FLAT
+    // per wave: the flat walk's row prefixes and starts, and the rows' accumulators
+    __shared__ int s_incl[kSum ? B : 1];
+    __shared__ OT s_base[kSum ? B : 1];
+    __shared__ typename H::V1T s_lv[kSum ? B : 1];
+    __shared__ double s_ls[kSum ? B : 1];
+#endif
     int cnt = 0;
     long long mf = 0;
     for (int w = wave; w < words; w += nwaves) {
@@ -343,6 +353,75 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
             }
         }
         if (kSum) {
+#if MG_PULL_FLAT
+            // The word's rows (those of at most `cut` edges; longer ones are k_pull_huge's)
+            // walked as one flat edge range: step by step, lane l takes kPullStep
+            // consecutive edges of it.  Every step is full whatever the row lengths, and
+            // the rows of consecutive vertices are consecutive in col, so a step reads one
+            // contiguous run.  A lane sums its edges' path counts per row in registers and
+            // adds each row's sum into that row's LDS slot (accumulate) when its edges move
+            // to the next row; path counts are integers, exact in FP64 in any order.
+            const int wl = threadIdx.x & ~(MG_WAVE - 1);
+            const int hd = (int)(e - b);  // 0 for a closed or cut row
+            int incl = hd;                // inclusive prefix of the row lengths over the lanes
+#pragma unroll
+            for (int o = 1; o < MG_WAVE; o <<= 1) {
+                const int t = __shfl_up(incl, o);
+                if (lane() >= o) incl += t;
+            }
+            const int T = __shfl(incl, MG_WAVE - 1);
+            {
+                Acc<H> id;
+                acc_init(id);
+                wave_sync();  // the previous word's slots have been read
+                s_incl[threadIdx.x] = incl;
+                s_base[threadIdx.x] = b - (OT)(incl - hd);  // col index of flat edge x of this row: s_base + x
+                s_lv[threadIdx.x] = id.v;
+                s_ls[threadIdx.x] = id.s;
+                wave_sync();
+            }
+            for (int x0 = 0; x0 < T; x0 += MG_WAVE * kPullStep) {
+                const int xb = x0 + lane() * kPullStep;
+                int r = 0;  // row of flat edge xb: the number of rows ending at or before it
+                if (xb < T) {
+#pragma unroll
+                    for (int d = MG_WAVE / 2; d > 0; d >>= 1)
+                        if (s_incl[wl + r + d - 1] <= xb) r += d;
+                }
+                int u[kPullStep], rj[kPullStep];
+                bool f[kPullStep];
+                double sv[kPullStep];
+#pragma unroll
+                for (int k = 0; k < kPullStep; ++k) {
+                    const int x = xb + k;
+                    f[k] = x < T;
+                    if (f[k])
+                        while (s_incl[wl + r] <= x) ++r;  // ends at r <= 63: incl[63] = T > x
+                    rj[k] = r;
+                    u[k] = f[k] ? col[s_base[wl + r] + x] : 0;
+                }
+#pragma unroll
+                for (int k = 0; k < kPullStep; ++k) f[k] = f[k] && bm_test(fr, u[k]);
+#pragma unroll
+                for (int k = 0; k < kPullStep; ++k) sv[k] = f[k] ? sigma[u[k]] : 0.0;
+                Acc<H> cur;
+                acc_init(cur);
+                int cr = rj[0];
+#pragma unroll
+                for (int k = 0; k < kPullStep; ++k) {
+                    if (rj[k] != cr) {
+                        if (!acc_is_identity<H>(cur)) acc_atomic<H>(&s_lv[wl + cr], &s_ls[wl + cr], cur, iter);
+                        acc_init(cur);
+                        cr = rj[k];
+                    }
+                    if (f[k]) acc_edge<H>(cur, plev, sv[k], 0, iter);
+                }
+                if (!acc_is_identity<H>(cur)) acc_atomic<H>(&s_lv[wl + cr], &s_ls[wl + cr], cur, iter);
+            }
+            wave_sync();
+            a.v = s_lv[threadIdx.x];
+            a.s = s_ls[threadIdx.x];
+#else
             // lanes walk rows of at most lane_max edges alone, stepping together for the
             // longest of them; longer rows are walked by the whole wave, one at a time
             // (lanes over edges, then the lanes' accumulators merged).  Path counts are
@@ -398,6 +477,7 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
                 h = acc_wave<H>(h, iter);
                 if (lane() == l) a = h;
             }
+#endif
         }
         // an unvisited vertex's level is the identity: its register copy is the old value
         Acc<H> none;

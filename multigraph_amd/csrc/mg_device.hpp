@@ -17,6 +17,14 @@ __device__ __forceinline__ int lane() { return (int)__lane_id(); }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p ? 1 : 0); }
 
+// order this wave's LDS stores before its loads of other lanes' slots, and its loads
+// before the next stores
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // number of set bits of `m` in lanes below this one
 __device__ __forceinline__ int mbcnt(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

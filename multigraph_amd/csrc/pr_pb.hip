@@ -43,13 +43,6 @@ constexpr int kG = 8;           // slots per group of either stream (one 16-byte
 constexpr int kBT = 512;        // threads per workgroup of the layout and phase kernels
 constexpr int kBuildEdges = 1 << 16;
 
-// order this wave's LDS stores before its loads of other lanes' slots, and its loads
-// before the next stores
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 int grid_for(long long items, int per_block, int cap) {
     long long g = (items + per_block - 1) / per_block;
