@@ -241,7 +241,7 @@ def uneven(mg, np, orc):
                      "tuned_options": {"alpha": opt.alpha, "beta": opt.beta, "batch_chunk": opt.batch_chunk,
                                        "grid_blocks": opt.grid_blocks, "graph_levels": opt.graph_levels,
                                        "persist_levels": opt.persist_levels, "sssp_delta": opt.sssp_delta,
-                                       "batch_light": opt.batch_light},
+                                       "batch_light": opt.batch_light, "sparse_grid": opt.sparse_grid},
                      "untuned_stats": st_def}
         log(f"uneven {name}: {t_def:.2f} ms untuned, {t_tun:.2f} ms tuned ({tune_s:.1f}s of tuning)")
     return out

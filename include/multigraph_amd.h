@@ -112,6 +112,8 @@ typedef struct mg_options {
                                backward kernels: 256, 512 or 1024 [256]                 */
     int    batch_light;     /* batched BFS pull: vertices of at most this many in-edges
                                are walked by one lane, heavier ones by a wave [32]      */
+    int    sparse_grid;     /* batched BFS / SSSP: workgroups of a sparse (queue) level
+                               (0: one per CU)                                          */
     int    wide_offsets;    /* 1: the engine keeps 64-bit row offsets on the device even
                                when the graph's fit int32 (a wide graph always has them;
                                BFS and SSSP run on such engines, symmetric graphs)      */

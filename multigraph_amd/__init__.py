@@ -61,6 +61,7 @@ class Options(C.Structure):
         ("back_chunk", C.c_int),
         ("block_threads", C.c_int),
         ("batch_light", C.c_int),
+        ("sparse_grid", C.c_int),
         ("wide_offsets", C.c_int),
     ]
 

@@ -138,6 +138,7 @@ extern "C" int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32
     if (kind != MG_BATCH_BC) {
         coords.push_back({[](mg_options &o, double v) { o.graph_levels = (int)v; }, {2, 8, 32}});
         coords.push_back({[](mg_options &o, double v) { o.persist_levels = (int)v; }, {0, -1}});
+        coords.push_back({[](mg_options &o, double v) { o.sparse_grid = (int)v; }, {0, 64, 128}});
     }
     if (kind == MG_BATCH_SSSP)  // near-far band (0: a quarter of the mean weight; -1: off)
         coords.push_back({[](mg_options &o, double v) { o.sssp_delta = (int)v; }, {-1, 0, 2, 4, 16}});

@@ -1730,7 +1730,8 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
     MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const LevelRule rule = rule_of(b);
     const int gq = grid_for(nv, 2048, 2048);
-    const int gsp = std::min(gv, std::max(1, b->cus * MG_SPARSE_PER_CU));
+    // sparse levels: mg_options.sparse_grid workgroups, else MG_SPARSE_PER_CU per CU
+    const int gsp = std::min(gv, std::max(1, b->opt.sparse_grid > 0 ? b->opt.sparse_grid : b->cus * MG_SPARSE_PER_CU));
     auto sparse_level = [&]() {
         if (sssp)
             hipLaunchKernelGGL(k_mbs_level_sssp, dim3(gsp), dim3(kBB), 0, s, b->ctl, b->row, b->col, b->w, b->F[0], b->F[1], b->Q[0],

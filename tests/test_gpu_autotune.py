@@ -66,6 +66,7 @@ def test_autotune_batch_union(mg, orc):
         assert t1 <= t0 + 1e-6 and opt.batch_chunk in (0, 256, 512, 1024, 4096)
         if kind == mg.BATCH_BFS:
             assert opt.batch_light in (0, 16, 32, 64)
+        assert opt.sparse_grid in (0, 64, 128)
         b = mg.Batch(u, kind, options=opt)
         b.run(src, slots)
         lv = b.levels()

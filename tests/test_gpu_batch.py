@@ -157,9 +157,12 @@ def test_batch_sssp_isolated_sources(mg, orc, mode):
     b.close()
 
 
+@pytest.mark.parametrize("grid", [0, 7], ids=["grid-per-cu", "grid-7"])
 @pytest.mark.parametrize("kind", ["bfs", "sssp", "bc"])
-def test_batch_union_slots(mg, orc, kind):
-    """uneven batch: disjoint member graphs share the slots of one run (4 sources each)"""
+def test_batch_union_slots(mg, orc, kind, grid):
+    """uneven batch: disjoint member graphs share the slots of one run (4 sources each);
+    sparse levels on one workgroup per CU or on 7 (each workgroup then walks many queue
+    entries)"""
     parts = [mg.Graph.rmat(10, 8, seed=31), mg.Graph.grid(23, 17), mg.Graph.rmat(8, 4, seed=32),
              mg.Graph.from_coo(50, np.arange(49), np.arange(1, 50), np.full(49, 2))]
     u = mg.Graph.union(parts)
@@ -173,7 +176,7 @@ def test_batch_union_slots(mg, orc, kind):
         slots += [0, 1, 2, 3]
         off += p.nv
     k = {"bfs": mg.BATCH_BFS, "sssp": mg.BATCH_SSSP, "bc": mg.BATCH_BC}[kind]
-    b = mg.Batch(u, k)
+    b = mg.Batch(u, k, sparse_grid=grid)
     b.run(srcs, slots)
     lv = b.levels()
     assert lv.shape == (4, u.nv)
