@@ -177,6 +177,8 @@ struct BCtl {
     int thr;      // SSSP: sparse sweeps relax the (vertex, source) pairs below this distance
     long long n, mf;  // size and degree sum of the last produced frontier
     unsigned long long all;  // the slots in use
+    int err;      // a persistent launch gave up a bounded wait: its results are not valid
+    int big;      // the last persistent launch ran no level: the frontier is too large for one XCD
 };
 
 // A level kernel gets two frontier buffers (A, B).  Without a control block A is the
@@ -981,6 +983,12 @@ struct PCtl {
     unsigned top;            // barrier: groups of workgroups arrived (zeroed before the launch)
     unsigned pad[15];
     unsigned grp[8 * 16];    // barrier: arrivals per group, one 64 B line each
+    // entry-queue levels (k_mbp_bfs): an entry queue overflowed; the handback queue's
+    // length, degree sum and largest degree
+    int over, fq, fmx, err;
+    long long fmf;
+    unsigned xcc0, reg, xbar, pad3;  // its XCD (XCC id + 1), workgroups counted in, barrier
+    int npart, pad4[3];
 };
 
 #ifndef MG_SPARSE_PER_CU
@@ -1116,6 +1124,374 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
         o.nsparse = c0.nsparse + ran;
         o.npersist = c0.npersist + ran;
         o.thr = thr;
+        *a.ctl = o;
+    }
+}
+
+// ---- persistent sparse BFS levels on one XCD, over entry queues ---------------------
+// The long tails of batched BFS (thousands of levels of a few thousand vertices: the
+// large grids of an uneven batch) are bound by the latency of a level, not by its
+// bytes.  Here one cooperative launch runs consecutive sparse levels on the workgroups
+// of ONE XCD (those whose XCC id is block 0's): they share that XCD's L2, so a level
+// hands its frontier to the next through plain stores (written through each CU's L1
+// into the L2) and L1-bypassing loads (sc0) that read the L2, and the barrier between
+// levels is an L2 atomic counter, with no cache write-back or invalidation.  The other
+// workgroups of the launch leave at once.
+//
+// A level's frontier is a list of entries (vertex, first edge, one past its last edge,
+// the sources it carries), so a level loads no frontier word or row before it walks
+// edges: a claim on the visited word (atomicOr) returns the new bits, and the
+// discoverer, which loads the new vertex's row bounds alongside the claim, appends the
+// entry.  Two discoverers of one vertex claim disjoint bits and append two entries.
+// Levels run here while the frontier is small (vertices + edges at most kXcdWork per
+// workgroup of the XCD); a larger or non-sparse level, an overflowing entry queue or
+// max_levels hand the frontier back to the captured graphs as words, queue and
+// partials.
+typedef unsigned int mg_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t l2_rsrc(const void *p, long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)min(bytes, 0x7FFFFFF0ll), 0x00020000);
+}
+// 64-bit word i of a buffer, read from the L2 (sc0: not from the CU's L1)
+__device__ __forceinline__ uint64_t ld_l2(__amdgpu_buffer_rsrc_t r, int i) {
+    const mg_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 1);
+    return (uint64_t)v.x | ((uint64_t)v.y << 32);
+}
+// the current value of a counter that is only changed by atomics: read by an atomic
+__device__ __forceinline__ unsigned long long ld_atomic(unsigned long long *p) {
+    return __hip_atomic_fetch_add(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_atomic(int *p) { return __hip_atomic_fetch_add(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ unsigned ld_atomic(unsigned *p) { return __hip_atomic_fetch_add(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+constexpr int kXcdWork = 16 * kBB;            // frontier vertices + edges per workgroup of a persistent level
+constexpr long long kSpinTicks = 100000000;   // bound of one barrier wait: 1 s of the 100 MHz wall clock
+
+// Barrier of the XCD's P workgroups.  Each thread waits for its own memory operations
+// to complete (stores are then in the L2), the workgroup counts in on an L2 counter and
+// polls it by atomics.  The wait is bounded: past kSpinTicks the launch is marked
+// failed (err) and every later wait returns at once, so a barrier cannot hang.
+__device__ __forceinline__ bool xcd_sync(PCtl *pc, unsigned &gen, int P) {
+    ++gen;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    __shared__ int s_err;
+    if (threadIdx.x == 0) {
+        atomicAdd(&pc->xbar, 1u);
+        const long long t0 = wall_clock64();
+        int err = ld_atomic(&pc->err);
+        while (!err && ld_atomic(&pc->xbar) < gen * (unsigned)P) {
+            __builtin_amdgcn_s_sleep(1);
+            if (wall_clock64() - t0 > kSpinTicks) { atomicExch(&pc->err, 1); err = 1; }
+            else err = ld_atomic(&pc->err);
+        }
+        s_err = err;
+    }
+    __syncthreads();
+    return s_err == 0;
+}
+
+// entry queue: ea = vertex | first edge << 32, eb = one past its last edge, ec = its bits
+struct EQ { uint64_t *ea, *eb, *ec; };
+
+// handback of a vertex with bits: or-ed into its frontier word F, queued in Q when the
+// word was empty (the form a sparse level of the captured graphs leaves)
+__device__ __forceinline__ void eq_handback(bool on, int w, int deg, uint64_t bits, uint64_t *F, int *Q, PCtl *pc) {
+    const bool add = on && atomicOr((unsigned long long *)&F[w], (unsigned long long)bits) == 0ull;
+    const int k = wave_append(&pc->fq, add);
+    if (add) {
+        Q[k] = w;
+        atomicAdd((unsigned long long *)&pc->fmf, (unsigned long long)deg);
+        atomicMax(&pc->fmx, deg);
+    }
+}
+
+// U candidate entries per lane (on[j]: vertex w[j] with edges [q0, q1) and bits m[j]):
+// one counter atomic per wave; entries past the capacity are handed back at once (and
+// the launch ends after this level)
+template <int U>
+__device__ __forceinline__ void eq_add_n(const bool *on, const int *w, const int *q0, const int *q1, const uint64_t *m, EQ out,
+                                         int *en, int ecap, uint64_t *F, int *Q, PCtl *pc, long long &mf, int &mx) {
+    uint64_t bm[U];
+    int tot = 0;
+#pragma unroll
+    for (int j = 0; j < U; ++j) { bm[j] = ballot(on[j]); tot += __builtin_popcountll(bm[j]); }
+    if (tot == 0) return;
+    int base = 0;
+    if (lane() == 0) base = atomicAdd(en, tot);
+    base = readlane(base, 0);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        const int k = base + mbcnt(bm[j]);
+        const bool fits = on[j] && k < ecap;
+        if (fits) {
+            out.ea[k] = (uint64_t)(uint32_t)w[j] | ((uint64_t)(uint32_t)q0[j] << 32);
+            out.eb[k] = (uint64_t)(uint32_t)q1[j];
+            out.ec[k] = m[j];
+        }
+        if (on[j]) {
+            mf += q1[j] - q0[j];
+            mx = max(mx, q1[j] - q0[j]);
+        }
+        const bool spill = on[j] && !fits;
+        if (ballot(spill)) {
+            if (spill) atomicExch(&pc->over, 1);
+            eq_handback(spill, w[j], q1[j] - q0[j], m[j], F, Q, pc);
+        }
+        base += __builtin_popcountll(bm[j]);
+    }
+}
+
+// one level: every entry's edges, claims on the visited words, level rows, next entries
+// (wave wv of nw walks entries wv * 64 ..)
+__device__ __forceinline__ void eq_bfs_level(int it, int n, int wv, int nw, const int *__restrict__ row, const int *__restrict__ col,
+                                             uint64_t *vis, __amdgpu_buffer_rsrc_t ra, __amdgpu_buffer_rsrc_t rb,
+                                             __amdgpu_buffer_rsrc_t rc, EQ out, int *en, int ecap, uint64_t *F, int *Q, PCtl *pc,
+                                             int *__restrict__ lev, long long &mf, int &mx) {
+    for (int base = wv * MG_WAVE; base < n; base += nw * MG_WAVE) {
+        const int i = base + lane();
+        int p0 = 0, p1 = 0;
+        uint64_t fu = 0;
+        if (i < n) {
+            p0 = (int)(ld_l2(ra, i) >> 32);
+            p1 = (int)ld_l2(rb, i);
+            fu = ld_l2(rc, i);
+        }
+        const bool alone = fu && p1 - p0 <= kLaneWork;
+        int steps = alone ? p1 - p0 : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
+        // kVisit edges per step: the claims go out with the new vertices' row loads
+        for (int k0 = 0; k0 < steps; k0 += kVisit) {
+            int w[kVisit], q0[kVisit], q1[kVisit];
+            uint64_t m[kVisit];
+            bool on[kVisit];
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) w[j] = alone && p0 + k0 + j < p1 ? col[p0 + k0 + j] : -1;
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) {
+                m[j] = w[j] >= 0 ? fu & ~(uint64_t)atomicOr((unsigned long long *)&vis[w[j]], (unsigned long long)fu) : 0ull;
+                q0[j] = w[j] >= 0 ? row[w[j]] : 0;
+                q1[j] = w[j] >= 0 ? row[w[j] + 1] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kVisit; ++j) {
+                on[j] = m[j] != 0ull;
+                for (uint64_t bits = m[j]; bits; bits &= bits - 1) lev[(long long)w[j] * 64 + __builtin_ctzll(bits)] = it;
+            }
+            eq_add_n<kVisit>(on, w, q0, q1, m, out, en, ecap, F, Q, pc, mf, mx);
+        }
+        uint64_t heavy = ballot(fu && !alone);
+        while (heavy) {  // the whole wave over one heavier vertex's edges
+            const int l = __builtin_ctzll(heavy);
+            heavy &= heavy - 1;
+            const int r0 = __shfl(p0, l), r1 = __shfl(p1, l);
+            const uint64_t fv = (uint64_t)__shfl((long long)fu, l);
+            for (int pb = r0; pb < r1; pb += MG_WAVE) {
+                const int pp = pb + lane();
+                int w = -1, q0 = 0, q1 = 0;
+                uint64_t m = 0ull;
+                if (pp < r1) {
+                    w = col[pp];
+                    m = fv & ~(uint64_t)atomicOr((unsigned long long *)&vis[w], (unsigned long long)fv);
+                    q0 = row[w];
+                    q1 = row[w + 1];
+                }
+                bool on = m != 0ull;
+                for (uint64_t bits = m; bits; bits &= bits - 1) lev[(long long)w * 64 + __builtin_ctzll(bits)] = it;
+                eq_add_n<1>(&on, &w, &q0, &q1, &m, out, en, ecap, F, Q, pc, mf, mx);
+            }
+        }
+    }
+}
+
+struct EqArgs {
+    BCtl *ctl;
+    PCtl *pc;
+    LevelRule rule;
+    const int *row, *col;
+    uint64_t *FA, *FB, *vis;
+    int *QA, *QB;
+    int *lev;
+    int *part_n;
+    long long *part_e;
+    int *part_x;
+    EQ q[2];
+    int ecap, nparts, max_levels;
+};
+
+// Persistent sparse BFS levels (the contract of k_mbs_persist): entered after a sparse
+// level of the captured graphs (frontier words F[L & 1] and queue Q[L & 1]), left with
+// the last level's frontier in the same form and the control block updated.
+__global__ __launch_bounds__(kBB) void k_mbp_bfs(EqArgs a) {
+    __shared__ long long s_tmp[kBB / MG_WAVE];
+    __shared__ long long s_v[4];
+    long long n, mf, mx;
+    {
+        long long x = 0, y = 0, z = 0;
+        for (int i = threadIdx.x; i < a.nparts; i += kBB) { x += a.part_n[i]; y += a.part_e[i]; z = max(z, (long long)a.part_x[i]); }
+        n = block_sum<kBB / MG_WAVE>(x, s_tmp);
+        mf = block_sum<kBB / MG_WAVE>(y, s_tmp);
+        mx = block_max<kBB / MG_WAVE>(z, s_tmp);
+    }
+    const BCtl c0 = *a.ctl;
+    int L = c0.it;
+    // the XCD's workgroups: block 0 publishes its XCC id, the workgroups on that XCC take
+    // ranks, and every workgroup counts in on `reg` (all bounded waits, across XCDs)
+    if (threadIdx.x == 0) {
+        const unsigned x = (__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((16 - 1) << 11)) & 0xFu) + 1u;
+        if (blockIdx.x == 0) atomicExch(&a.pc->xcc0, x);
+        const long long t0 = wall_clock64();
+        unsigned x0;
+        while ((x0 = ld_atomic(&a.pc->xcc0)) == 0u && wall_clock64() - t0 < kSpinTicks) __builtin_amdgcn_s_sleep(1);
+        const int rank = x0 == x ? atomicAdd(&a.pc->npart, 1) : -1;
+        __builtin_amdgcn_s_waitcnt(0);
+        atomicAdd(&a.pc->reg, 1u);
+        int P = 0;
+        if (rank >= 0) {
+            while (ld_atomic(&a.pc->reg) < gridDim.x && wall_clock64() - t0 < kSpinTicks) __builtin_amdgcn_s_sleep(1);
+            if (ld_atomic(&a.pc->reg) < gridDim.x) atomicExch(&a.pc->err, 1);
+            P = ld_atomic(&a.pc->npart);
+        }
+        s_v[0] = rank;
+        s_v[1] = P;
+    }
+    __syncthreads();
+    const int rank = (int)s_v[0], P = (int)s_v[1];
+    if (rank < 0) return;
+    auto next_block = [&](int Lv) {
+        BCtl c = c0;
+        c.it = Lv;
+        c.qvalid[Lv & 1] = 1;
+        c.prev_sparse = 1;
+        decide(c, n, mf, mx, a.rule);
+        return c;
+    };
+    const long long small = (long long)kXcdWork * P;
+    BCtl c = next_block(L);
+    if (c.done || !c.sparse || c.compact || c.fill || n + mf > small || a.max_levels <= 0) {
+        if (rank == 0 && threadIdx.x == 0) {  // nothing run: the state is the captured graph's
+            if (c.done) { a.ctl->done = 1; a.ctl->n = 0; }
+            else if (c.sparse && !c.compact && !c.fill) a.ctl->big = 1;  // a sparse level, too large for one XCD
+            else a.ctl->need_full = 1;
+        }
+        return;
+    }
+    unsigned gen = 0;
+    const int gt = rank * kBB + threadIdx.x, gn = P * kBB;
+    const int wv = gt >> 6, nw = gn >> 6;
+    {  // the queue of level L as entries (its words cleared: both word buffers end clean)
+        uint64_t *F = (L & 1) ? a.FB : a.FA;
+        const int *Q = (L & 1) ? a.QB : a.QA;
+        const EQ e = a.q[L & 1];
+        for (int i = gt; i < (int)n; i += gn) {
+            const int w = Q[i];
+            const uint64_t bits = atomicExch((unsigned long long *)&F[w], 0ull);
+            e.ea[i] = (uint64_t)(uint32_t)w | ((uint64_t)(uint32_t)a.row[w] << 32);
+            e.eb[i] = (uint64_t)(uint32_t)a.row[w + 1];
+            e.ec[i] = bits;
+        }
+    }
+    bool ok = xcd_sync(a.pc, gen, P);
+    const long long qbytes = (long long)a.ecap * 8;
+    int ran = 0;
+    bool handback = false, over = false;
+    while (ok) {
+        const int it = L + 1, slot = it % 3;
+        if (rank == 0 && threadIdx.x == 0) {  // the counters of the level after
+            const int ns = (it + 1) % 3;
+            atomicExch(&a.pc->qn[ns], 0);
+            atomicExch(&a.pc->mx[ns], 0);
+            atomicExch((unsigned long long *)&a.pc->mf[ns], 0ull);
+        }
+        long long sm = 0;
+        int xm = 0;
+        uint64_t *F = (it & 1) ? a.FB : a.FA;  // spills: the level's frontier words and queue
+        int *Q = (it & 1) ? a.QB : a.QA;
+        const EQ in = a.q[L & 1];
+        eq_bfs_level(it, (int)min(n, (long long)a.ecap), wv, nw, a.row, a.col, a.vis, l2_rsrc(in.ea, qbytes), l2_rsrc(in.eb, qbytes),
+                     l2_rsrc(in.ec, qbytes), a.q[it & 1], &a.pc->qn[slot], a.ecap, F, Q, a.pc, a.lev, sm, xm);
+        const long long bs = block_sum<kBB / MG_WAVE>(sm, s_tmp);
+        const long long bx = block_max<kBB / MG_WAVE>((long long)xm, s_tmp);
+        if (threadIdx.x == 0) {
+            if (bs) atomicAdd((unsigned long long *)&a.pc->mf[slot], (unsigned long long)bs);
+            if (bx) atomicMax(&a.pc->mx[slot], (int)bx);
+        }
+        ok = xcd_sync(a.pc, gen, P);
+        if (threadIdx.x == 0) {
+            s_v[0] = ld_atomic(&a.pc->qn[slot]);
+            s_v[1] = (long long)ld_atomic((unsigned long long *)&a.pc->mf[slot]);
+            s_v[2] = ld_atomic(&a.pc->mx[slot]);
+            s_v[3] = ld_atomic(&a.pc->over);
+        }
+        __syncthreads();
+        n = s_v[0];
+        mf = s_v[1];
+        mx = s_v[2];
+        over = s_v[3] != 0;
+        __syncthreads();
+        L = it;
+        ++ran;
+        if (over || !ok) break;
+        c = next_block(L);
+        if (c.done) break;
+        if (!c.sparse || c.compact || c.fill || n + mf > small) { handback = true; break; }
+        if (ran >= a.max_levels) break;
+    }
+    // hand the last level's entries back as frontier words and queue (spilled entries
+    // are there already)
+    {
+        uint64_t *F = (L & 1) ? a.FB : a.FA;
+        int *Q = (L & 1) ? a.QB : a.QA;
+        const EQ e = a.q[L & 1];
+        const __amdgpu_buffer_rsrc_t ra = l2_rsrc(e.ea, qbytes), rb = l2_rsrc(e.eb, qbytes), rc = l2_rsrc(e.ec, qbytes);
+        const int ne = (int)min(n, (long long)a.ecap);
+        for (int b0 = wv * MG_WAVE; b0 < ne; b0 += nw * MG_WAVE) {
+            const int i = b0 + lane();
+            int w = 0, deg = 0;
+            uint64_t bits = 0;
+            if (i < ne) {
+                const uint64_t x = ld_l2(ra, i);
+                w = (int)(uint32_t)x;
+                deg = (int)ld_l2(rb, i) - (int)(x >> 32);
+                bits = ld_l2(rc, i);
+            }
+            eq_handback(i < ne, w, deg, bits, F, Q, a.pc);
+        }
+    }
+    xcd_sync(a.pc, gen, P);
+    if (threadIdx.x == 0) {
+        s_v[0] = ld_atomic(&a.pc->fq);
+        s_v[1] = (long long)ld_atomic((unsigned long long *)&a.pc->fmf);
+        s_v[2] = ld_atomic(&a.pc->fmx);
+        s_v[3] = ld_atomic(&a.pc->err);
+    }
+    __syncthreads();
+    const int qn = (int)s_v[0];
+    const long long qf = s_v[1];
+    const int qx = (int)s_v[2];
+    for (int i = gt; i < a.nparts; i += gn) {
+        a.part_n[i] = i == 0 ? qn : 0;
+        a.part_e[i] = i == 0 ? qf : 0;
+        a.part_x[i] = i == 0 ? qx : 0;
+    }
+    if (rank == 0 && threadIdx.x == 0) {
+        BCtl o = c0;
+        o.it = L;
+        o.n = qn;
+        o.mf = qf;
+        o.push = 1;
+        o.sparse = 1;
+        o.compact = 0;
+        o.fill = 0;
+        o.prev_sparse = 1;
+        o.qn[L & 1] = qn;
+        o.qvalid[L & 1] = 1;
+        o.done = qn == 0;
+        o.need_full = (handback || over) && qn > 0 ? 1 : 0;
+        o.nsparse = c0.nsparse + ran;
+        o.npersist = c0.npersist + ran;
+        o.err = s_v[3] != 0;
+        o.big = 0;
         *a.ctl = o;
     }
 }
@@ -1416,6 +1792,8 @@ struct mg_batch {
     int gexec_levels = 0;
     PCtl *pc = nullptr;           // counters of the persistent sparse levels
     int persist_grid = 0;         // workgroups of its cooperative launch (0: not available)
+    uint64_t *eq = nullptr;       // BFS: the entry queues of the persistent levels (2 x 3 x ecap words)
+    int ecap = 0;
     long long stats[5] = {0, 0, 0, 0, 0};  // of the last BFS / SSSP run (mg_batch_stats)
     int auto_delta = 0;           // SSSP near-far band by default: a quarter of the mean weight
     int mirror_seq = 0;           // host checks of the control block so far (k_mb_mirror)
@@ -1430,7 +1808,7 @@ static void batch_free(mg_batch *b) {
     if (b->gsparse_long) (void)hipGraphExecDestroy(b->gsparse_long);
     void *ptrs[] = {b->Q[0], b->Q[1], b->row, b->col, b->w, b->vis, b->F[0], b->F[1], b->D, b->E, b->lev, b->dist, b->src, b->slot, b->sigma,
                     b->delta, b->bc, b->chunks, b->logv, b->logw, b->cw, b->part_n, b->part_x, b->part_e, b->tot, b->ctl, b->pc,
-                    b->tchunks, b->trow, b->tcol, b->tw, b->hchunks, b->dlog, b->dlev, b->perm, b->mv, b->ctl2, b->fst};
+                    b->tchunks, b->trow, b->tcol, b->tw, b->hchunks, b->dlog, b->dlev, b->perm, b->mv, b->ctl2, b->fst, b->eq};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (b->h_tot) (void)hipHostFree(b->h_tot);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1569,10 +1947,15 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     }
     if (kind != MG_BATCH_BC && b->opt.persist_levels != 0) {  // one resident workgroup per CU
         int dev = 0, cus = 0, per = 0;
-        const void *fn = kind == MG_BATCH_SSSP ? (const void *)k_mbs_persist<true> : (const void *)k_mbs_persist<false>;
+        const void *fn = kind == MG_BATCH_SSSP ? (const void *)k_mbs_persist<true> : (const void *)k_mbp_bfs;
         if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kBB, 0) == hipSuccess && per > 0 && cus > 0)
             b->persist_grid = cus * std::min(per, MG_PERSIST_PER_CU);
+        if (b->persist_grid > 0 && kind == MG_BATCH_BFS) {
+            b->ecap = std::min<int>(nv + 1024, 1 << 27);  // entry words are addressed with 31-bit byte offsets
+            if (hipMalloc(&b->eq, sizeof(uint64_t) * 6 * (size_t)b->ecap) != hipSuccess)
+                return fail(mg_fail(MG_ERR_NOMEM, "batch: entry queues of the persistent levels"));
+        }
     }
     if (hipMemcpy(b->row, g->row.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice) != hipSuccess ||
         (g->ne > 0 && hipMemcpy(b->col, g->col.data(), sizeof(int) * g->ne, hipMemcpyHostToDevice) != hipSuccess) ||
@@ -1823,17 +2206,28 @@ static int run_levels(mg_batch *b) {
     const int max_levels = b->opt.persist_levels > 0 ? b->opt.persist_levels : (1 << 30);  // -1: unbounded
     long long trips = 0;
     bool was_sparse = false;  // the last replay was the sparse-only graph and ran on
+    bool skip_persist = false;  // the last persistent launch found the frontier too large
     for (;;) {
         // after a sparse level the persistent kernel (or the sparse-only graph) runs on
         // until a level needs more
         const bool sparse_next = b->h_ctl->it > 0 && b->h_ctl->sparse && !b->h_ctl->need_full;
-        if (sparse_next && b->persist_grid > 0) {
-            PersistArgs pa{b->ctl, b->pc, rule_of(b), b->row, b->col, b->w, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1],
-                           sssp ? b->dist : b->lev, b->part_n, b->part_e, b->part_x, gv, max_levels};
-            void *args[] = {&pa};
-            hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(256), 0, s, (int *)&b->pc->top, (long long)(16 + 8 * 16), 0);
-            const void *fn = sssp ? (const void *)k_mbs_persist<true> : (const void *)k_mbs_persist<false>;
-            MG_HIP(hipLaunchCooperativeKernel(fn, dim3(b->persist_grid), dim3(kBB), args, 0, s));
+        const bool persist = sparse_next && b->persist_grid > 0 && !skip_persist;
+        if (persist) {
+            hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(256), 0, s, (int *)b->pc, (long long)(sizeof(PCtl) / sizeof(int)), 0);
+            if (sssp) {
+                PersistArgs pa{b->ctl, b->pc, rule_of(b), b->row, b->col, b->w, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1],
+                               b->dist, b->part_n, b->part_e, b->part_x, gv, max_levels};
+                void *args[] = {&pa};
+                MG_HIP(hipLaunchCooperativeKernel((const void *)k_mbs_persist<true>, dim3(b->persist_grid), dim3(kBB), args, 0, s));
+            } else {
+                const size_t c = (size_t)b->ecap;
+                EqArgs ea{b->ctl, b->pc, rule_of(b), b->row, b->col, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1], b->lev,
+                          b->part_n, b->part_e, b->part_x,
+                          {EQ{b->eq, b->eq + c, b->eq + 2 * c}, EQ{b->eq + 3 * c, b->eq + 4 * c, b->eq + 5 * c}},
+                          b->ecap, gv, max_levels};
+                void *args[] = {&ea};
+                MG_HIP(hipLaunchCooperativeKernel((const void *)k_mbp_bfs, dim3(b->persist_grid), dim3(kBB), args, 0, s));
+            }
         } else {
             // a tail that outlasted one short sparse replay gets the long one: fewer
             // host checks; a short tail does not pay the long graph's idle nodes
@@ -1857,6 +2251,9 @@ static int run_levels(mg_batch *b) {
                 }
         }
         ++trips;
+        if (b->h_ctl->err) return mg_fail(MG_ERR_DEVICE, "persistent levels: a barrier wait passed its bound");
+        // too large for the persistent launch: one replay of the sparse-only graph first
+        skip_persist = persist && b->h_ctl->big;
         if (b->h_ctl->done) break;
     }
     if (!sssp && b->dlog)
