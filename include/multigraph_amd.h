@@ -74,14 +74,16 @@ typedef struct mg_options {
                                BFS / SSSP push levels walk a queue of the frontier when
                                its vertices + edges < (nv + E) / beta                   */
     int    huge_degree;     /* vertices at or above this degree are split across blocks */
-    int    grid_blocks;     /* workgroups of the grid-stride frontier kernels           */
+    int    grid_blocks;     /* workgroups of the grid-stride frontier kernels (0, the
+                               default: 2048; batched runs: 4096, also their maximum)   */
     int    graph_levels;    /* levels / iterations captured per hipGraph launch (0: BFS,
                                SSSP, BC step eagerly; PageRank replays 8 per host check) */
     float  pr_damping;      /* PageRank damping (0.85, common.h:59)                     */
     float  pr_tolerance;    /* PageRank tolerance (0.01, common.h:60)                   */
     int    pr_max_iter;     /* PageRank iteration cap (0: unbounded, as the reference)  */
     int    part_log2;       /* log2 of the vertex-partition size of the tiled layout    */
-    int    batch_chunk;     /* batched runs: edges per static chunk (0: 1024)           */
+    int    batch_chunk;     /* batched runs: edges per static chunk (0: 1024 for BFS,
+                               128 for BC and SSSP)                                     */
     int    persist_levels;  /* batched BFS / SSSP: run sparse levels in a persistent
                                cooperative launch, at most this many per launch (-1:
                                unbounded; 0, the default: every level from the captured

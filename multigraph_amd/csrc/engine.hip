@@ -19,7 +19,7 @@ extern "C" void mg_default_options(mg_options *o) {
     o->alpha = 15.0f;
     o->beta = 18.0f;
     o->huge_degree = 8192;
-    o->grid_blocks = 2048;
+    o->grid_blocks = 0;        // 2048 for single-source engines, 4096 for batched runs
     o->graph_levels = 0;
     o->pr_damping = 0.85f;     // common.h:59
     o->pr_tolerance = 0.01f;   // common.h:60
@@ -34,7 +34,7 @@ static void fill_defaults(mg_options *o) {
     if (o->beta <= 0) o->beta = d.beta;
     if (o->huge_degree <= 0) o->huge_degree = d.huge_degree;
     if (o->huge_degree < kHugeChunk) o->huge_degree = kHugeChunk;
-    if (o->grid_blocks <= 0) o->grid_blocks = d.grid_blocks;
+    if (o->grid_blocks <= 0) o->grid_blocks = 2048;
     if (o->pr_damping <= 0) o->pr_damping = d.pr_damping;
     if (o->pr_tolerance <= 0) o->pr_tolerance = d.pr_tolerance;
     if (o->lane_edges <= 0) o->lane_edges = 32;

@@ -1,6 +1,6 @@
 // hbm_copy: streaming kernels over buffers of known size, timed with HIP events.
 //
-//   hbm_copy [MiB per buffer, default 1024] [repetitions, default 10] [sweep: 0 or 1, default 1]
+//   hbm_copy [MiB per buffer, default 1024] [repetitions, default 10] [sweep: 0, 1 or 2, default 1]
 //
 // k_copy (a plain grid-stride float4 copy, 4096 x 256 threads) runs first, `reps`
 // times: it calibrates what the HBM traffic counters read for known byte counts.
@@ -14,6 +14,7 @@
 //   k_write_nt<U>  nontemporal 16-byte stores only (write bandwidth alone).
 // These bound what a streaming kernel of the engine can reach on this chip: the
 // engine's gathers are mostly reads, so the read ceiling matters as much as the copy's.
+// With sweep = 2, the per-workgroup chunk kernels below run instead of the tuned sweep.
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdlib>
@@ -72,6 +73,45 @@ __global__ void k_write_nt(f32x4 *__restrict__ dst, long long n, float x) {
     }
 }
 
+// k_read_chunks<U> / k_write_chunks<U>: workgroup b streams its own contiguous chunk
+// (the buffer cut into gridDim.x * rounds chunks; chunk r * gridDim.x + b in round r),
+// the access pattern of the partitioned-layout gathers and scatters (one workgroup per
+// partition, each on its own region), against the grid-stride kernels above
+template <int U>
+__global__ void k_read_chunks(const f32x4 *__restrict__ src, float *__restrict__ out, long long n, int rounds) {
+    const long long csz = n / ((long long)gridDim.x * rounds);
+    float acc = 0.f;
+    for (int r = 0; r < rounds; ++r) {
+        const long long b0 = ((long long)r * gridDim.x + blockIdx.x) * csz, b1 = b0 + csz;
+        for (long long t0 = b0; t0 < b1; t0 += (long long)blockDim.x * U) {
+            f32x4 v[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const long long i = t0 + (long long)j * blockDim.x + threadIdx.x;
+                v[j] = i < b1 ? __builtin_nontemporal_load(&src[i]) : f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) acc += v[j].x + v[j].y + v[j].z + v[j].w;
+        }
+    }
+    if (acc == 12345.f) out[blockIdx.x] = acc;
+}
+
+template <int U>
+__global__ void k_write_chunks(f32x4 *__restrict__ dst, long long n, int rounds, float x) {
+    const long long csz = n / ((long long)gridDim.x * rounds);
+    for (int r = 0; r < rounds; ++r) {
+        const long long b0 = ((long long)r * gridDim.x + blockIdx.x) * csz, b1 = b0 + csz;
+        for (long long t0 = b0; t0 < b1; t0 += (long long)blockDim.x * U) {
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const long long i = t0 + (long long)j * blockDim.x + threadIdx.x;
+                if (i < b1) __builtin_nontemporal_store(f32x4{x, x, x, x}, &dst[i]);
+            }
+        }
+    }
+}
+
 #define CHECK(x)                                                                        \
     do {                                                                                \
         hipError_t e_ = (x);                                                            \
@@ -113,7 +153,7 @@ int main(int argc, char **argv) {
     const long long mib = argc > 1 ? atoll(argv[1]) : 1024;
     const int reps = argc > 2 ? atoi(argv[2]) : 10;
     const int do_sweep = argc > 3 ? atoi(argv[3]) : 1;
-    if (mib < 1 || reps < 1) { fprintf(stderr, "usage: hbm_copy [MiB] [reps] [sweep 0|1]\n"); return 2; }
+    if (mib < 1 || reps < 1) { fprintf(stderr, "usage: hbm_copy [MiB] [reps] [sweep 0|1|2]\n"); return 2; }
     const long long bytes = mib << 20;
     const long long n = bytes / 16;
     float4 *a = nullptr, *b = nullptr;
@@ -143,7 +183,7 @@ int main(int argc, char **argv) {
     printf("k_copy: %lld bytes read + %lld bytes written per launch, %d launches, avg %.4f ms, best %.4f ms, "
            "%.1f GB/s (avg), %.1f GB/s (best)\n",
            bytes, bytes, reps + 1, total / reps, best, 2.0 * bytes / (total / reps * 1e-3) / 1e9, 2.0 * bytes / (best * 1e-3) / 1e9);
-    if (do_sweep) {
+    if (do_sweep == 1) {
         const f32x4 *s4 = (const f32x4 *)a;
         f32x4 *d4 = (f32x4 *)b;
 #define SWEEP_U(U)                                                                                                    \
@@ -158,6 +198,31 @@ int main(int argc, char **argv) {
         SWEEP_U(4)
         SWEEP_U(8)
 #undef SWEEP_U
+    }
+    if (do_sweep == 2) {  // one chunk per workgroup: 256 / 512 / 768 workgroups of 1024 threads, 1 or 3 rounds
+        const f32x4 *s4 = (const f32x4 *)a;
+        f32x4 *d4 = (f32x4 *)b;
+        for (int g : {256, 512, 768})
+            for (int rounds : {1, 3}) {
+                float best_r = 1e30f, best_w = 1e30f;
+                for (int k = 0; k <= reps; ++k) {  // the first launch warms up
+                    float ms = 0.f;
+                    CHECK(hipEventRecord(e0, 0));
+                    hipLaunchKernelGGL(k_read_chunks<4>, dim3(g), dim3(1024), 0, 0, s4, o, n, rounds);
+                    CHECK(hipEventRecord(e1, 0));
+                    CHECK(hipEventSynchronize(e1));
+                    CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    if (k > 0 && ms < best_r) best_r = ms;
+                    CHECK(hipEventRecord(e0, 0));
+                    hipLaunchKernelGGL(k_write_chunks<4>, dim3(g), dim3(1024), 0, 0, d4, n, rounds, 0.f);
+                    CHECK(hipEventRecord(e1, 0));
+                    CHECK(hipEventSynchronize(e1));
+                    CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    if (k > 0 && ms < best_w) best_w = ms;
+                }
+                printf("chunks: %d workgroups x 1024 threads, %d round(s): read %.1f GB/s, write %.1f GB/s (best of %d)\n", g, rounds,
+                       bytes / (best_r * 1e-3) / 1e9, bytes / (best_w * 1e-3) / 1e9, reps);
+            }
     }
     CHECK(hipFree(a));
     CHECK(hipFree(b));

@@ -15,7 +15,7 @@ def test_autotune_engine(mg, orc):
         opt, t0, t1 = mg.autotune(g, algo, src, reps=1)
         assert t1 <= t0 + 1e-6
         assert opt.alpha in (4, 8, 15, 30, 60) and opt.beta in (6, 12, 18, 36, 72)
-        assert opt.grid_blocks in (1024, 2048, 4096) and opt.graph_levels in (0, 4, 16)
+        assert opt.grid_blocks in (0, 1024, 2048, 4096) and opt.graph_levels in (0, 4, 16)
         # bin edges and block shape (0: the default, kept when no candidate was faster)
         assert opt.huge_degree in (4096, 8192, 16384, 32768) and opt.block_threads in (0, 256, 512, 1024)
         if algo != mg.SSSP:
@@ -63,7 +63,7 @@ def test_autotune_batch_union(mg, orc):
         off += p.nv
     for kind in (mg.BATCH_BFS, mg.BATCH_SSSP):
         opt, t0, t1 = mg.batch_autotune(u, kind, src, slots, reps=1)
-        assert t1 <= t0 + 1e-6 and opt.batch_chunk in (0, 256, 512, 1024, 4096)
+        assert t1 <= t0 + 1e-6 and opt.batch_chunk in (0, 128, 256, 512, 1024, 4096)
         if kind == mg.BATCH_BFS:
             assert opt.batch_light in (0, 16, 32, 64)
         assert opt.sparse_grid in (0, 64, 128)

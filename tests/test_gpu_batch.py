@@ -96,6 +96,32 @@ def test_batch_sssp(mg, orc, gname):
     b.close()
 
 
+@pytest.mark.parametrize("opts", [dict(), dict(mode=2), dict(mode=2, sssp_delta=-1)], ids=["hybrid", "pull", "pull-bf"])
+def test_batch_sssp_wide_distances(mg, orc, opts):
+    """pull sweeps gather 16-bit copies of the distance rows until a finite distance
+    above 65534 is met, then 32-bit rows for the rest of the run: weights up to 30000
+    pass 65534 within a few hops, and a 1,500-vertex path of weight-64 edges does it
+    after ~1,000 hops, mid-run; every row as the oracle's"""
+    rng = np.random.default_rng(43)
+    n, k = 2000, 1500
+    src = np.concatenate([rng.integers(0, n, 16000), np.arange(n, n + k - 1), [0]])
+    dst = np.concatenate([rng.integers(0, n, 16000), np.arange(n + 1, n + k), [n]])
+    val = np.concatenate([rng.integers(1, 30001, 16000), np.full(k - 1, 64), [5]])
+    g = mg.Graph.from_coo(n + k, src, dst, val)
+    row, col, w = g.csr()
+    sources = np.concatenate([[0, g.nv - 1, 7, n + 3], _sources(g, 60, 44)]).astype(np.int32)
+    b = mg.Batch(g, mg.BATCH_SSSP, **opts)
+    b.run(sources)
+    d = b.levels()
+    wide = False
+    for i, s in enumerate(sources):
+        ref, _ = orc.sssp_bf(row, col, w, int(s))
+        wide |= bool(ref[ref != 0x64646464].max() > 65535)
+        np.testing.assert_array_equal(d[i], ref, err_msg=f"{opts} source {i}={s}")
+    assert wide
+    b.close()
+
+
 def _fan_in(mg, n=3000):
     """0 -> 1..n -> n+1 -> n+2, directed: the hub n+1 has n in-edges (several
     1024-edge in-chunks, merged by the pulls) and one out-edge"""
