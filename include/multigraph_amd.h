@@ -69,13 +69,16 @@ typedef struct mg_engine mg_engine;   /* device layout + run state (struct HYB +
 /* Tunables of one engine.  Zero fields take the defaults; the autotuner fills them. */
 typedef struct mg_options {
     int    mode;            /* MG_MODE_*                                                */
-    float  alpha;           /* push->pull when frontier edges > unexplored edges / alpha */
+    float  alpha;           /* push->pull when frontier edges > unexplored edges / alpha
+                               (0, the default: 30 for engines, 15 for batched runs)   */
     float  beta;            /* pull->push when frontier vertices < nv / beta; batched
                                BFS / SSSP push levels walk a queue of the frontier when
-                               its vertices + edges < (nv + E) / beta                   */
+                               its vertices + edges < (nv + E) / beta (0, the default:
+                               36 for engines, 18 for batched runs)                     */
     int    huge_degree;     /* vertices at or above this degree are split across blocks */
     int    grid_blocks;     /* workgroups of the grid-stride frontier kernels (0, the
-                               default: 2048; batched runs: 4096, also their maximum)   */
+                               default: 2048, BC and CC engines 4096; batched runs:
+                               4096, also their maximum)                                */
     int    graph_levels;    /* levels / iterations captured per hipGraph launch (0: BFS,
                                SSSP, BC step eagerly; PageRank replays 8 per host check) */
     float  pr_damping;      /* PageRank damping (0.85, common.h:59)                     */

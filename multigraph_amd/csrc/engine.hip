@@ -16,10 +16,10 @@ extern "C" void mg_default_options(mg_options *o) {
     if (!o) return;
     memset(o, 0, sizeof(*o));
     o->mode = MG_MODE_HYBRID;
-    o->alpha = 15.0f;
-    o->beta = 18.0f;
+    o->alpha = 0.0f;           // 0: the path's own (kAlpha / kBeta for engines, 15 / 18 for batched runs)
+    o->beta = 0.0f;
     o->huge_degree = 8192;
-    o->grid_blocks = 0;        // 2048 for single-source engines, 4096 for batched runs
+    o->grid_blocks = 0;        // 2048 for single-source engines (4096 for BC and CC), 4096 for batched runs
     o->graph_levels = 0;
     o->pr_damping = 0.85f;     // common.h:59
     o->pr_tolerance = 0.01f;   // common.h:60
@@ -27,14 +27,17 @@ extern "C" void mg_default_options(mg_options *o) {
     o->part_log2 = 0;          // automatic
 }
 
-static void fill_defaults(mg_options *o) {
+static void fill_defaults(mg_options *o, int algo) {
     mg_options d;
     mg_default_options(&d);
-    if (o->alpha <= 0) o->alpha = d.alpha;
-    if (o->beta <= 0) o->beta = d.beta;
+    if (o->alpha <= 0) o->alpha = kAlpha;
+    if (o->beta <= 0) o->beta = kBeta;
     if (o->huge_degree <= 0) o->huge_degree = d.huge_degree;
     if (o->huge_degree < kHugeChunk) o->huge_degree = kHugeChunk;
-    if (o->grid_blocks <= 0) o->grid_blocks = 2048;
+    // BC's pulls and backward tiers and CC's chunk hooks fill a 4096-workgroup grid
+    // better (RMAT-22: BC from 8 random sources 1.79 -> 1.70 ms, CC 0.141 -> 0.139 ms);
+    // BFS and SSSP levels are a little faster on 2048
+    if (o->grid_blocks <= 0) o->grid_blocks = (algo == MG_BC || algo == MG_CC) ? 4096 : 2048;
     if (o->pr_damping <= 0) o->pr_damping = d.pr_damping;
     if (o->pr_tolerance <= 0) o->pr_tolerance = d.pr_tolerance;
     if (o->lane_edges <= 0) o->lane_edges = 32;
@@ -85,7 +88,7 @@ extern "C" int mg_build(const mg_graph *g, int algo, const mg_options *opt, mg_e
     mg_engine *e = new mg_engine();
     e->algo = algo;
     if (opt) e->opt = *opt; else mg_default_options(&e->opt);
-    fill_defaults(&e->opt);
+    fill_defaults(&e->opt, algo);
     e->nv = g->nv;
     e->ne = g->ne;
     auto fail = [&](int rc) { release_all(e); return rc; };

@@ -1628,8 +1628,8 @@ extern "C" int mg_bfs_noalloc(int nv, int64_t ne, const int32_t *row, const int3
     e->h_ctl = &hc;
     e->mirror = false;
     e->opt.mode = mode;
-    e->opt.alpha = 15.0f;
-    e->opt.beta = 18.0f;
+    e->opt.alpha = kAlpha;
+    e->opt.beta = kBeta;
     e->opt.grid_blocks = kNaGrid;
     e->opt.lane_edges = 32;
     e->opt.pull_split = kPullHuge;

@@ -38,7 +38,7 @@ def test_version_and_defaults(mg):
     L.mg_version.restype = C.c_char_p
     assert L.mg_version().decode().startswith("multigraph-amd")
     o = mg.default_options()
-    assert o.mode == mg.MODE_HYBRID and o.alpha == 15.0 and o.beta == 18.0
+    assert o.mode == mg.MODE_HYBRID and o.alpha == 0.0 and o.beta == 0.0 and o.grid_blocks == 0  # the paths' own
     assert abs(o.pr_damping - 0.85) < 1e-7 and abs(o.pr_tolerance - 0.01) < 1e-9   # common.h:59-60
     assert o.pr_max_iter == 0 and o.graph_levels == 0
 

@@ -14,7 +14,7 @@ def test_autotune_engine(mg, orc):
     for algo in (mg.BFS, mg.SSSP, mg.BC):
         opt, t0, t1 = mg.autotune(g, algo, src, reps=1)
         assert t1 <= t0 + 1e-6
-        assert opt.alpha in (4, 8, 15, 30, 60) and opt.beta in (6, 12, 18, 36, 72)
+        assert opt.alpha in (0, 4, 8, 15, 30, 60) and opt.beta in (0, 6, 12, 18, 36, 72)
         assert opt.grid_blocks in (0, 1024, 2048, 4096) and opt.graph_levels in (0, 4, 16)
         # bin edges and block shape (0: the default, kept when no candidate was faster)
         assert opt.huge_degree in (4096, 8192, 16384, 32768) and opt.block_threads in (0, 256, 512, 1024)
