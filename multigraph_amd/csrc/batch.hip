@@ -968,8 +968,9 @@ __global__ void k_mbs_fused_exit(int K, BCtl *ctl, const BCtl *c2, const FStat *
     }
 }
 
-// Persistent sparse levels: one cooperative launch (every workgroup resident) runs
-// consecutive sparse levels with a grid barrier between them, so a long tail of
+// Persistent sparse SSSP levels (k_mbs_persist; BFS has its own, k_mbp_bfs, below):
+// one cooperative launch (every workgroup resident) runs consecutive sparse levels
+// with a grid barrier between them, so a long tail of
 // small frontiers (the thousands of levels of a large grid in an uneven batch) costs
 // a barrier per level instead of a control kernel, a level kernel and a graph node
 // each.  It starts after a sparse level of the captured graph, decides every level
@@ -1028,14 +1029,13 @@ struct PersistArgs {
     const int *row, *col, *wgt;
     uint64_t *FA, *FB, *vis;
     int *QA, *QB;
-    int *out;  // lev (BFS) or dist (SSSP)
+    int *out;  // distances
     int *part_n;
     long long *part_e;
     int *part_x;
     int nparts, max_levels;
 };
 
-template <bool kSSSP>
 __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
     __shared__ long long s_tmp[kBB / MG_WAVE];
     unsigned gen = 0;
@@ -1074,12 +1074,8 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
             a.pc->mx[ns] = 0;
             a.pc->mf[ns] = 0;
         }
-        if (kSSSP)
-            sparse_sssp<MG_SSSP_HOOKS>(it, (int)n, c.thr, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA,
-                        cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
-        else
-            sparse_bfs(it, (int)n, a.row, a.col, cur ? a.FB : a.FA, cur ? a.FA : a.FB, a.vis, cur ? a.QB : a.QA,
-                       cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
+        sparse_sssp<MG_SSSP_HOOKS>(it, (int)n, c.thr, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA,
+                                   cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
         (void)cnt;
         const long long bs = block_sum<kBB / MG_WAVE>(sm, s_tmp);
         const long long bx = block_max<kBB / MG_WAVE>((long long)xm, s_tmp);
@@ -1954,7 +1950,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     }
     if (kind != MG_BATCH_BC && b->opt.persist_levels != 0) {  // one resident workgroup per CU
         int dev = 0, cus = 0, per = 0;
-        const void *fn = kind == MG_BATCH_SSSP ? (const void *)k_mbs_persist<true> : (const void *)k_mbp_bfs;
+        const void *fn = kind == MG_BATCH_SSSP ? (const void *)k_mbs_persist : (const void *)k_mbp_bfs;
         if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kBB, 0) == hipSuccess && per > 0 && cus > 0)
             b->persist_grid = cus * std::min(per, MG_PERSIST_PER_CU);
@@ -2225,7 +2221,7 @@ static int run_levels(mg_batch *b) {
                 PersistArgs pa{b->ctl, b->pc, rule_of(b), b->row, b->col, b->w, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1],
                                b->dist, b->part_n, b->part_e, b->part_x, gv, max_levels};
                 void *args[] = {&pa};
-                MG_HIP(hipLaunchCooperativeKernel((const void *)k_mbs_persist<true>, dim3(b->persist_grid), dim3(kBB), args, 0, s));
+                MG_HIP(hipLaunchCooperativeKernel((const void *)k_mbs_persist, dim3(b->persist_grid), dim3(kBB), args, 0, s));
             } else {
                 const size_t c = (size_t)b->ecap;
                 EqArgs ea{b->ctl, b->pc, rule_of(b), b->row, b->col, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1], b->lev,
