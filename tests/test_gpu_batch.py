@@ -297,6 +297,32 @@ def test_batch_persistent_levels(mg, orc, kind, persist):
         assert sweeps[0] == max(int(orc.bfs(row, col, s)[orc.bfs(row, col, s) != 0x64646464].max()) for s in srcs) + 1
 
 
+def test_batch_persistent_large_sparse_levels(mg, orc):
+    """persistent BFS levels run on one XCD only while a level's vertices + edges stay
+    small: an RMAT-16 member whose wide levels are made sparse (low alpha and beta)
+    sends those back to the sparse-only graph, and a 7 x 300 grid member's long tail
+    then runs in persistent launches; every slot as the oracle's, the same level count
+    as graph replay"""
+    parts = [mg.Graph.rmat(16, 16, seed=27), mg.Graph.grid(7, 300)]
+    u = mg.Graph.union(parts)
+    row, col, _ = u.csr()
+    rng = np.random.default_rng(28)
+    src = np.concatenate([rng.integers(0, parts[0].nv, 32), parts[0].nv + rng.integers(0, parts[1].nv, 32)]).astype(np.int32)
+    ref = orc.bfs_multi(row, col, src)
+    runs = {}
+    for persist in (0, -1):
+        b = mg.Batch(u, mg.BATCH_BFS, persist_levels=persist, alpha=2, beta=1.5)
+        b.run(src)
+        np.testing.assert_array_equal(b.levels(), ref, err_msg=f"persist {persist}")
+        runs[persist] = b.stats()
+        b.close()
+    print(runs, flush=True)
+    assert runs[0]["levels"] == runs[-1]["levels"]
+    assert runs[0]["persistent_levels"] == 0
+    # the grid's tail ran persistent, the RMAT member's wide sparse levels did not
+    assert 0 < runs[-1]["persistent_levels"] < runs[-1]["sparse_levels"]
+
+
 @pytest.mark.parametrize("delta", [-1, 0, 1, 16, 100000])
 @pytest.mark.parametrize("gname", ["grid", "rmat13", "union"])
 @pytest.mark.parametrize("persist", [0, -1], ids=["graph", "persistent"])
