@@ -199,14 +199,24 @@ __global__ __launch_bounds__(1024) void k_cc_sample(int nv, int *__restrict__ pa
             }
             __syncthreads();
         }
-    if (t == 0) {
-        int best = s[0], bc = 0, cur = s[0], cc = 0;
-        for (int i = 0; i < 1024; ++i) {
-            if (s[i] == cur) ++cc; else { cur = s[i]; cc = 1; }
-            if (cc > bc) { bc = cc; best = cur; }
+    // the thread at the start of each run of equal roots finds the run's end by binary
+    // search; the block keeps the longest run, the smallest root among equally long ones
+    long long key = -1;
+    if (t == 0 || s[t - 1] != s[t]) {
+        const int a = s[t];
+        int lo = t + 1, hi = 1024;  // first index past t holding a larger root
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s[mid] > a) hi = mid;
+            else lo = mid + 1;
         }
-        out[0] = best;
-        out[1] = bc;
+        key = ((long long)(lo - t) << 32) | (long long)(unsigned)(0x7FFFFFFF - a);
+    }
+    __shared__ long long s_key[1024 / MG_WAVE];
+    key = block_max<1024 / MG_WAVE>(key, s_key);
+    if (t == 0) {
+        out[0] = 0x7FFFFFFF - (int)(key & 0xFFFFFFFFll);
+        out[1] = (int)(key >> 32);
     }
 }
 

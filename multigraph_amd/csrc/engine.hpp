@@ -27,8 +27,22 @@ enum { MODE_TD = 0, MODE_BU = 1, MODE_DONE = 2 };
 constexpr int kBlock = 256;          // threads per workgroup of the frontier kernels
 constexpr int kHugeChunk = 4096;     // edges per chunk of a split high-degree vertex
 constexpr int kTile = 1024;          // frontier entries per degree-scan tile
-constexpr int kEPB = 2048;           // edges per workgroup chunk of the edge-balanced push
+constexpr int kEPB = 2048;           // edges per workgroup chunk of the edge-balanced push (at most)
+constexpr int kEPBMin = 256;         // ... and at least
 constexpr long long kSmallMF = 16384;  // frontiers with fewer edges use the single-pass push
+// Edges per chunk of the edge-balanced push for a frontier of `total` edges: enough to
+// make about 2048 chunks, rounded up to a multiple of 256, within [kEPBMin, kEPB].  A
+// hub's 163 K edges then spread over 640 workgroups instead of 80.
+__host__ __device__ constexpr int lb_epb(long long total) {
+    long long e = (total + 2047) / 2048;
+    e = (e + kEPBMin - 1) / kEPBMin * kEPBMin;
+    return (int)(e < kEPBMin ? kEPBMin : e > kEPB ? kEPB : e);
+}
+// the most chunks a frontier of at most ne edges is cut into
+__host__ __device__ constexpr long long lb_max_chunks(long long ne) {
+    const long long c = (ne + kEPB - 1) / kEPB;
+    return c > 2048 ? c : 2048;
+}
 // Direction rule of the single-source engines (mg_options alpha / beta left at 0): push
 // -> pull when the frontier's edges exceed the unexplored edges / kAlpha, pull -> push
 // when the frontier has fewer than nv / kBeta vertices.  RMAT-22 from 8 random sources

@@ -74,17 +74,38 @@ __global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf, Ctl *
     }
 }
 
-template <bool kTrackDeg, class OT>
-__device__ __forceinline__ void append(Ctl *ctl, int *q, int out_begin, const OT *row, bool win, int u) {
-    int pos = wave_append(&ctl->out_n, win);
-    if (win) q[out_begin + pos] = u;
-    if (kTrackDeg) {
-        long long d = win ? (long long)(row[u + 1] - row[u]) : 0;
-        if (ballot(win)) {
-            d = wave_sum(d);
-            if (lane() == 0) atomicAdd((unsigned long long *)&ctl->out_mf, (unsigned long long)d);
-        }
+// Appends of one step of a whole workgroup (every thread of it calls this, in step):
+// the winners' slots in the next queue come from ONE counter atomic per workgroup and
+// step, their degree sum from one more.  Atomics per wave on the shared counter
+// serialised a hub's push level (RMAT-22, 163 K winners: 59 -> 20 us).
+template <int B, bool kTrackDeg, class OT>
+__device__ __forceinline__ void block_append(Ctl *ctl, int *q, int out_begin, const OT *row, bool win, int u,
+                                             int *s_wc, long long *s_wd) {
+    constexpr int NW = B / MG_WAVE;
+    const uint64_t m = ballot(win);
+    const int wv = threadIdx.x / MG_WAVE;
+    long long d = 0;
+    if (kTrackDeg) d = wave_sum(win ? (long long)(row[u + 1] - row[u]) : 0ll);
+    if (lane() == 0) {
+        s_wc[wv] = __builtin_popcountll(m);
+        s_wd[wv] = d;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        long long dt = 0;
+        for (int i = 0; i < NW; ++i) {
+            const int c = s_wc[i];
+            s_wc[i] = tot;
+            tot += c;
+            dt += s_wd[i];
+        }
+        s_wc[NW] = tot ? atomicAdd(&ctl->out_n, tot) : 0;
+        if (kTrackDeg && dt) atomicAdd((unsigned long long *)&ctl->out_mf, (unsigned long long)dt);
+    }
+    __syncthreads();
+    if (win) q[out_begin + s_wc[NW] + s_wc[wv] + mbcnt(m)] = u;
+    __syncthreads();  // s_wc is refilled by the next step
 }
 
 // Push level: expand the frontier's edges, contract winners into the next queue.
@@ -101,6 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
     __shared__ int s_val[kBlock];
     __shared__ double s_val2[kBlock];
     __shared__ int s_tmp[kBlock / MG_WAVE + 1];
+    __shared__ int s_wc[kBlock / MG_WAVE + 1];
+    __shared__ long long s_wd[kBlock / MG_WAVE];
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const int i = base + threadIdx.x;
         int v = -1, deg = 0, val = 0;
@@ -133,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_push(Ctl *ctl, const Params P, const
                 u = ld_stream(&col[p]);
                 win = relax<H>(s_val[lo], s_val2[lo], u, p, wgt, state, vfc, sigma, iter);
             }
-            append<kTrackDeg>(ctl, q, out_begin, row, win != 0, u);
+            block_append<kBlock, kTrackDeg>(ctl, q, out_begin, row, win != 0, u, s_wc, s_wd);
         }
         __syncthreads();
     }
@@ -191,8 +214,9 @@ __global__ __launch_bounds__(1024) void k_lb_tscan(Ctl *ctl, int *__restrict__ t
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        const int epb = lb_epb(s_carry);
         ctl->lb_total = s_carry;
-        ctl->lb_chunks = (s_carry + kEPB - 1) / kEPB;
+        ctl->lb_chunks = (s_carry + epb - 1) / epb;
     }
 }
 
@@ -213,9 +237,9 @@ __device__ int lb_owner(const int *eoff, const int *toff, int n, int e) {
 // 3. owner of the first edge of every chunk (cstart[chunks] = owner of the last edge)
 __global__ void k_lb_part(Ctl *ctl, const int *__restrict__ eoff, const int *__restrict__ toff, int *__restrict__ cstart) {
     if (ctl->mode != MODE_TD || !ctl->lb) return;
-    const int n = ctl->in_n, nch = ctl->lb_chunks, total = ctl->lb_total;
+    const int n = ctl->in_n, nch = ctl->lb_chunks, total = ctl->lb_total, epb = lb_epb(total);
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= nch; c += gridDim.x * blockDim.x) {
-        const int e = c < nch ? c * kEPB : total - 1;
+        const int e = c < nch ? c * epb : total - 1;
         cstart[c] = lb_owner(eoff, toff, n, e);
     }
 }
@@ -229,11 +253,13 @@ __global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const OT *__restrict__ 
                                                     const int *__restrict__ toff, const int *__restrict__ cstart) {
     if (ctl->mode != MODE_TD || !ctl->lb) return;
     const int iter = ctl->iter, in_begin = ctl->in_begin, n = ctl->in_n, out_begin = ctl->out_begin;
-    const int nch = ctl->lb_chunks, total = ctl->lb_total;
+    const int nch = ctl->lb_chunks, total = ctl->lb_total, epb = lb_epb(total);
     constexpr int kCap = kEPB + 1;
     __shared__ int s_off[kCap];
     __shared__ OT s_rb[kCap];
     __shared__ int s_v[kCap];
+    __shared__ int s_wc[B / MG_WAVE + 1];
+    __shared__ long long s_wd[B / MG_WAVE];
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const int i0 = cstart[c];
         const int i1 = cstart[c + 1];
@@ -249,7 +275,7 @@ __global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const OT *__restrict__ 
             }
         }
         __syncthreads();
-        const int e0 = c * kEPB, e1 = min(total, e0 + kEPB);
+        const int e0 = c * epb, e1 = min(total, e0 + epb);
         for (int eb = e0; eb < e1; eb += B) {
             const int e = eb + threadIdx.x;
             int win = 0, u = 0;
@@ -268,7 +294,7 @@ __global__ __launch_bounds__(B) void k_push_lb(Ctl *ctl, const OT *__restrict__ 
                 u = ld_stream(&col[p]);
                 win = relax<H>(state[v], sigma ? sigma[v] : 0.0, u, p, wgt, state, vfc, sigma, iter);
             }
-            append<true>(ctl, q, out_begin, row, win != 0, u);
+            block_append<B, true>(ctl, q, out_begin, row, win != 0, u, s_wc, s_wd);
         }
         __syncthreads();
     }
@@ -828,6 +854,7 @@ __global__ __launch_bounds__(B) void k_bc_back(int l, const int *__restrict__ lv
                                                     int *__restrict__ wl, int *__restrict__ wn, int lane_max, int split,
                                                     int chunk) {
     const int b = lvl_off[l], n = lvl_off[l + 1] - b;
+    __shared__ int s_wc[B / MG_WAVE + 1];
     for (int base = blockIdx.x * B; base < n; base += gridDim.x * B) {
         const int i = base + threadIdx.x;
         int v = 0, rb = 0, re = 0;
@@ -838,7 +865,7 @@ __global__ __launch_bounds__(B) void k_bc_back(int l, const int *__restrict__ lv
         }
         const int deg = re - rb;
         const bool mid = i < n && deg > lane_max && deg < split;
-        const int pos = wave_append(wn, mid);
+        const int pos = block_append_slot<B>(wn, mid, s_wc);
         if (mid) wl[pos] = v;
         if (i < n && deg <= lane_max) {
             const double sv = sigma[v];
@@ -1103,7 +1130,7 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMalloc(&e->chunks, sizeof(HugeChunk) * e->chunk_cap));
     MG_HIP(hipMalloc(&e->eoff, sizeof(int) * e->qcap));
     MG_HIP(hipMalloc(&e->tsum, sizeof(int) * (e->qcap / kTile + 2)));
-    MG_HIP(hipMalloc(&e->cstart, sizeof(int) * (ne / kEPB + 4)));
+    MG_HIP(hipMalloc(&e->cstart, sizeof(int) * (lb_max_chunks(ne) + 4)));
     MG_HIP(hipMalloc(&e->part_n, sizeof(int) * (max_grid(e) + 8)));
     MG_HIP(hipMalloc(&e->part_mf, sizeof(long long) * (max_grid(e) + 8)));
     MG_HIP(hipMalloc(&e->ctl, sizeof(Ctl)));
@@ -1311,7 +1338,7 @@ static int launch_level(mg_engine *e, bool known) {
             if (e->prof) mg_prof_end(e, slot);
         }
         if (!known || c.lb) {
-            const long long maxch = known ? (c.mf + kEPB - 1) / kEPB : (e->ne + kEPB - 1) / kEPB;
+            const long long maxch = known ? (c.mf + lb_epb(c.mf) - 1) / lb_epb(c.mf) : lb_max_chunks(e->ne);
             const int gd = known ? grid_for(c.in_n, kTile, G) : G;
             if (e->prof) mg_prof_begin(e, "k_lb_scan", 12.0 * c.in_n, &slot);
             MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_lb_deg<OT>), dim3(gd), dim3(kBlock), 0, s, e->ctl, rowp, e->q, e->eoff, e->tsum));
@@ -1557,7 +1584,7 @@ constexpr size_t na_al(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t kNaQ = na_al(sizeof(int) * ((size_t)kNaMaxNv + 2));                 // queue / level log
 constexpr size_t kNaEoff = kNaQ;                                                      // tile-local degree offsets
 constexpr size_t kNaTsum = na_al(sizeof(int) * ((size_t)(kNaMaxNv + 2) / kTile + 2)); // tile sums
-constexpr size_t kNaCst = na_al(sizeof(int) * ((size_t)(kNaMaxNe / kEPB) + 4));      // chunk owners
+constexpr size_t kNaCst = na_al(sizeof(int) * ((size_t)lb_max_chunks(kNaMaxNe) + 4)); // chunk owners
 constexpr size_t kNaBm = na_al(sizeof(uint64_t) * ((size_t)kNaMaxNv / 64 + 1));      // one bitmap
 constexpr size_t kNaPn = na_al(sizeof(int) * (kNaGrid + 8));
 constexpr size_t kNaPm = na_al(sizeof(long long) * (kNaGrid + 8));

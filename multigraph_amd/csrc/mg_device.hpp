@@ -50,6 +50,32 @@ __device__ __forceinline__ int wave_append(int *ctr, bool pred) {
     return pred ? base + mbcnt(m) : -1;
 }
 
+// Workgroup-aggregated append: ONE atomicAdd per workgroup of B threads, all of which
+// call this together (it holds three barriers); s_wc is B / 64 + 1 ints of LDS.
+// Returns this thread's slot (or -1).  Where many workgroups append to one counter at
+// once, the per-wave atomics of wave_append queue up on its cache line.
+template <int B>
+__device__ __forceinline__ int block_append_slot(int *ctr, bool pred, int *s_wc) {
+    constexpr int NW = B / 64;
+    const uint64_t m = ballot(pred);
+    const int wv = threadIdx.x / 64;
+    if (lane() == 0) s_wc[wv] = (int)__builtin_popcountll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int i = 0; i < NW; ++i) {
+            const int c = s_wc[i];
+            s_wc[i] = tot;
+            tot += c;
+        }
+        s_wc[NW] = tot ? atomicAdd(ctr, tot) : 0;
+    }
+    __syncthreads();
+    const int pos = s_wc[NW] + s_wc[wv] + mbcnt(m);
+    __syncthreads();  // s_wc is refilled by the next call
+    return pred ? pos : -1;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
