@@ -85,8 +85,8 @@ typedef struct mg_options {
     float  pr_tolerance;    /* PageRank tolerance (0.01, common.h:60)                   */
     int    pr_max_iter;     /* PageRank iteration cap (0: unbounded, as the reference)  */
     int    part_log2;       /* log2 of the vertex-partition size of the tiled layout    */
-    int    batch_chunk;     /* batched runs: edges per static chunk (0: 1024 for BFS,
-                               128 for BC and SSSP)                                     */
+    int    batch_chunk;     /* batched runs: edges per static chunk, at least 64 (0:
+                               1024 for BFS, 128 for BC, 64 for SSSP)                   */
     int    persist_levels;  /* batched BFS / SSSP: run sparse levels in a persistent
                                cooperative launch, at most this many per launch (-1:
                                unbounded; 0, the default: every level from the captured

@@ -131,7 +131,7 @@ extern "C" int mg_batch_autotune(const mg_graph *g, int kind, int n, const int32
     coords.push_back({[](mg_options &o, double v) { o.alpha = (float)v; }, {4, 8, 15, 30, 60, 120}});
     if (kind != MG_BATCH_BC)  // the sparse-level threshold (queue levels below (nv + E) / beta)
         coords.push_back({[](mg_options &o, double v) { o.beta = (float)v; }, {4, 9, 18, 36, 72}});
-    coords.push_back({[](mg_options &o, double v) { o.batch_chunk = (int)v; }, {128, 256, 512, 1024, 4096}});
+    coords.push_back({[](mg_options &o, double v) { o.batch_chunk = (int)v; }, {64, 128, 256, 512, 1024, 4096}});
     coords.push_back({[](mg_options &o, double v) { o.grid_blocks = (int)v; }, {1024, 2048, 4096}});
     if (kind == MG_BATCH_BFS)  // lane / wave split of the pull's in-edge rows
         coords.push_back({[](mg_options &o, double v) { o.batch_light = (int)v; }, {16, 32, 64}});

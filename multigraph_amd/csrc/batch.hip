@@ -129,7 +129,13 @@ constexpr int kCH = 1024;  // default edges per chunk (mg_options.batch_chunk)
 #ifndef MG_HIT_STEP
 #define MG_HIT_STEP 8
 #endif
-constexpr int kHitStep = MG_HIT_STEP;  // matched edges whose row gathers a wave issues together
+constexpr int kHitStep = MG_HIT_STEP;  // matched edges whose row gathers a wave issues together (BC)
+#ifndef MG_SSSP_HIT_STEP
+#define MG_SSSP_HIT_STEP 4
+#endif
+// the same for the batched SSSP sweeps, whose 256-byte distance rows congest the memory
+// system at 8 gathers in flight per wave (RMAT-22, 64 sources: 32.0 ms at 8, 27.4 at 4)
+constexpr int kSsspHitStep = MG_SSSP_HIT_STEP;
 
 // Wave w walks chunks w, w + nw, w + 2 nw, ... (nw waves in the grid) 64 at a time: each
 // lane loads one chunk and its word (word_of(chunk); 0 = nothing to do), and the wave
@@ -638,11 +644,66 @@ __device__ __forceinline__ bool bfs_visit(int it, int w, uint64_t fu, uint64_t *
     return add;
 }
 
-__device__ __forceinline__ void queue_add(bool add, int w, const int *__restrict__ row, int *Qn, int *qn_next, int &cnt,
+// Next-queue appends of a sparse level.  With a stage (the workgroup's list in LDS), a
+// wave's appends take their slots there by one LDS atomic, and the workgroup moves the
+// list to the queue at the end of the level with ONE atomic on the queue counter:
+// atomics per wave on that one counter queue up behind each other.  Entries past the
+// stage's capacity, and every append without a stage, go to the queue at once.  SSSP
+// levels stage (a wave appends once per bit-edge step: uneven batch 194 -> 189 ms);
+// BFS levels do not (one append per wave and step; staged, 55.7 -> 58.5 ms).
+constexpr int kQStage = 4096;
+struct QStage {
+    int n;
+    int v[kQStage];
+};
+struct QOut {
+    int *Qn;     // the next queue
+    int *qn;     // its length counter
+    QStage *st;  // the workgroup's stage, or null
+};
+
+// atomicAdd on an int known to be in LDS (the cast keeps it a ds_add, where a pointer
+// that may also be global would make it a flat atomic)
+typedef __attribute__((address_space(3))) int mg_lds_int;
+__device__ __forceinline__ int lds_atomic_add(int *p, int v) {
+    return __hip_atomic_fetch_add((mg_lds_int *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void qstage_begin(QStage &st) {
+    if (threadIdx.x == 0) st.n = 0;
+    __syncthreads();
+}
+// every thread of the workgroup calls this once, after its last append of the level
+__device__ __forceinline__ void qstage_flush(QStage &st, const QOut &o) {
+    __shared__ int s_base;
+    __syncthreads();
+    const int n = min(st.n, kQStage);
+    if (threadIdx.x == 0) s_base = n ? atomicAdd(o.qn, n) : 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) o.Qn[s_base + i] = st.v[i];
+}
+
+// lane's entry w (when add) at position k of the wave's reserved range
+__device__ __forceinline__ void qout_put(const QOut &o, bool add, int k, int w) {
+    if (!o.st) {
+        if (add) o.Qn[k] = w;
+        return;
+    }
+    const bool spill = add && k >= kQStage;
+    if (add && !spill) o.st->v[k] = w;
+    const int g = wave_append(o.qn, spill);
+    if (spill) o.Qn[g] = w;
+}
+
+__device__ __forceinline__ void queue_add(bool add, int w, const int *__restrict__ row, const QOut &o, int &cnt,
                                           long long &mf, int &mx) {
-    const int slot = wave_append(qn_next, add);
+    const uint64_t m = ballot(add);
+    if (!m) return;
+    int base = 0;
+    if (lane() == 0) base = o.st ? lds_atomic_add(&o.st->n, (int)__builtin_popcountll(m)) : atomicAdd(o.qn, (int)__builtin_popcountll(m));
+    base = readlane(base, 0);
+    qout_put(o, add, base + mbcnt(m), w);
     if (add) {
-        Qn[slot] = w;
         const int dg = row[w + 1] - row[w];
         ++cnt;
         mf += dg;
@@ -653,7 +714,7 @@ __device__ __forceinline__ void queue_add(bool add, int w, const int *__restrict
 // U appends per lane at once (lane l's j-th entry when add[j], of degree dg[j], loaded
 // by the caller alongside its atomics): one counter atomic for the wave
 template <int U>
-__device__ __forceinline__ void queue_add_n(const bool *add, const int *w, const int *dgs, int *Qn, int *qn_next, int &cnt,
+__device__ __forceinline__ void queue_add_n(const bool *add, const int *w, const int *dgs, const QOut &o, int &cnt,
                                             long long &mf, int &mx) {
     uint64_t bm[U];
     int tot = 0;
@@ -661,12 +722,12 @@ __device__ __forceinline__ void queue_add_n(const bool *add, const int *w, const
     for (int j = 0; j < U; ++j) { bm[j] = ballot(add[j]); tot += __builtin_popcountll(bm[j]); }
     if (tot == 0) return;
     int base = 0;
-    if (lane() == 0) base = atomicAdd(qn_next, tot);
+    if (lane() == 0) base = o.st ? lds_atomic_add(&o.st->n, tot) : atomicAdd(o.qn, tot);
     base = readlane(base, 0);
 #pragma unroll
     for (int j = 0; j < U; ++j) {
+        qout_put(o, add[j], base + mbcnt(bm[j]), w[j]);
         if (add[j]) {
-            Qn[base + mbcnt(bm[j])] = w[j];
             const int dg = dgs[j];
             ++cnt;
             mf += dg;
@@ -680,7 +741,7 @@ constexpr int kSparseLevels = 32;  // levels per replay of the sparse-only graph
 constexpr int kVisit = 4;  // edges a lane of a sparse BFS level visits together
 
 __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict__ row, const int *__restrict__ col, uint64_t *F,
-                                           uint64_t *Fn, uint64_t *vis, const int *Q, int *Qn, int *qn_next,
+                                           uint64_t *Fn, uint64_t *vis, const int *Q, const QOut &o,
                                            int *__restrict__ lev, int &cnt, long long &mf, int &mx) {
     const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
     const int nw = (gridDim.x * kBB) >> 6;
@@ -714,7 +775,7 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
 #pragma unroll
             for (int j = 0; j < kVisit; ++j)
                 for (uint64_t bits = m[j]; bits; bits &= bits - 1) lev[(long long)w[j] * 64 + __builtin_ctzll(bits)] = it;
-            queue_add_n<kVisit>(add, w, dg, Qn, qn_next, cnt, mf, mx);
+            queue_add_n<kVisit>(add, w, dg, o, cnt, mf, mx);
         }
         uint64_t heavy = ballot(u >= 0 && !alone);
         while (heavy) {  // the whole wave over one heavier vertex's edges
@@ -726,7 +787,7 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
                 const int pp = pb + lane();
                 const int w = pp < q1 ? col[pp] : 0;
                 const bool add = pp < q1 && bfs_visit(it, w, fv, Fn, vis, lev);
-                queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
+                queue_add(add, w, row, o, cnt, mf, mx);
             }
         }
         if (u >= 0) F[u] = 0ull;
@@ -745,8 +806,8 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
 // being lowered again and again as in plain Bellman-Ford; the result is the same).
 template <class H>
 __device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *__restrict__ row, const int *__restrict__ col,
-                                            const int *__restrict__ wgt, uint64_t *C, uint64_t *Cn, const int *Q, int *Qn,
-                                            int *qn_next, int *__restrict__ dist, int &cnt, long long &mf, int &mx) {
+                                            const int *__restrict__ wgt, uint64_t *C, uint64_t *Cn, const int *Q, const QOut &o,
+                                            int *__restrict__ dist, int &cnt, long long &mf, int &mx) {
     const int wave = (blockIdx.x * kBB + threadIdx.x) >> 6;
     const int nw = (gridDim.x * kBB) >> 6;
     for (int base = wave * MG_WAVE; base < n; base += nw * MG_WAVE) {
@@ -800,7 +861,7 @@ __device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *_
 #pragma unroll
             for (int j = 0; j < kVisit; ++j)
                 add[j] = imp[j] && atomicOr((unsigned long long *)&Cn[w[j]], 1ull << sl) == 0ull;
-            queue_add_n<kVisit>(add, w, dg, Qn, qn_next, cnt, mf, mx);
+            queue_add_n<kVisit>(add, w, dg, o, cnt, mf, mx);
             if (rem) {
                 p += kVisit;
                 if (p >= p1) {
@@ -812,7 +873,7 @@ __device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *_
         }
         {  // far pairs wait in the next changed word
             const bool add = far && atomicOr((unsigned long long *)&Cn[u], (unsigned long long)far) == 0ull;
-            queue_add(add, u, row, Qn, qn_next, cnt, mf, mx);
+            queue_add(add, u, row, o, cnt, mf, mx);
         }
         uint64_t heavy = ballot(u >= 0 && !alone);
         while (heavy) {
@@ -827,7 +888,7 @@ __device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *_
             {
                 const uint64_t fv = ballot(bit && !mine);
                 const bool add = lane() == 0 && fv && atomicOr((unsigned long long *)&Cn[v], (unsigned long long)fv) == 0ull;
-                queue_add(add, v, row, Qn, qn_next, cnt, mf, mx);
+                queue_add(add, v, row, o, cnt, mf, mx);
             }
             for (int pp = q0; pp < q1; ++pp) {
                 const int w = col[pp];
@@ -841,7 +902,7 @@ __device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *_
                 const uint64_t m = ballot(imp);
                 bool add = false;
                 if (lane() == 0 && m) add = atomicOr((unsigned long long *)&Cn[w], (unsigned long long)m) == 0ull;
-                queue_add(add, w, row, Qn, qn_next, cnt, mf, mx);
+                queue_add(add, w, row, o, cnt, mf, mx);
             }
         }
         if (u >= 0) C[u] = 0ull;
@@ -871,8 +932,8 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_bfs(BCtl *ctl, const int *__r
     const int cur = (it - 1) & 1;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    sparse_bfs(it, ctl->qn[cur], row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
-               lev, cnt, mf, mx);
+    const QOut o{cur ? QA : QB, &ctl->qn[cur ^ 1], nullptr};
+    sparse_bfs(it, ctl->qn[cur], row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, o, lev, cnt, mf, mx);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
     zero_rest(nparts, part_n, part_e, part_x);
 }
@@ -885,8 +946,12 @@ __global__ __launch_bounds__(kBB) void k_mbs_level_sssp(BCtl *ctl, const int *__
     const int cur = (ctl->it - 1) & 1;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    sparse_sssp<MG_SSSP_HOOKS>(ctl->it, ctl->qn[cur], ctl->thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB, &ctl->qn[cur ^ 1],
-                dist, cnt, mf, mx);
+    __shared__ QStage st;
+    qstage_begin(st);
+    const QOut o{cur ? QA : QB, &ctl->qn[cur ^ 1], &st};
+    sparse_sssp<MG_SSSP_HOOKS>(ctl->it, ctl->qn[cur], ctl->thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, o, dist,
+                               cnt, mf, mx);
+    qstage_flush(st, o);
     put_parts(cnt, mf, mx, part_n, part_e, part_x);
     zero_rest(nparts, part_n, part_e, part_x);
 }
@@ -937,11 +1002,16 @@ __global__ __launch_bounds__(kBB) void k_mbs_fused(int k, BCtl *c2, FStat *st, L
     const int it = c.it, cur = (it - 1) & 1;
     int cnt = 0, mx = 0;
     long long mf = 0;
-    if (kSSSP)
-        sparse_sssp<MG_SSSP_HOOKS>(it, s0.cnt, c.thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, cur ? QA : QB,
-                                   &nx->cnt, out, cnt, mf, mx);
-    else
-        sparse_bfs(it, s0.cnt, row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, cur ? QA : QB, &nx->cnt, out, cnt, mf, mx);
+    if (kSSSP) {  // staged appends (BFS levels append once per wave and step: staging them measured slower)
+        __shared__ QStage qs;
+        qstage_begin(qs);
+        const QOut o{cur ? QA : QB, &nx->cnt, &qs};
+        sparse_sssp<MG_SSSP_HOOKS>(it, s0.cnt, c.thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, o, out, cnt, mf, mx);
+        qstage_flush(qs, o);
+    } else {
+        const QOut o{cur ? QA : QB, &nx->cnt, nullptr};
+        sparse_bfs(it, s0.cnt, row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, o, out, cnt, mf, mx);
+    }
     __shared__ long long s_tmp[kBB / MG_WAVE];
     const long long tm = block_sum<kBB / MG_WAVE>(mf, s_tmp);
     const long long tx = block_max<kBB / MG_WAVE>((long long)mx, s_tmp);
@@ -1074,8 +1144,9 @@ __global__ __launch_bounds__(kBB) void k_mbs_persist(PersistArgs a) {
             a.pc->mx[ns] = 0;
             a.pc->mf[ns] = 0;
         }
+        const QOut o{cur ? a.QA : a.QB, &a.pc->qn[slot], nullptr};
         sparse_sssp<MG_SSSP_HOOKS>(it, (int)n, c.thr, a.row, a.col, a.wgt, cur ? a.FB : a.FA, cur ? a.FA : a.FB, cur ? a.QB : a.QA,
-                                   cur ? a.QA : a.QB, &a.pc->qn[slot], a.out, cnt, sm, xm);
+                                   o, a.out, cnt, sm, xm);
         (void)cnt;
         const long long bs = block_sum<kBB / MG_WAVE>(sm, s_tmp);
         const long long bx = block_max<kBB / MG_WAVE>((long long)xm, s_tmp);
@@ -1637,19 +1708,19 @@ __global__ __launch_bounds__(kBB) void k_mb_sssp(const BCtl *ctl, long long nch,
             s_f[threadIdx.x] = f;
             wave_sync();
             uint64_t hits = ballot(f != 0ull);  // only neighbours that changed for some source
-            while (hits) {  // kHitStep neighbours per step: their row gathers overlap
-                int j[kHitStep];
+            while (hits) {  // kSsspHitStep neighbours per step: their row gathers overlap
+                int j[kSsspHitStep];
 #pragma unroll
-                for (int q = 0; q < kHitStep; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
-                int du[kHitStep];
-                bool on[kHitStep];
+                for (int q = 0; q < kSsspHitStep; ++q) { j[q] = hits ? __builtin_ctzll(hits) : -1; hits &= hits - 1; }
+                int du[kSsspHitStep];
+                bool on[kSsspHitStep];
 #pragma unroll
-                for (int q = 0; q < kHitStep; ++q) {
+                for (int q = 0; q < kSsspHitStep; ++q) {
                     on[q] = j[q] >= 0 && ((s_f[wl + j[q]] >> lane()) & 1ull);
                     du[q] = on[q] ? dist[(long long)s_u[wl + j[q]] * 64 + lane()] : 0;
                 }
 #pragma unroll
-                for (int q = 0; q < kHitStep; ++q)
+                for (int q = 0; q < kSsspHitStep; ++q)
                     if (on[q]) {
                         int cand;
                         H::get_partial_result(&cand, du[q], s_w[wl + j[q]]);
@@ -1834,7 +1905,7 @@ extern "C" int mg_batch_build(const mg_graph *g, int kind, int max_sources, cons
     // 19.4 ms against 1024-edge chunks on 2048 workgroups; BFS, which stops at the
     // first parent, 1.94 ms with 1024 and 2.24 with 128)
     if (b->opt.grid_blocks <= 0 || b->opt.grid_blocks > 4096) b->opt.grid_blocks = 4096;
-    if (b->opt.batch_chunk <= 0) b->opt.batch_chunk = kind == MG_BATCH_BFS ? kCH : 128;
+    if (b->opt.batch_chunk <= 0) b->opt.batch_chunk = kind == MG_BATCH_BFS ? kCH : kind == MG_BATCH_SSSP ? 64 : 128;
     auto fail = [&](int rc) { batch_free(b); return rc; };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(mg_fail(MG_ERR_DEVICE, "stream"));
     // static edge chunks (a hub's edges are cut into pieces of at most `chunk`)

@@ -63,7 +63,7 @@ def test_autotune_batch_union(mg, orc):
         off += p.nv
     for kind in (mg.BATCH_BFS, mg.BATCH_SSSP):
         opt, t0, t1 = mg.batch_autotune(u, kind, src, slots, reps=1)
-        assert t1 <= t0 + 1e-6 and opt.batch_chunk in (0, 128, 256, 512, 1024, 4096)
+        assert t1 <= t0 + 1e-6 and opt.batch_chunk in (0, 64, 128, 256, 512, 1024, 4096)
         if kind == mg.BATCH_BFS:
             assert opt.batch_light in (0, 16, 32, 64)
         assert opt.sparse_grid in (0, 64, 128)
