@@ -330,7 +330,10 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // the in-edges; orow, the out-edge rows of a directed graph (null when symmetric),
 // gives the degree sum.  The new frontier is the bitmap itself; its size and degree sum
 // leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
-constexpr int kPullStep = 8;      // BC pull: edges a lane loads together per step
+#ifndef MG_PULL_STEP
+#define MG_PULL_STEP 4
+#endif
+constexpr int kPullStep = MG_PULL_STEP;  // BC pull: edges a lane loads together per step
 #ifndef MG_PULL_FLAT
 #define MG_PULL_FLAT 1            // BC pull: a word's rows as one flat edge range (0: lane / wave tiers)
 #endif
@@ -839,6 +842,11 @@ __device__ __forceinline__ void bc_term(KSum &acc, double sv, float cw) {
     H::back_accumulate(&acc, t);
 }
 
+#ifndef MG_BACK_STEP
+#define MG_BACK_STEP 4
+#endif
+constexpr int kBackStep = MG_BACK_STEP;  // BC backward: edges (or rounds of edges) a lane loads together
+
 // Tiers: a lane per vertex of at most lane_max edges (here), a wave per vertex below
 // split (k_bc_back_wave, over a list this kernel appends to), chunks of `chunk` edges
 // from split up (k_bc_back_huge).  The wave tier runs as its own kernel so that every listed vertex
@@ -870,19 +878,19 @@ __global__ __launch_bounds__(B) void k_bc_back(int l, const int *__restrict__ lv
         if (i < n && deg <= lane_max) {
             const double sv = sigma[v];
             KSum acc;
-            // four edges per step: their column, level and c loads are in flight together
-            for (int p = rb; p < re; p += 4) {
-                int w4[4];
-                bool f[4];
-                float c4[4];
+            // kBackStep edges per step: their column, level and c loads are in flight together
+            for (int p = rb; p < re; p += kBackStep) {
+                int w4[kBackStep];
+                bool f[kBackStep];
+                float c4[kBackStep];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) { f[j] = p + j < re; w4[j] = f[j] ? col[p + j] : 0; }
+                for (int j = 0; j < kBackStep; ++j) { f[j] = p + j < re; w4[j] = f[j] ? col[p + j] : 0; }
 #pragma unroll
-                for (int j = 0; j < 4; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
+                for (int j = 0; j < kBackStep; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
+                for (int j = 0; j < kBackStep; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < kBackStep; ++j)
                     if (f[j]) bc_term<H>(acc, sv, c4[j]);
             }
             H::back_update(&delta[v], &cw[v], acc.value(), sv);
@@ -894,8 +902,8 @@ __global__ __launch_bounds__(B) void k_bc_back(int l, const int *__restrict__ lv
     }
 }
 
-// a wave per listed vertex (32 to 2047 edges): lanes over edges, four rounds of loads
-// in flight, the lanes' compensated sums merged
+// a wave per listed vertex (32 to 2047 edges): lanes over edges, kBackStep rounds of
+// loads in flight, the lanes' compensated sums merged
 template <class H, int B>
 __global__ __launch_bounds__(B) void k_bc_back_wave(int l, const int *__restrict__ wl, const int *__restrict__ wn,
                                                          const int *__restrict__ row, const int *__restrict__ col,
@@ -910,22 +918,22 @@ __global__ __launch_bounds__(B) void k_bc_back_wave(int l, const int *__restrict
         const int rb = row[v], re = row[v + 1];
         const double sv = sigma[v];
         KSum acc;
-        for (int p0 = rb; p0 < re; p0 += 4 * MG_WAVE) {
-            int w4[4];
-            bool f[4];
-            float c4[4];
+        for (int p0 = rb; p0 < re; p0 += kBackStep * MG_WAVE) {
+            int w4[kBackStep];
+            bool f[kBackStep];
+            float c4[kBackStep];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < kBackStep; ++j) {
                 const int p = p0 + j * MG_WAVE + lane();
                 f[j] = p < re;
                 w4[j] = f[j] ? ld_stream(&col[p]) : 0;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
+            for (int j = 0; j < kBackStep; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
+            for (int j = 0; j < kBackStep; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < kBackStep; ++j)
                 if (f[j]) bc_term<H>(acc, sv, c4[j]);
         }
         acc = wave_ksum(acc);
@@ -947,22 +955,22 @@ __global__ __launch_bounds__(B) void k_bc_back_huge(int l, const int *__restrict
         const int b = row[v] + hc.j * chunk, e = min(row[v + 1], b + chunk);
         const double sv = sigma[v];
         KSum acc;
-        for (int p0 = b; p0 < e; p0 += 4 * B) {
-            int w4[4];
-            bool f[4];
-            float c4[4];
+        for (int p0 = b; p0 < e; p0 += kBackStep * B) {
+            int w4[kBackStep];
+            bool f[kBackStep];
+            float c4[kBackStep];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < kBackStep; ++j) {
                 const int p = p0 + j * B + threadIdx.x;
                 f[j] = p < e;
                 w4[j] = f[j] ? ld_stream(&col[p]) : 0;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
+            for (int j = 0; j < kBackStep; ++j) f[j] = f[j] && bc_next(l, w4[j], lev8, level);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
+            for (int j = 0; j < kBackStep; ++j) c4[j] = f[j] ? cw[w4[j]] : 0.f;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < kBackStep; ++j)
                 if (f[j]) bc_term<H>(acc, sv, c4[j]);
         }
         acc = wave_ksum(acc);
