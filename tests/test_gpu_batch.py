@@ -323,7 +323,7 @@ def test_batch_persistent_large_sparse_levels(mg, orc):
     assert 0 < runs[-1]["persistent_levels"] < runs[-1]["sparse_levels"]
 
 
-@pytest.mark.parametrize("delta", [-1, 0, 1, 16, 100000])
+@pytest.mark.parametrize("delta", [-1, 0, 1, 14, 16, 100000])
 @pytest.mark.parametrize("gname", ["grid", "rmat13", "union"])
 @pytest.mark.parametrize("persist", [0, -1], ids=["graph", "persistent"])
 def test_batch_sssp_near_far(mg, orc, delta, gname, persist):
