@@ -633,12 +633,22 @@ constexpr int kLaneWork = 16;
 // joins the next queue.  Every queued vertex is queued once and has at most one chunk,
 // so its consumed word is cleared by the lane that owns it.  The thread counts the new
 // frontier's vertices, degree sum and largest degree.
+// kDup (the sparse-only graph's levels): every discoverer that claims new bits queues
+// the vertex, without waiting for the next word's old value (one dependent device
+// atomic less per level).  A vertex found by several discoverers is then queued
+// several times: each copy walks the whole word (or finds it cleared by another), and
+// the claims stay exclusive, so results are unchanged and copies do not multiply.
+// Appends past the queue's capacity are dropped (the words still hold every bit) and
+// the next level rebuilds the queue from the words instead.
+template <bool kDup = false>
 __device__ __forceinline__ bool bfs_visit(int it, int w, uint64_t fu, uint64_t *Fn, uint64_t *vis, int *__restrict__ lev) {
     const uint64_t m = fu & ~vis[w];
     if (!m) return false;
     const uint64_t nb = m & ~(uint64_t)atomicOr((unsigned long long *)&vis[w], (unsigned long long)m);
     if (!nb) return false;
-    const bool add = atomicOr((unsigned long long *)&Fn[w], (unsigned long long)nb) == 0ull;
+    bool add = true;
+    if (kDup) atomicOr((unsigned long long *)&Fn[w], (unsigned long long)nb);
+    else add = atomicOr((unsigned long long *)&Fn[w], (unsigned long long)nb) == 0ull;
     uint64_t bits = nb;
     while (bits) { const int sl = __builtin_ctzll(bits); lev[(long long)w * 64 + sl] = it; bits &= bits - 1; }
     return add;
@@ -660,6 +670,7 @@ struct QOut {
     int *Qn;     // the next queue
     int *qn;     // its length counter
     QStage *st;  // the workgroup's stage, or null
+    int cap = 0x7FFFFFFF;  // entries the queue holds (appends past it are dropped: see sparse_bfs)
 };
 
 // atomicAdd on an int known to be in LDS (the cast keeps it a ds_add, where a pointer
@@ -686,7 +697,7 @@ __device__ __forceinline__ void qstage_flush(QStage &st, const QOut &o) {
 // lane's entry w (when add) at position k of the wave's reserved range
 __device__ __forceinline__ void qout_put(const QOut &o, bool add, int k, int w) {
     if (!o.st) {
-        if (add) o.Qn[k] = w;
+        if (add && k < o.cap) o.Qn[k] = w;
         return;
     }
     const bool spill = add && k >= kQStage;
@@ -740,6 +751,7 @@ __device__ __forceinline__ void queue_add_n(const bool *add, const int *w, const
 constexpr int kSparseLevels = 32;  // levels per replay of the sparse-only graph (at least)
 constexpr int kVisit = 4;  // edges a lane of a sparse BFS level visits together
 
+template <bool kDup = false>
 __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict__ row, const int *__restrict__ col, uint64_t *F,
                                            uint64_t *Fn, uint64_t *vis, const int *Q, const QOut &o,
                                            int *__restrict__ lev, int &cnt, long long &mf, int &mx) {
@@ -770,8 +782,14 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
                 dg[j] = w[j] >= 0 ? row[w[j] + 1] - row[w[j]] : 0;
             }
 #pragma unroll
-            for (int j = 0; j < kVisit; ++j)
-                add[j] = m[j] && atomicOr((unsigned long long *)&Fn[w[j]], (unsigned long long)m[j]) == 0ull;
+            for (int j = 0; j < kVisit; ++j) {
+                if (kDup) {
+                    if (m[j]) atomicOr((unsigned long long *)&Fn[w[j]], (unsigned long long)m[j]);
+                    add[j] = m[j] != 0ull;
+                } else {
+                    add[j] = m[j] && atomicOr((unsigned long long *)&Fn[w[j]], (unsigned long long)m[j]) == 0ull;
+                }
+            }
 #pragma unroll
             for (int j = 0; j < kVisit; ++j)
                 for (uint64_t bits = m[j]; bits; bits &= bits - 1) lev[(long long)w[j] * 64 + __builtin_ctzll(bits)] = it;
@@ -786,7 +804,7 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
             for (int pb = q0; pb < q1; pb += MG_WAVE) {
                 const int pp = pb + lane();
                 const int w = pp < q1 ? col[pp] : 0;
-                const bool add = pp < q1 && bfs_visit(it, w, fv, Fn, vis, lev);
+                const bool add = pp < q1 && bfs_visit<kDup>(it, w, fv, Fn, vis, lev);
                 queue_add(add, w, row, o, cnt, mf, mx);
             }
         }
@@ -991,7 +1009,12 @@ __global__ __launch_bounds__(kBB) void k_mbs_fused(int k, BCtl *c2, FStat *st, L
         decide(c, s0.cnt, s0.mf, s0.mx, r);
         if (c.done) run = false;
         else if (!c.sparse || c.compact || c.fill) { c = c0; c.need_full = 1; run = false; }
-        else ++c.nsparse;
+        else if (!kSSSP && s0.cnt > r.nv + 1) {  // appends were dropped: the full graph rebuilds the queue from the words
+            c = c0;
+            c.qvalid[c0.it & 1] = 0;
+            c.need_full = 1;
+            run = false;
+        } else ++c.nsparse;
     }
     if (!run) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { c2[(k + 1) & 1] = c; st[(k + 1) % 3] = s0; }
@@ -1009,8 +1032,8 @@ __global__ __launch_bounds__(kBB) void k_mbs_fused(int k, BCtl *c2, FStat *st, L
         sparse_sssp<MG_SSSP_HOOKS>(it, s0.cnt, c.thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, o, out, cnt, mf, mx);
         qstage_flush(qs, o);
     } else {
-        const QOut o{cur ? QA : QB, &nx->cnt, nullptr};
-        sparse_bfs(it, s0.cnt, row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, o, out, cnt, mf, mx);
+        const QOut o{cur ? QA : QB, &nx->cnt, nullptr, r.nv + 1};
+        sparse_bfs<true>(it, s0.cnt, row, col, cur ? FB : FA, cur ? FA : FB, vis, cur ? QB : QA, o, out, cnt, mf, mx);
     }
     __shared__ long long s_tmp[kBB / MG_WAVE];
     const long long tm = block_sum<kBB / MG_WAVE>(mf, s_tmp);
@@ -1024,8 +1047,10 @@ __global__ __launch_bounds__(kBB) void k_mbs_fused(int k, BCtl *c2, FStat *st, L
 // after K fused nodes: the control block (with the produced frontier's queue length)
 // and the frontier's totals as partials, as a sparse level of the full graph leaves them
 __global__ void k_mbs_fused_exit(int K, BCtl *ctl, const BCtl *c2, const FStat *st, int *__restrict__ pn,
-                                 long long *__restrict__ pe, int *__restrict__ px, int nparts) {
-    const FStat t = st[K % 3];
+                                 long long *__restrict__ pe, int *__restrict__ px, int nparts, int qcap) {
+    FStat t = st[K % 3];
+    const bool dropped = t.cnt > qcap;  // the last level's appends outgrew the queue
+    if (dropped) t.cnt = qcap;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
         pn[i] = i ? 0 : t.cnt;
         pe[i] = i ? 0 : t.mf;
@@ -1034,6 +1059,7 @@ __global__ void k_mbs_fused_exit(int K, BCtl *ctl, const BCtl *c2, const FStat *
     if (threadIdx.x == 0) {
         BCtl c = c2[K & 1];
         c.qn[c.it & 1] = t.cnt;
+        if (dropped) c.qvalid[c.it & 1] = 0;  // the next sparse level rebuilds the queue from the words
         *ctl = c;
     }
 }
@@ -2209,7 +2235,7 @@ static int capture_levels(mg_batch *b, int K, bool sparse_only, hipGraphExec_t *
                                    (const int *)nullptr, b->F[0], b->F[1], b->vis, b->Q[0], b->Q[1], b->lev);
         }
         hipLaunchKernelGGL(k_mbs_fused_exit, dim3(1), dim3(256), 0, s, K, b->ctl, (const BCtl *)b->ctl2, (const FStat *)b->fst,
-                           b->part_n, b->part_e, b->part_x, gv);
+                           b->part_n, b->part_e, b->part_x, gv, (int)b->nv + 1);
     }
     for (int k = 0; k < K && !sparse_only; ++k) {
         hipLaunchKernelGGL(k_mb_ctl, dim3(1), dim3(256), 0, s, b->ctl, b->part_n, b->part_e, b->part_x, gv, rule);

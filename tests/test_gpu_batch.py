@@ -252,6 +252,35 @@ def test_batch_graph_replay(mg, orc, kind, graph_levels):
     b.close()
 
 
+@pytest.mark.parametrize("graph_levels", [1, 8])
+def test_batch_bfs_duplicate_appends(mg, orc, graph_levels):
+    """Levels of the sparse-only graph queue a vertex once per discoverer that claims
+    new bits.  Here 64 sources reach 64 middle vertices (one each), which all reach the
+    same 16 vertices in one sparse level: 1,024 appends into a queue of nv + 1 = 465
+    entries.  The appends past it are dropped, the next level rebuilds the queue from the
+    frontier words, and a 20-vertex tail then runs on.  Every row matches the oracle."""
+    src, dst = [], []
+    for i in range(64):
+        src.append(i); dst.append(64 + i)                      # source -> its middle vertex
+        for j in range(16):
+            src.append(64 + i); dst.append(128 + j)            # middle -> all 16 shared vertices
+    f = np.arange(144, 444)
+    a, c = np.triu_indices(len(f), 1)
+    src += list(f[a]); dst += list(f[c])                       # a 300-vertex clique: E large, nv small
+    src += [128] + list(range(444, 463)); dst += list(range(444, 464))  # a tail off the shared vertices
+    g = mg.Graph.from_coo(464, np.array(src), np.array(dst))
+    row, col, w = g.csr()
+    assert g.nv == 464
+    b = mg.Batch(g, mg.BATCH_BFS, graph_levels=graph_levels)
+    sources = np.arange(64, dtype=np.int32)
+    for rep in range(2):
+        b.run(sources)
+        lv = b.levels()
+        for i in range(64):
+            np.testing.assert_array_equal(lv[i], orc.bfs(row, col, i), err_msg=f"run {rep} source {i}")
+    b.close()
+
+
 @pytest.mark.parametrize("kind", ["bfs", "sssp"])
 @pytest.mark.parametrize("persist", [0, 1, 7, -1])
 def test_batch_persistent_levels(mg, orc, kind, persist):
