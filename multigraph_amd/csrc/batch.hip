@@ -691,7 +691,7 @@ __device__ __forceinline__ void qstage_flush(QStage &st, const QOut &o) {
     const int n = min(st.n, kQStage);
     if (threadIdx.x == 0) s_base = n ? atomicAdd(o.qn, n) : 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) o.Qn[s_base + i] = st.v[i];
+    for (int i = threadIdx.x; i < n && s_base + i < o.cap; i += blockDim.x) o.Qn[s_base + i] = st.v[i];
 }
 
 // lane's entry w (when add) at position k of the wave's reserved range
@@ -703,7 +703,7 @@ __device__ __forceinline__ void qout_put(const QOut &o, bool add, int k, int w) 
     const bool spill = add && k >= kQStage;
     if (add && !spill) o.st->v[k] = w;
     const int g = wave_append(o.qn, spill);
-    if (spill) o.Qn[g] = w;
+    if (spill && g < o.cap) o.Qn[g] = w;
 }
 
 __device__ __forceinline__ void queue_add(bool add, int w, const int *__restrict__ row, const QOut &o, int &cnt,
@@ -822,6 +822,10 @@ __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict_
 // back into the vertex's next changed word (near-far order: the threshold rises by a
 // fixed band per level, so distances are settled roughly in increasing order instead of
 // being lowered again and again as in plain Bellman-Ford; the result is the same).
+// (Unlike BFS levels, SSSP levels queue a vertex only when its changed word was empty:
+// an improvement is tested against a plain load of the distance, so copies of a vertex
+// racing on one neighbour would both improve it and queue it twice, and the copies
+// multiply along the wave: the uneven batch at band 14 took 445 ms instead of 132.)
 template <class H>
 __device__ __forceinline__ void sparse_sssp(int it, int n, int thr, const int *__restrict__ row, const int *__restrict__ col,
                                             const int *__restrict__ wgt, uint64_t *C, uint64_t *Cn, const int *Q, const QOut &o,
@@ -1009,7 +1013,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_fused(int k, BCtl *c2, FStat *st, L
         decide(c, s0.cnt, s0.mf, s0.mx, r);
         if (c.done) run = false;
         else if (!c.sparse || c.compact || c.fill) { c = c0; c.need_full = 1; run = false; }
-        else if (!kSSSP && s0.cnt > r.nv + 1) {  // appends were dropped: the full graph rebuilds the queue from the words
+        else if (s0.cnt > r.nv + 1) {  // appends were dropped: the full graph rebuilds the queue from the words
             c = c0;
             c.qvalid[c0.it & 1] = 0;
             c.need_full = 1;
@@ -1028,7 +1032,7 @@ __global__ __launch_bounds__(kBB) void k_mbs_fused(int k, BCtl *c2, FStat *st, L
     if (kSSSP) {  // staged appends (BFS levels append once per wave and step: staging them measured slower)
         __shared__ QStage qs;
         qstage_begin(qs);
-        const QOut o{cur ? QA : QB, &nx->cnt, &qs};
+        const QOut o{cur ? QA : QB, &nx->cnt, &qs, r.nv + 1};
         sparse_sssp<MG_SSSP_HOOKS>(it, s0.cnt, c.thr, row, col, wgt, cur ? FB : FA, cur ? FA : FB, cur ? QB : QA, o, out, cnt, mf, mx);
         qstage_flush(qs, o);
     } else {
