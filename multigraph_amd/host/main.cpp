@@ -79,7 +79,10 @@ static void verifyResults(const csc_package *inp, HYB *, OUT *out, int src) {
     bad = verify_cc(inp, out->r);
 #else
     (void)src;
-    // fixpoint of r = 0.15 + 0.85 * sum r_u/deg_u within the stopping tolerance
+    // fixpoint of r = 0.15 + 0.85 * sum r_u/deg_u within the stopping tolerance: both
+    // stopping rules (processi.cu:636-654) leave |f - r| <= tol * max(1, |f|) (PR_T at
+    // 1.0 x tol, PR_D at 0.75 x tol on RMAT graphs); 1.5 x tol leaves room for the
+    // difference between the FP32 ranks and this FP64 sum
     const float tol = mg_compat_options().pr_tolerance;
     for (int v = 0; v < inp->nv && bad < 0; ++v) {
         double s = 0.0;
@@ -88,7 +91,7 @@ static void verifyResults(const csc_package *inp, HYB *, OUT *out, int src) {
             s += out->r[u] / (double)(inp->csc_v[u + 1] - inp->csc_v[u]);
         }
         const double f = 0.15 + 0.85 * s;
-        if (std::fabs(f - out->r[v]) > 20.0 * tol * std::fmax(1.0, std::fabs(f))) bad = v;
+        if (std::fabs(f - out->r[v]) > 1.5 * tol * std::fmax(1.0, std::fabs(f))) bad = v;
     }
 #endif
     if (bad < 0) fprintf(stdout, "validation : PASS\n");
