@@ -54,7 +54,21 @@ __global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int
 // mirror: the host's pinned copy of the control block (eager levels).  It gets the new
 // block, then, after a system-scope fence, its sequence number: the host spins on that
 // number instead of copying the block back and blocking on the stream.
-__global__ void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf, Ctl *mirror, int seq) {
+// A pull level leaves its frontier's size and degree sum as one partial per workgroup
+// (nparts of them; 0 after a push level): the control kernel's workgroup totals them
+// first, so a pull level needs no reduction kernel of its own.
+__global__ __launch_bounds__(256) void k_ctl(Ctl *ctl, Params p, int *lvl_off, long long *lvl_mf, Ctl *mirror, int seq,
+                                             const int *__restrict__ part_n, const long long *__restrict__ part_mf,
+                                             int nparts) {
+    if (nparts > 0 && ctl->mode == MODE_BU) {
+        __shared__ long long s_tmp[256 / MG_WAVE];
+        long long n = 0, m = 0;
+        for (int i = threadIdx.x; i < nparts; i += blockDim.x) { n += part_n[i]; m += part_mf[i]; }
+        n = block_sum<256 / MG_WAVE>(n, s_tmp);
+        m = block_sum<256 / MG_WAVE>(m, s_tmp);
+        if (threadIdx.x == 0) { ctl->out_n = (int)n; ctl->out_mf = m; }
+    }
+    if (threadIdx.x != 0) return;
     Ctl c = *ctl;
     if (c.mode != MODE_DONE) {
         if (lvl_off) {
@@ -329,7 +343,7 @@ __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[
 // message is final); BC scans every parent, summing their path counts.  row / col are
 // the in-edges; orow, the out-edge rows of a directed graph (null when symmetric),
 // gives the degree sum.  The new frontier is the bitmap itself; its size and degree sum
-// leave as one partial per workgroup (no global atomics), summed by k_reduce_parts.
+// leave as one partial per workgroup (no global atomics), summed by k_ctl.
 #ifndef MG_PULL_STEP
 #define MG_PULL_STEP 4
 #endif
@@ -606,17 +620,6 @@ __global__ void k_pull_slots_init(int nh, int *__restrict__ hlev, double *__rest
 }
 
 // sum the per-workgroup partials of a pull level into the control block
-__global__ __launch_bounds__(256) void k_reduce_parts(Ctl *ctl, const int *__restrict__ part_n,
-                                                      const long long *__restrict__ part_mf, int nparts) {
-    if (ctl->mode != MODE_BU) return;
-    __shared__ long long s_tmp[256 / MG_WAVE];
-    long long n = 0, m = 0;
-    for (int i = threadIdx.x; i < nparts; i += 256) { n += part_n[i]; m += part_mf[i]; }
-    n = block_sum<256 / MG_WAVE>(n, s_tmp);
-    m = block_sum<256 / MG_WAVE>(m, s_tmp);
-    if (threadIdx.x == 0) { ctl->out_n = (int)n; ctl->out_mf = m; }
-}
-
 // Queue of the frontier bitmap bm[qsrc] (after a pull level), in vertex order, at
 // q[in_begin]: popcounts per workgroup range, one-workgroup scan, ordered write.
 __global__ __launch_bounds__(kBlock) void k_bmq_count(Ctl *ctl, int words, const uint64_t *__restrict__ bm0,
@@ -783,7 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_sssp(Ctl *ctl, int nch, long lo
 }
 
 // Frontier of stamp `iter + offset` from the stamps: a bitmap word per wave
-// (ballot) and per-workgroup partial counts (size, degree sum) for k_reduce_parts.
+// (ballot) and per-workgroup partial counts (size, degree sum) for k_ctl.
 // gate 0: always; 1: only on a pull level; 2: only right after a switch to pull
 template <class OT = int>
 __global__ __launch_bounds__(kBlock) void k_sssp_scan(Ctl *ctl, int nv, int offset, int gate, const int *__restrict__ vfc,
@@ -1378,6 +1381,7 @@ static int launch_level(mg_engine *e, bool known) {
     }
     const int words = (nv + 63) / 64;
     const int gsc = grid_for(words, kBlock / MG_WAVE, G);  // pull / scan grid = number of partials
+    int red_parts = 0;  // partials of a pull level, totalled by the control kernel
     if (bu && e->algo == MG_SSSP) {
         // one full relaxation pass: row offsets, col + weight + the source's distance per
         // edge, the vertex's distance write: 12E + 8V (+4)
@@ -1392,7 +1396,7 @@ static int launch_level(mg_engine *e, bool known) {
         }
         MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_sssp_scan<OT>), dim3(gsc), dim3(kBlock), 0, s, e->ctl, nv, 0, 1, e->vfc, rowp, e->bm[0],
                                               e->part_n, e->part_mf));
-        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gsc);
+        red_parts = gsc;
         bu = false;
     }
     if (bu) {
@@ -1431,11 +1435,11 @@ static int launch_level(mg_engine *e, bool known) {
                 });
         }
         if (e->prof) mg_prof_end(e, slot);
-        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, e->ctl, e->part_n, e->part_mf, gpl);
+        red_parts = gpl;
     }
     const bool mirror = known && e->mirror;
-    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf, mirror ? e->h_ctl : (Ctl *)nullptr,
-                       mirror ? ++e->ctl_seq : 0);
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(256), 0, s, e->ctl, e->P, e->lvl_off, e->lvl_mf, mirror ? e->h_ctl : (Ctl *)nullptr,
+                       mirror ? ++e->ctl_seq : 0, (const int *)e->part_n, (const long long *)e->part_mf, red_parts);
     if (!known) MG_TRY(launch_after_ctl(e));
     MG_HIP(hipGetLastError());
     return MG_OK;
