@@ -619,7 +619,6 @@ __global__ void k_pull_slots_init(int nh, int *__restrict__ hlev, double *__rest
     }
 }
 
-// sum the per-workgroup partials of a pull level into the control block
 // Queue of the frontier bitmap bm[qsrc] (after a pull level), in vertex order, at
 // q[in_begin]: popcounts per workgroup range, one-workgroup scan, ordered write.
 __global__ __launch_bounds__(kBlock) void k_bmq_count(Ctl *ctl, int words, const uint64_t *__restrict__ bm0,
