@@ -110,6 +110,18 @@ struct KStat {
     double ms = 0.0, bytes = 0.0;
 };
 
+// the control block of a run from `source` of out-degree deg_src: level 1 next, the
+// queue holding the source alone (written by k_seed on the device and by the host for
+// its own copy, so a reset needs no read back)
+__host__ __device__ inline Ctl seed_ctl(int deg_src, long long ne, int out_begin) {
+    Ctl c{};
+    c.iter = 1; c.mode = MODE_TD; c.in_begin = 0; c.in_n = 1; c.out_begin = out_begin; c.out_n = 0;
+    c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0; c.msg_iter = -1;
+    c.lb = deg_src > kSmallMF ? 1 : 0; c.lb_total = 0; c.lb_chunks = 0; c.need_queue = 0; c.qsrc = 0;
+    c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0; c.seq = 0; c.sat = 0;
+    return c;
+}
+
 __host__ __device__ inline void advance(Ctl &c, const Params &p) {
     // record the level just produced (BC level log)
     const int produced = c.out_n;
@@ -185,6 +197,9 @@ struct mg_engine {
     // mg_options.wide_offsets): BFS and SSSP on symmetric graphs
     long long *row64 = nullptr;
     bool wide = false;
+    // out-degrees on the host (frontier engines): a reset seeds its control block from
+    // the source's degree without reading the device
+    std::vector<int32_t> hdeg;
     // in-edges of a directed graph (the transposed CSR, rows sorted by source, weights
     // carried along) for pull levels; null on a symmetric graph, whose rows are both
     int *trow = nullptr, *tcol = nullptr, *tw = nullptr;

@@ -43,12 +43,7 @@ __global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int
     state[src] = 0;
     if (sigma) sigma[src] = 1.0;
     q[0] = src;
-    Ctl c;
-    c.iter = 1; c.mode = MODE_TD; c.in_begin = 0; c.in_n = 1; c.out_begin = out_begin; c.out_n = 0;
-    c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0; c.msg_iter = -1;
-    c.lb = deg_src > kSmallMF ? 1 : 0; c.lb_total = 0; c.lb_chunks = 0; c.need_queue = 0; c.qsrc = 0;
-    c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0; c.seq = 0; c.sat = 0;
-    *ctl = c;
+    *ctl = seed_ctl(deg_src, ne, out_begin);
 }
 
 // mirror: the host's pinned copy of the control block (eager levels).  It gets the new
@@ -1147,6 +1142,8 @@ int mg_frontier_build(mg_engine *e, const mg_graph *g) {
     MG_HIP(hipMalloc(&e->aux, sizeof(int) * 16));
     // coherent: k_ctl writes the host's copy of the control block directly
     MG_HIP(hipHostMalloc(&e->h_ctl, sizeof(Ctl), hipHostMallocCoherent));
+    e->hdeg.resize((size_t)nv);
+    for (int v = 0; v < nv; ++v) e->hdeg[(size_t)v] = (int32_t)(g->off((int64_t)v + 1) - g->off(v));
     // a directed graph's pull levels read in-edges: transpose the CSR once (counting
     // sort by destination; scanning sources in order leaves every row sorted by source)
     std::vector<int32_t> hrow_t, hcol_t;
@@ -1281,7 +1278,9 @@ int mg_frontier_reset(mg_engine *e, int source) {
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(padv, 256, 4096)), dim3(256), 0, s, e->delta, padv, 0.0f);
     }
     int deg_src = 0;
-    if (nv > 0) {
+    if (nv > 0 && !e->hdeg.empty()) {
+        deg_src = e->hdeg[(size_t)source];
+    } else if (nv > 0) {
         if (e->wide) {
             long long rr[2];
             MG_HIP(hipMemcpy(rr, e->row64 + source, sizeof(long long) * 2, hipMemcpyDeviceToHost));
@@ -1302,10 +1301,7 @@ int mg_frontier_reset(mg_engine *e, int source) {
     e->levels = 1;
     e->pb16 = e->pb16_ok;
     e->pb_force = false;
-    if (nv > 0) {
-        MG_HIP(hipMemcpyAsync(e->h_ctl, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-        MG_HIP(hipStreamSynchronize(s));
-    }
+    if (nv > 0) *e->h_ctl = seed_ctl(deg_src, e->ne, out_begin);  // what k_seed writes on the device
     return MG_OK;
 }
 
