@@ -119,6 +119,10 @@ __host__ __device__ inline Ctl seed_ctl(int deg_src, long long ne, int out_begin
     c.nchunks = 0; c.convert = 0; c.bm_cur = 0; c.levels = 1; c.changed = 0; c.msg_iter = -1;
     c.lb = deg_src > kSmallMF ? 1 : 0; c.lb_total = 0; c.lb_chunks = 0; c.need_queue = 0; c.qsrc = 0;
     c.out_mf = 0; c.mf = deg_src; c.mu = ne - deg_src; c.resid = 0.0; c.edges = 0; c.seq = 0; c.sat = 0;
+    if (c.lb) {  // the source's edges, already scanned (k_seed writes the scan's tables)
+        c.lb_total = deg_src;
+        c.lb_chunks = (deg_src + lb_epb(deg_src) - 1) / lb_epb(deg_src);
+    }
     return c;
 }
 

@@ -39,11 +39,20 @@ __global__ void k_fill_f32(float *p, long long n, float v) {
 }
 
 // seed a run: state[src] = 0, queue = {src}, control block at level 1
-__global__ void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int deg_src, long long ne, int out_begin) {
+// A source of more than kSmallMF edges is pushed by the edge-balanced kernels; its
+// degree scan is trivial (one entry at offset 0, every chunk owned by it), so the seed
+// writes the scan's tables and the first level skips the scan kernels.
+__global__ __launch_bounds__(256) void k_seed(Ctl *ctl, int *state, int *q, double *sigma, int src, int deg_src, long long ne,
+                                              int out_begin, int *eoff, int *tsum, int *cstart) {
+    const Ctl c = seed_ctl(deg_src, ne, out_begin);
+    if (c.lb)
+        for (int k = threadIdx.x; k <= c.lb_chunks; k += blockDim.x) cstart[k] = 0;
+    if (threadIdx.x != 0) return;
     state[src] = 0;
     if (sigma) sigma[src] = 1.0;
     q[0] = src;
-    *ctl = seed_ctl(deg_src, ne, out_begin);
+    if (c.lb) { eoff[0] = 0; tsum[0] = 0; }
+    *ctl = c;
 }
 
 // mirror: the host's pinned copy of the control block (eager levels).  It gets the new
@@ -1293,7 +1302,8 @@ int mg_frontier_reset(mg_engine *e, int source) {
     }
     const int out_begin = (e->P.log_queue) ? 1 : nv;
     if (nv > 0)
-        hipLaunchKernelGGL(k_seed, dim3(1), dim3(1), 0, s, e->ctl, e->level, e->q, e->sigma, source, deg_src, e->ne, out_begin);
+        hipLaunchKernelGGL(k_seed, dim3(1), dim3(256), 0, s, e->ctl, e->level, e->q, e->sigma, source, deg_src, e->ne, out_begin, e->eoff,
+                           e->tsum, e->cstart);
     MG_HIP(hipGetLastError());
     e->source = source;
     e->iter = 1;
@@ -1346,11 +1356,13 @@ static int launch_level(mg_engine *e, bool known) {
         if (!known || c.lb) {
             const long long maxch = known ? (c.mf + lb_epb(c.mf) - 1) / lb_epb(c.mf) : lb_max_chunks(e->ne);
             const int gd = known ? grid_for(c.in_n, kTile, G) : G;
-            if (e->prof) mg_prof_begin(e, "k_lb_scan", 12.0 * c.in_n, &slot);
-            MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_lb_deg<OT>), dim3(gd), dim3(kBlock), 0, s, e->ctl, rowp, e->q, e->eoff, e->tsum));
-            hipLaunchKernelGGL(k_lb_tscan, dim3(1), dim3(1024), 0, s, e->ctl, e->tsum);
-            hipLaunchKernelGGL(k_lb_part, dim3(grid_for(maxch + 1, 256, G)), dim3(256), 0, s, e->ctl, e->eoff, e->tsum, e->cstart);
-            if (e->prof) mg_prof_end(e, slot);
+            if (!(known && c.iter == 1)) {  // the source level's scan tables come from k_seed
+                if (e->prof) mg_prof_begin(e, "k_lb_scan", 12.0 * c.in_n, &slot);
+                MG_WITH_OFFSETS(e, hipLaunchKernelGGL((k_lb_deg<OT>), dim3(gd), dim3(kBlock), 0, s, e->ctl, rowp, e->q, e->eoff, e->tsum));
+                hipLaunchKernelGGL(k_lb_tscan, dim3(1), dim3(1024), 0, s, e->ctl, e->tsum);
+                hipLaunchKernelGGL(k_lb_part, dim3(grid_for(maxch + 1, 256, G)), dim3(256), 0, s, e->ctl, e->eoff, e->tsum, e->cstart);
+                if (e->prof) mg_prof_end(e, slot);
+            }
             const int gp = known ? grid_for(maxch, 1, G) : G;
             if (e->prof) mg_prof_begin(e, "k_push_lb", bytes, &slot);
             if (e->wide) {  // BFS / SSSP only
@@ -1684,7 +1696,8 @@ extern "C" int mg_bfs_noalloc(int nv, int64_t ne, const int32_t *row, const int3
     int rr[2] = {0, 0};
     MG_HIP(hipMemcpyAsync(rr, row + source, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));
-    hipLaunchKernelGGL(k_seed, dim3(1), dim3(1), 0, s, e->ctl, level, e->q, (double *)nullptr, source, rr[1] - rr[0], ne, 1);
+    hipLaunchKernelGGL(k_seed, dim3(1), dim3(256), 0, s, e->ctl, level, e->q, (double *)nullptr, source, rr[1] - rr[0], ne, 1, e->eoff,
+                       e->tsum, e->cstart);
     MG_HIP(hipGetLastError());
     MG_HIP(hipMemcpyAsync(&hc, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));
