@@ -753,7 +753,10 @@ __device__ __forceinline__ void queue_add_n(const bool *add, const int *w, const
 }
 
 constexpr int kSparseLevels = 32;  // levels per replay of the sparse-only graph (at least)
-constexpr int kVisit = 4;  // edges a lane of a sparse BFS level visits together
+#ifndef MG_VISIT
+#define MG_VISIT 4
+#endif
+constexpr int kVisit = MG_VISIT;  // edges a lane of a sparse level visits together (uneven batch, 2 / 4 / 8: BFS 53.7 / 51.9 / 51.6 ms, SSSP at band 14 133.8 / 129.5 / 135.2 ms)
 
 template <bool kDup = false>
 __device__ __forceinline__ void sparse_bfs(int it, int n, const int *__restrict__ row, const int *__restrict__ col, uint64_t *F,
