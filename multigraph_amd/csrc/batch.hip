@@ -274,6 +274,11 @@ __global__ __launch_bounds__(kBB) void k_mb_push(const BCtl *ctl, long long nch,
 
 // pull, BFS: a wave per chunk of a vertex still missing bits; lanes or the frontier
 // words of 64 neighbours at a time and stop once the missing bits are covered
+#ifndef MG_HEAVY_STEP
+#define MG_HEAVY_STEP 1
+#endif
+constexpr int kHeavyStep = MG_HEAVY_STEP;  // rounds of 64 in-edges the heavy BFS pull loads together (1 / 2 / 4: 1.90 / 1.97 / 2.05 ms per 64-source run: the early exit wastes the extra rounds)
+
 __global__ __launch_bounds__(kBB) void k_mb_pull_bfs(const BCtl *ctl, long long nch, const Chunk *__restrict__ ch, uint64_t all,
                                                      const int *__restrict__ col, uint64_t *FA,
                                                      const uint64_t *__restrict__ vis, uint64_t *FB) {
@@ -283,9 +288,16 @@ __global__ __launch_bounds__(kBB) void k_mb_pull_bfs(const BCtl *ctl, long long 
     if (ctl) all = ctl->all;
     for_active_chunks(nch, ch, [&](const Chunk &k) { return all & ~vis[k.v]; }, [&](const Chunk &k, const uint64_t need) {
         uint64_t acc = 0;
-        for (int p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
-            const int p = p0 + lane();
-            const uint64_t f = p < k.p1 ? (F[ld_stream(&col[p])] & need) : 0ull;
+        for (int p0 = k.p0; p0 < k.p1; p0 += kHeavyStep * MG_WAVE) {  // kHeavyStep rounds of loads in flight
+            int u[kHeavyStep];
+#pragma unroll
+            for (int j = 0; j < kHeavyStep; ++j) {
+                const int p = p0 + j * MG_WAVE + lane();
+                u[j] = p < k.p1 ? ld_stream(&col[p]) : -1;
+            }
+            uint64_t f = 0ull;
+#pragma unroll
+            for (int j = 0; j < kHeavyStep; ++j) f |= u[j] >= 0 ? F[u[j]] & need : 0ull;
             acc |= wave_or(f);
             if (acc == need) break;
         }
