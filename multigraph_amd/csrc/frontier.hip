@@ -339,6 +339,42 @@ __global__ __launch_bounds__(kBlock) void k_level2bm(Ctl *ctl, int nv, const int
 
 __device__ __forceinline__ bool bm_test(const uint64_t *bm, int u) { return (bm[u >> 6] >> (u & 63)) & 1ull; }
 
+// Unvisited vertices above pull_split edges, cut into static kPullChunk-edge chunks
+// (PullChunk {v, p0, p1, k}, k = the vertex's slot): a wave per chunk scans it and
+// merges its accumulator into the vertex's slot (hlev, hsum) with accumulate (FP64
+// path-count sums: exact in any order); k_pull_huge_fin then applies the slots to
+// level, sigma, bitmaps and counts and resets them to the identity.  Without this
+// split, one lane walks a 160 K-edge hub alone (BC scans every parent).  The chunks are
+// walked by the last workgroups of the k_pull launch (waves wave, wave + nwaves, ...).
+template <class H, class OT>
+__device__ __forceinline__ void pull_huge_chunks(int iter, const uint64_t *fr, int nch, const PullChunkT<OT> *__restrict__ ch,
+                                                 const int *__restrict__ col, const double *__restrict__ sigma,
+                                                 const uint64_t *vis, int *__restrict__ hlev, double *__restrict__ hsum,
+                                                 int wave, int nwaves) {
+    for (int c = wave; c < nch; c += nwaves) {
+        const PullChunkT<OT> k = ch[c];
+        if (bm_test(vis, k.v)) continue;
+        Acc<H> a;
+        acc_init(a);
+        bool found = false;
+        for (OT p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
+            const OT p = p0 + lane();
+            bool f = false;
+            if (p < k.p1) {
+                const int u = ld_stream(&col[p]);
+                f = bm_test(fr, u);
+                if (f) acc_edge<H>(a, iter - 1, H::kV2 ? sigma[u] : 0.0, 0, iter);
+            }
+            const bool any = ballot(f) != 0;
+            found = found || any;
+            if (H::kFirstWins && any) break;
+        }
+        if (!found) continue;
+        a = acc_wave<H>(a, iter);
+        if (lane() == 0) acc_atomic<H>(&hlev[k.k], H::kV2 ? &hsum[k.k] : nullptr, a, iter);
+    }
+}
+
 // Pull level for BFS and the BC forward pass: every unvisited vertex accumulates the
 // messages of its parents in the current frontier (a parent u of the frontier has
 // level iter - 1; get_partial_result gives its message, accumulate_nonAtomics combines
@@ -362,16 +398,27 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
                                                  int *__restrict__ level,
                                                  double *__restrict__ sigma, uint64_t *__restrict__ vis,
                                                  uint64_t *__restrict__ bm0, uint64_t *__restrict__ bm1,
-                                                 int *__restrict__ part_n, long long *__restrict__ part_mf) {
+                                                 int *__restrict__ part_n, long long *__restrict__ part_mf,
+                                                 int hblocks, int hnch, const PullChunkT<OT> *__restrict__ hch,
+                                                 int *__restrict__ hlev, double *__restrict__ hsum) {
     if (ctl->mode != MODE_BU) return;
     constexpr bool kSum = !H::kFirstWins;  // every parent contributes
     const int iter = ctl->iter;
     const int plev = iter - 1;  // level of the frontier's vertices
     const uint64_t *fr = ctl->bm_cur ? bm1 : bm0;
     uint64_t *nx = ctl->bm_cur ? bm0 : bm1;
+    // the last hblocks workgroups take the chunks of the rows above `cut` edges (in the
+    // same launch: those rows are never walked by the others, and the words the others
+    // write keep those vertices' bits)
+    const int mblocks = gridDim.x - hblocks;
+    if ((int)blockIdx.x >= mblocks) {
+        pull_huge_chunks<H, OT>(iter, fr, hnch, hch, col, sigma, vis, hlev, hsum, ((blockIdx.x - mblocks) * B + threadIdx.x) >> 6,
+                                (hblocks * B) >> 6);
+        return;
+    }
     const int words = (nv + 63) >> 6;
     const int wave = (blockIdx.x * B + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * B) >> 6;
+    const int nwaves = (mblocks * B) >> 6;
 #if MG_PULL_FLAT
     // per wave: the flat walk's row prefixes and starts, and the rows' accumulators
     __shared__ int s_incl[kSum ? B : 1];
@@ -393,7 +440,7 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
             b = row[v];
             e = row[v + 1];
             deg = (int)(e - b);
-            if (e - b > cut) e = b;  // left to k_pull_huge
+            if (e - b > cut) e = b;  // left to the chunk workgroups
             if (!kSum) {
                 for (OT p = b; p < e; ++p)
                     if (bm_test(fr, col[p])) { acc_edge<H>(a, plev, 0.0, 0, iter); break; }
@@ -401,7 +448,7 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
         }
         if (kSum) {
 #if MG_PULL_FLAT
-            // The word's rows (those of at most `cut` edges; longer ones are k_pull_huge's)
+            // The word's rows (those of at most `cut` edges; longer ones are chunked)
             // walked as one flat edge range: step by step, lane l takes kPullStep
             // consecutive edges of it.  Every step is full whatever the row lengths, and
             // the rows of consecutive vertices are consecutive in col, so a step reads one
@@ -543,47 +590,7 @@ __global__ __launch_bounds__(B) void k_pull(Ctl *ctl, int nv, int cut, int lane_
     if (threadIdx.x == 0) { part_n[blockIdx.x] = (int)tn; part_mf[blockIdx.x] = tm; }
 }
 
-// Unvisited vertices above pull_split edges, cut into static kPullChunk-edge chunks
-// (PullChunk {v, p0, p1, k}, k = the vertex's slot): a wave per chunk scans it and
-// merges its accumulator into the vertex's slot (hlev, hsum) with accumulate (FP64
-// path-count sums: exact in any order); k_pull_huge_fin then applies the slots to
-// level, sigma, bitmaps and counts and resets them to the identity.  Without this
-// split, one lane walks a 160 K-edge hub alone (BC scans every parent).
-template <class H, class OT = int>
-__global__ __launch_bounds__(kBlock) void k_pull_huge(Ctl *ctl, int nch, const PullChunkT<OT> *__restrict__ ch,
-                                                      const int *__restrict__ col, const double *__restrict__ sigma,
-                                                      const uint64_t *__restrict__ vis, const uint64_t *__restrict__ bm0,
-                                                      const uint64_t *__restrict__ bm1, int *__restrict__ hlev,
-                                                      double *__restrict__ hsum) {
-    if (ctl->mode != MODE_BU) return;
-    const int iter = ctl->iter;
-    const uint64_t *fr = ctl->bm_cur ? bm1 : bm0;
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * kBlock) >> 6;
-    for (int c = wave; c < nch; c += nwaves) {
-        const PullChunkT<OT> k = ch[c];
-        if (bm_test(vis, k.v)) continue;
-        Acc<H> a;
-        acc_init(a);
-        bool found = false;
-        for (OT p0 = k.p0; p0 < k.p1; p0 += MG_WAVE) {
-            const OT p = p0 + lane();
-            bool f = false;
-            if (p < k.p1) {
-                const int u = ld_stream(&col[p]);
-                f = bm_test(fr, u);
-                if (f) acc_edge<H>(a, iter - 1, H::kV2 ? sigma[u] : 0.0, 0, iter);
-            }
-            const bool any = ballot(f) != 0;
-            found = found || any;
-            if (H::kFirstWins && any) break;
-        }
-        if (!found) continue;
-        a = acc_wave<H>(a, iter);
-        if (lane() == 0) acc_atomic<H>(&hlev[k.k], H::kV2 ? &hsum[k.k] : nullptr, a, iter);
-    }
-}
-
+// the chunked rows' slots applied (after the k_pull launch whose chunk workgroups filled them)
 template <class H, class OT = int>
 __global__ __launch_bounds__(kBlock) void k_pull_huge_fin(Ctl *ctl, int nh, const int *__restrict__ hv,
                                                           const OT *__restrict__ row, int *__restrict__ level,
@@ -1410,36 +1417,35 @@ static int launch_level(mg_engine *e, bool known) {
         const double bytes = known ? (8.0 * nv / 64.0 * 3 + 4.0 * (double)c.mu + 8.0 * nv) : 0.0;
         if (e->prof) mg_prof_begin(e, "k_pull", bytes, &slot);
         const int *orow = e->trow ? e->row : nullptr;
-        // rows above pull_split edges go to k_pull_huge's chunks; an engine without
+        // rows above pull_split edges go to the chunk workgroups; an engine without
         // chunks (none was that long, or no table was built) walks every row here
         const int cut = e->pull_ch ? e->opt.pull_split : 0x7FFFFFFF;
         const int lanes = e->opt.lane_edges;
-        int gpl = gsc;  // k_pull's workgroups = its partials
+        int gpl = gsc;  // k_pull's row workgroups = its partials; the chunk workgroups follow
+        const int hn = e->pull_nch;
         if (e->wide) {  // BFS only: a wide graph is symmetric (in-edge rows = out-edge rows)
-            hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kBlock, long long>), dim3(gpl), dim3(kBlock), 0, s, e->ctl, nv, cut, lanes, e->row64,
-                               e->col, (const long long *)nullptr, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+            const int hb = hn > 0 ? grid_for(hn, kBlock / MG_WAVE, G) : 0;
+            hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kBlock, long long>), dim3(gpl + hb), dim3(kBlock), 0, s, e->ctl, nv, cut, lanes,
+                               e->row64, e->col, (const long long *)nullptr, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n,
+                               e->part_mf, hb, hn, (const PullChunkT<long long> *)e->pull_ch, e->pull_found, e->pull_sum);
         } else {
+            const PullChunk *pch = (const PullChunk *)e->pull_ch;
             MG_WITH_BLOCK(e, gpl = grid_for(words, kB / MG_WAVE, G);
+                const int hb = hn > 0 ? grid_for(hn, kB / MG_WAVE, G) : 0;
                 if (e->algo == MG_BC)
-                    hipLaunchKernelGGL((k_pull<MG_BC_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf);
+                    hipLaunchKernelGGL((k_pull<MG_BC_HOOKS, kB>), dim3(gpl + hb), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf, hb, hn, pch, e->pull_found, e->pull_sum);
                 else
-                    hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kB>), dim3(gpl), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf));
+                    hipLaunchKernelGGL((k_pull<MG_BFS_HOOKS, kB>), dim3(gpl + hb), dim3(kB), 0, s, e->ctl, nv, cut, lanes, e->in_row(), e->in_col(), orow, e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->part_n, e->part_mf, hb, hn, pch, e->pull_found, e->pull_sum));
         }
-        if (e->pull_nch > 0) {
-            const int gh = grid_for(e->pull_nch, kBlock / MG_WAVE, G);
+        if (hn > 0) {  // the chunked rows' slots, applied once every chunk is in
             const int gf = grid_for(e->pull_nh, kBlock, 64);
-            MG_WITH_OFFSETS(e, const PullChunkT<OT> *pch = (const PullChunkT<OT> *)e->pull_ch;
-                if (e->algo == MG_BC) {
-                    hipLaunchKernelGGL((k_pull_huge<MG_BC_HOOKS, OT>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, pch, e->in_col(), e->sigma,
-                                       e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
+            MG_WITH_OFFSETS(e,
+                if (e->algo == MG_BC)
                     hipLaunchKernelGGL((k_pull_huge_fin<MG_BC_HOOKS, OT>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, rowp,
                                        e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
-                } else {
-                    hipLaunchKernelGGL((k_pull_huge<MG_BFS_HOOKS, OT>), dim3(gh), dim3(kBlock), 0, s, e->ctl, e->pull_nch, pch, e->in_col(), e->sigma,
-                                       e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum);
+                else
                     hipLaunchKernelGGL((k_pull_huge_fin<MG_BFS_HOOKS, OT>), dim3(gf), dim3(kBlock), 0, s, e->ctl, e->pull_nh, e->pull_hv, rowp,
-                                       e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf);
-                });
+                                       e->level, e->sigma, e->vis, e->bm[0], e->bm[1], e->pull_found, e->pull_sum, e->part_n, e->part_mf));
         }
         if (e->prof) mg_prof_end(e, slot);
         red_parts = gpl;

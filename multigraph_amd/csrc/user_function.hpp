@@ -13,7 +13,7 @@
 // structs and call the hooks at the reference's points, so editing a hook changes
 // every path that runs the algorithm:
 //   push levels (k_push, k_push_lb)        get_partial_result + fused_update_condition
-//   pull levels (k_pull, k_pull_huge,      initialize (register / slot identity),
+//   pull levels (k_pull + its chunks,      initialize (register / slot identity),
 //     k_pull_sssp, k_ps_gather,            get_partial_result, accumulate_nonAtomics
 //     k_pb_gather, k_pr_gather)            (register), accumulate (LDS / global slot),
 //                                          update_condition (apply), fused_update_
