@@ -1719,15 +1719,24 @@ __global__ __launch_bounds__(kBB) void k_mb_back(long long nch, const Chunk *__r
 
 // bc[v] += sum over slots s of delta[v][s]; the slots v is a source of (level 0) hold
 // zero, as the backward pass stops at level 1
-__global__ void k_ms_bc_sum(int nv, int nslots, const float *__restrict__ delta, float *__restrict__ bc) {
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int v = wave; v < nv; v += nw) {
-        KSum k;
-        const long long kv = (long long)v * 64 + lane();
-        if (lane() < nslots) k.add(delta[kv]);
-        k = wave_ksum(k);
-        if (lane() == 0) bc[v] += k.value();
+__global__ __launch_bounds__(256) void k_ms_bc_sum(int nv, int nslots, const float *__restrict__ delta, float *__restrict__ bc) {
+    // a wave takes 64 vertices: their rows pass through an LDS tile (one coalesced row
+    // load each), then lane l sums row l alone, with no cross-lane reduction per vertex
+    // (one vertex a wave with a wave-wide compensated reduction: 0.39 ms at RMAT-22)
+    __shared__ float s_t[256 / MG_WAVE][MG_WAVE * (MG_WAVE + 1)];
+    float *t = s_t[threadIdx.x / MG_WAVE];
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long v0 = wave * MG_WAVE; v0 < nv; v0 += nw * MG_WAVE) {
+        const int nl = (int)min((long long)MG_WAVE, nv - v0);
+        wave_sync();  // the previous tile is consumed
+        for (int l = 0; l < nl; ++l) t[l * (MG_WAVE + 1) + lane()] = delta[(v0 + l) * 64 + lane()];
+        wave_sync();
+        if (lane() < nl) {
+            KSum k;
+            for (int q = 0; q < nslots; ++q) k.add(t[lane() * (MG_WAVE + 1) + q]);
+            bc[v0 + lane()] += k.value();
+        }
     }
 }
 
@@ -2219,7 +2228,7 @@ static int run_bfs_bc(mg_batch *b) {
         if (n1 > 0) hipLaunchKernelGGL(k_ms_scatter_words, dim3(grid_for(n1, 256, 4096)), dim3(256), 0, s, n1, b->logv + off[l + 1], b->logw + off[l + 1], D, 1);
         std::swap(E, D);
     }
-    hipLaunchKernelGGL(k_ms_bc_sum, dim3(grid_for(nv, 4, 8192)), dim3(256), 0, s, nv, b->nsrc, b->delta, b->bc);
+    hipLaunchKernelGGL(k_ms_bc_sum, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, nv, b->nsrc, b->delta, b->bc);
     MG_HIP(hipGetLastError());
     return MG_OK;
 }
